@@ -1,0 +1,154 @@
+"""Headline benchmark: closest-hit Mrays/s on the 871,200-triangle mesh.
+
+Workload (one step): one closest-hit launch over 16M rays, origins uniform on
+the mesh AABB enlarged 1.5x about its centre, directions uniform on the unit
+sphere, against the UV sphere of 660 x 660 bands radially displaced by 6-octave
+fBm (871,200 fp32 triangles).  Rays, BVH and outputs are resident in HBM before
+the timed region.  Beside it: LBVH build ms (2^20-triangle soup and the mesh)
+and the CPU oracle on a bounded sample of the same rays.
+
+Prints one JSON line.  Options: --gpus N (only 1 is supported), --steps K,
+--warmup W, --rays R, --no-cpu (skip the CPU baseline), --profile (short run
+for rocprofv3: 2 warmup + 3 steps, no CPU baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-path-tracer_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=16 * 1024 * 1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: only --gpus 1 is supported (this build runs on one GPU; got {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.profile:
+        args.steps, args.warmup, args.no_cpu = 3, 2, True
+
+    import numpy as np
+
+    import ptamd
+    from ptamd import hip
+
+    seed_mesh, seed_rays = 2024, 77
+    t0 = time.time()
+    tris, infos = ptamd.gen_sphere(660, seed_mesh, room=False)
+    bmin, bmax = ptamd.scene_bounds(tris)
+    rays = ptamd.gen_shell_rays(args.rays, seed_rays, bmin, bmax, 1.5)
+    log(f"[bench] generated {len(tris)} triangles, {len(rays)} rays in {time.time() - t0:.1f}s")
+
+    ctx = ptamd.Context(0)
+    stream = hip.Stream()
+    ctx.set_stream(stream.handle)
+    ctx.upload_scene(tris, infos)
+    d_rays = hip.DeviceArray.from_host(rays)
+    n = len(rays)
+    d_id = hip.DeviceArray(n, np.uint32)
+    d_t = hip.DeviceArray(n, np.float32)
+
+    # Visit counts of the reference-order traversal on exactly these rays
+    # (equal to the oracle's, which the parity tests check) -> algorithmic bytes.
+    cnt = ctx.trace_closest_counted(d_rays.ptr, n, stream=stream.handle)
+    iters, boxes, tri_tests = (int(x) for x in cnt)
+    bytes_per_launch = 24 * n + 8 * n + 4 * iters + 24 * boxes + 36 * tri_tests
+    log(f"[bench] per ray: {iters / n:.2f} node visits, {boxes / n:.2f} box tests, "
+        f"{tri_tests / n:.2f} triangle tests -> {bytes_per_launch / n:.0f} B/ray")
+
+    for _ in range(args.warmup):
+        ctx.trace_closest_device(d_rays.ptr, n, d_id.ptr, d_t.ptr, 0, stream.handle)
+    stream.synchronize()
+    hip.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    w0 = time.time()
+    e0.record(stream)
+    for _ in range(args.steps):
+        ctx.trace_closest_device(d_rays.ptr, n, d_id.ptr, d_t.ptr, 0, stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    hip.synchronize()
+    wall = time.time() - w0
+    ms = e0.elapsed_ms(e1)
+    ms_per_step = ms / args.steps
+    mrays = n / (ms_per_step * 1e-3) / 1e6
+    achieved = bytes_per_launch / (ms_per_step * 1e-3) / 1e9
+    hits = d_id.to_host()
+    hit_rate = float((hits != ptamd.INVALID).mean())
+    log(f"[bench] {mrays:.1f} Mrays/s ({ms_per_step:.3f} ms/launch, wall {wall:.2f}s), hit rate {hit_rate:.3f}")
+
+    # LBVH build time (HIP events around the build kernels).
+    build_ms_mesh = float(np.median([ctx.build_bvh() for _ in range(5)]))
+    soup, soup_infos = ptamd.gen_soup(1 << 20, 1, emitter=False)
+    ctx.upload_scene(soup, soup_infos)
+    build_ms_1m = float(np.median([ctx.build_bvh() for _ in range(10)]))
+    log(f"[bench] LBVH build: {build_ms_1m:.3f} ms @ 2^20 soup, {build_ms_mesh:.3f} ms @ 871,200 mesh")
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_py as orc
+
+        o = orc.build(tris, bmin, bmax)
+        threads = orc.THREADS
+        probe = rays[: 1 << 16]
+        t1 = time.time()
+        orc.trace(tris, o, probe, threads=threads)
+        rate = len(probe) / max(time.time() - t1, 1e-6)
+        m = int(min(n, max(len(probe), rate * args.cpu_seconds)))
+        sample = rays[:m]
+        t1 = time.time()
+        orc.trace(tris, o, sample, threads=threads)
+        dt = time.time() - t1
+        cpu = {"value": round(m / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} of the same {n} rays, oracle reference-order traversal, "
+                         f"OpenMP {threads} threads"}
+        log(f"[bench] CPU oracle: {cpu['value']} Mrays/s on {threads} threads ({m} rays, {dt:.1f}s)")
+
+    out = {
+        "metric": "closest-hit Mrays/s @871K tris",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "closest-hit microbench: 16M shell rays (AABB x1.5, uniform S2 directions) "
+                               "vs 660x660-band fBm UV sphere, 871,200 fp32 triangles",
+                   "rays": n, "triangles": int(len(tris)), "seed_mesh": seed_mesh, "seed_rays": seed_rays},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
+        "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),
+                      "tri_tests_per_ray": round(tri_tests / n, 3),
+                      "algorithmic_bytes_per_ray": round(bytes_per_launch / n, 1), "hit_rate": round(hit_rate, 4)},
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,336 @@
+// C-ABI entry points of libptamd (declared in include/pt_api.h).  Each entry
+// converts library exceptions into a status code plus a message retrievable
+// with pt_last_error; nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+#include "pt_api.h"
+#include "pt_scenes.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(pt_context* ctx, F&& f) {
+    try {
+        if (ctx) PT_HIP(hipSetDevice(ctx->device));
+        f();
+        return PT_OK;
+    } catch (const pt::Error& e) {
+        if (ctx) ctx->last_error = e.what();
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (ctx) ctx->last_error = e.what();
+        g_last_error = e.what();
+        return PT_E_INTERNAL;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unknown error";
+        g_last_error = "unknown error";
+        return PT_E_INTERNAL;
+    }
+}
+
+void require(bool cond, const char* msg) {
+    if (!cond) throw pt::Error(PT_E_INVALID_ARG, msg);
+}
+
+hipStream_t pick(pt_context* ctx, void* stream) {
+    return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_create(pt_context** out, int device) {
+    if (!out) return PT_E_INVALID_ARG;
+    *out = nullptr;
+    pt_context* ctx = nullptr;
+    int rc = guarded(nullptr, [&] {
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count <= 0) throw pt::Error(PT_E_NO_DEVICE, "no HIP device visible");
+        if (device < 0 || device >= count) throw pt::Error(PT_E_NO_DEVICE, "device index out of range");
+        PT_HIP(hipSetDevice(device));
+        ctx = new pt_context();
+        ctx->device = device;
+        PT_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+        ctx->stream = ctx->own_stream;
+        PT_HIP(hipEventCreate(&ctx->ev0));
+        PT_HIP(hipEventCreate(&ctx->ev1));
+    });
+    if (rc != PT_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return PT_OK;
+}
+
+int pt_destroy(pt_context* ctx) {
+    if (!ctx) return PT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    delete ctx;  // releases device buffers
+    return PT_OK;
+}
+
+int pt_last_error(const pt_context* ctx, char* buf, size_t len) {
+    if (!buf || len == 0) return PT_E_INVALID_ARG;
+    const std::string& s = ctx ? ctx->last_error : g_last_error;
+    size_t k = std::min(len - 1, s.size());
+    std::memcpy(buf, s.data(), k);
+    buf[k] = '\0';
+    return PT_OK;
+}
+
+int pt_set_stream(pt_context* ctx, void* stream) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    });
+}
+
+int pt_synchronize(pt_context* ctx) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { PT_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int pt_upload_scene(pt_context* ctx, const pt_triangle* tris, uint32_t n,
+                    const pt_triangle_info* infos, uint32_t ninfos, const pt_dir_light* lights,
+                    uint32_t nlights, const float* bmin, const float* bmax,
+                    const pt_settings* settings) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        require(n == 0 || tris != nullptr, "tris is NULL");
+        require(ninfos == 0 || infos != nullptr, "infos is NULL");
+        require(nlights == 0 || lights != nullptr, "lights is NULL");
+        require((bmin == nullptr) == (bmax == nullptr), "bmin and bmax must both be given or both NULL");
+        require(n < (1u << 30), "at most 2^30 - 1 triangles");
+        uint32_t max_id = 0;
+        for (uint32_t i = 0; i < n; ++i) max_id = std::max(max_id, tris[i].id);
+        require(n == 0 || (infos != nullptr && ninfos > max_id),
+                "infos must hold a record for every triangle id");
+        pt::DeviceScene& s = ctx->scene;
+        hipStream_t st = ctx->stream;
+        s.ready = false;
+        s.n = n;
+        s.ninfos = ninfos;
+        // Scene::upload (Scene.cu:28-55): deep copy, then the BVH build.
+        if (bmin) {
+            for (int k = 0; k < 3; ++k) {
+                s.bmin[k] = bmin[k];
+                s.bmax[k] = bmax[k];
+            }
+        } else {
+            pt_scene_bounds(tris, n, s.bmin, s.bmax);
+        }
+        if (n) {
+            s.tris.reserve(sizeof(pt_triangle) * static_cast<size_t>(n));
+            PT_HIP(hipMemcpyAsync(s.tris.as<void>(), tris, sizeof(pt_triangle) * static_cast<size_t>(n),
+                                  hipMemcpyHostToDevice, st));
+        }
+        if (ninfos) {
+            s.infos.reserve(sizeof(pt_triangle_info) * static_cast<size_t>(ninfos));
+            PT_HIP(hipMemcpyAsync(s.infos.as<void>(), infos,
+                                  sizeof(pt_triangle_info) * static_cast<size_t>(ninfos),
+                                  hipMemcpyHostToDevice, st));
+        }
+        if (settings) ctx->renderer.settings = *settings;
+        ctx->renderer.lights = nlights;
+        pt::build_lbvh(s, st);
+        PT_HIP(hipStreamSynchronize(st));
+        s.ready = true;
+        // CUDAPathTracer::upload resets the frame counter (cuda_path_tracer.cu:469).
+        pt::render_reset(ctx->renderer, st);
+        PT_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int pt_set_settings(pt_context* ctx, const pt_settings* settings) {
+    if (!ctx || !settings) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { ctx->renderer.settings = *settings; });
+}
+
+int pt_build_bvh(pt_context* ctx, float* ms_out) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        hipStream_t st = ctx->stream;
+        PT_HIP(hipEventRecord(ctx->ev0, st));
+        pt::build_lbvh(ctx->scene, st);
+        PT_HIP(hipEventRecord(ctx->ev1, st));
+        if (ms_out) {
+            PT_HIP(hipEventSynchronize(ctx->ev1));
+            PT_HIP(hipEventElapsedTime(ms_out, ctx->ev0, ctx->ev1));
+        }
+    });
+}
+
+int pt_get_bvh(pt_context* ctx, uint32_t* morton, uint32_t* perm, uint32_t* children,
+               uint32_t* parents_leaf, uint32_t* parents_internal, float* bboxes) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        pt::dump_bvh(ctx->scene, ctx->stream, morton, perm, children, parents_leaf,
+                     parents_internal, bboxes);
+    });
+}
+
+int pt_get_scene_info(pt_context* ctx, uint32_t* ntris, float* bmin, float* bmax) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (ntris) *ntris = ctx->scene.n;
+        for (int k = 0; k < 3; ++k) {
+            if (bmin) bmin[k] = ctx->scene.bmin[k];
+            if (bmax) bmax[k] = ctx->scene.bmax[k];
+        }
+    });
+}
+
+int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n, uint32_t* d_id,
+                            float* d_t, float* d_uv, void* stream) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        require(n == 0 || d_rays != nullptr, "rays is NULL");
+        pt::trace_closest(ctx->scene, d_rays, n, d_id, d_t, d_uv, pick(ctx, stream));
+    });
+}
+
+int pt_trace_closest(pt_context* ctx, const float* rays, uint32_t n, uint32_t* id, float* t,
+                     float* uv) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        require(n == 0 || rays != nullptr, "rays is NULL");
+        if (n == 0) return;
+        hipStream_t st = ctx->stream;
+        const size_t nn = n;
+        ctx->io_rays.reserve(nn * 24);
+        ctx->io_id.reserve(nn * 4);
+        ctx->io_t.reserve(nn * 4);
+        ctx->io_uv.reserve(nn * 8);
+        PT_HIP(hipMemcpyAsync(ctx->io_rays.as<void>(), rays, nn * 24, hipMemcpyHostToDevice, st));
+        pt::trace_closest(ctx->scene, ctx->io_rays.as<float>(), n, id ? ctx->io_id.as<uint32_t>() : nullptr,
+                          t ? ctx->io_t.as<float>() : nullptr, uv ? ctx->io_uv.as<float>() : nullptr, st);
+        if (id) PT_HIP(hipMemcpyAsync(id, ctx->io_id.as<void>(), nn * 4, hipMemcpyDeviceToHost, st));
+        if (t) PT_HIP(hipMemcpyAsync(t, ctx->io_t.as<void>(), nn * 4, hipMemcpyDeviceToHost, st));
+        if (uv) PT_HIP(hipMemcpyAsync(uv, ctx->io_uv.as<void>(), nn * 8, hipMemcpyDeviceToHost, st));
+        PT_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int pt_trace_closest_counted(pt_context* ctx, const float* d_rays, uint32_t n, uint32_t* d_id,
+                             float* d_t, float* d_uv, uint64_t* counters, void* stream) {
+    if (!ctx || !counters) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        require(n == 0 || d_rays != nullptr, "rays is NULL");
+        hipStream_t st = pick(ctx, stream);
+        pt::DevBuf c;
+        c.reserve(3 * sizeof(unsigned long long));
+        PT_HIP(hipMemsetAsync(c.as<void>(), 0, 3 * sizeof(unsigned long long), st));
+        pt::trace_closest_counted(ctx->scene, d_rays, n, d_id, d_t, d_uv,
+                                  c.as<unsigned long long>(), st);
+        PT_HIP(hipMemcpyAsync(counters, c.as<void>(), 3 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, st));
+        PT_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int pt_set_primary_rays(pt_context* ctx, const float* dirs, const float* eye) {
+    if (!ctx || !dirs || !eye) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        std::memcpy(ctx->renderer.dirs, dirs, sizeof ctx->renderer.dirs);
+        std::memcpy(ctx->renderer.eye, eye, sizeof ctx->renderer.eye);
+    });
+}
+
+int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        if (ctx->renderer.lights != 0)
+            throw pt::Error(PT_E_UNSUPPORTED, "directional lights are not supported by the renderer");
+        hipStream_t st = ctx->stream;
+        pt::render_resize(ctx->renderer, w, h, st);
+        PT_HIP(hipEventRecord(ctx->ev0, st));
+        pt::render_pass(ctx->renderer, ctx->scene, pass_seed, st);
+        PT_HIP(hipEventRecord(ctx->ev1, st));
+        ctx->renderer.timed = true;
+    });
+}
+
+int pt_reset_frames(pt_context* ctx) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { pt::render_reset(ctx->renderer, ctx->stream); });
+}
+
+int pt_read_accumulator(pt_context* ctx, double* rgb_sum, uint32_t* spp) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        pt::Renderer& r = ctx->renderer;
+        if (rgb_sum && r.w > 0) {
+            PT_HIP(hipMemcpyAsync(rgb_sum, r.accum.as<void>(),
+                                  static_cast<size_t>(r.w) * r.h * 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        }
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        if (spp) *spp = r.spp;
+    });
+}
+
+int pt_write_accumulator(pt_context* ctx, int w, int h, const double* rgb_sum, uint32_t spp) {
+    if (!ctx || !rgb_sum) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        pt::Renderer& r = ctx->renderer;
+        pt::render_resize(r, w, h, ctx->stream);
+        PT_HIP(hipMemcpyAsync(r.accum.as<void>(), rgb_sum,
+                              static_cast<size_t>(w) * h * 3 * sizeof(double),
+                              hipMemcpyHostToDevice, ctx->stream));
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        r.spp = spp;
+    });
+}
+
+int pt_read_image(pt_context* ctx, float* rgba) {
+    if (!ctx || !rgba) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        pt::Renderer& r = ctx->renderer;
+        const size_t np = static_cast<size_t>(r.w) * r.h;
+        std::vector<double> sum(np * 3);
+        if (np) {
+            PT_HIP(hipMemcpyAsync(sum.data(), r.accum.as<void>(), np * 3 * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        }
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        const double inv = r.spp ? 1.0 / static_cast<double>(r.spp) : 0.0;
+        for (size_t p = 0; p < np; ++p) {
+            for (int c = 0; c < 3; ++c) rgba[4 * p + c] = static_cast<float>(sum[3 * p + c] * inv);
+            rgba[4 * p + 3] = 1.0f;
+        }
+    });
+}
+
+int pt_last_render_ms(pt_context* ctx, float* ms) {
+    if (!ctx || !ms) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->renderer.timed) throw pt::Error(PT_E_INVALID_ARG, "no pass rendered");
+        PT_HIP(hipEventSynchronize(ctx->ev1));
+        PT_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,165 @@
+// Host-side engine objects behind the C-ABI: device buffers, the device scene
+// (reference: CUDA::DeviceSceneDescription, DeviceSceneDescription.h:13-21),
+// the LBVH builder (CUDA::BVHBuilder, BVHBuilder.h:12-19) and the progressive
+// tracer (CUDAPathTracer, cuda_path_tracer.h:13-40).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pt_api.h"
+
+namespace pt {
+
+// Errors inside the library are C++ exceptions; the C-ABI layer converts them
+// to status codes and never lets one cross the ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PT_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::pt::Error(PT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Grow-only device allocation.
+class DevBuf {
+   public:
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void reserve(size_t bytes) {
+        if (bytes <= cap_) return;
+        release();
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            throw Error(PT_E_OOM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        ptr_ = p;
+        cap_ = bytes;
+    }
+    void release() {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        cap_ = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr_); }
+    size_t capacity() const { return cap_; }
+
+   private:
+    void* ptr_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// Pinned host staging (hipHostMalloc), grow-only.
+class HostBuf {
+   public:
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() {
+        if (ptr_) (void)hipHostFree(ptr_);
+    }
+    void reserve(size_t bytes) {
+        if (bytes <= cap_) return;
+        if (ptr_) (void)hipHostFree(ptr_);
+        ptr_ = nullptr;
+        cap_ = 0;
+        PT_HIP(hipHostMalloc(&ptr_, bytes, hipHostMallocDefault));
+        cap_ = bytes;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr_); }
+
+   private:
+    void* ptr_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// LBVH in device memory.  Two layouts live side by side: the reference's
+// parity arrays (keys, perm, children, parents) and the traversal layout:
+//   nodes[4*i .. 4*i+3]   one 64-byte record per internal node i:
+//       (L.minx, L.maxx, L.miny, L.maxy) (L.minz, L.maxz, R.minx, R.maxx)
+//       (R.miny, R.maxy, R.minz, R.maxz) (children word, 0, 0, 0)
+//     -- both child boxes next to the child word, so one visit is one
+//        contiguous 64-byte read; a leaf child's slot holds its triangle box.
+//   tris[3*k .. 3*k+2]   one 48-byte record per sorted triangle k:
+//       (p0.xyz, original id) (e1.xyz, 0) (e2.xyz, 0), e = p - p0.
+struct DeviceBvh {
+    uint32_t n = 0;
+    DevBuf keys, vals, keys_alt, vals_alt;  // sorted Morton keys, perm (+ ping-pong)
+    DevBuf children, parents_leaf, parents_internal, counters;
+    DevBuf nodes, tris, root_box;
+    DevBuf tile_hist, digit_total;  // radix sort scratch
+};
+
+struct DeviceScene {
+    uint32_t n = 0;
+    uint32_t ninfos = 0;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    DevBuf tris;   // pt_triangle[n] as uploaded
+    DevBuf infos;  // pt_triangle_info[ninfos]
+    DeviceBvh bvh;
+    bool ready = false;
+};
+
+// Builder timings (ms) of the last build, from HIP events.
+struct BuildTimes {
+    float total = 0.0f;
+};
+
+// LBVH build: Morton keys -> 4-pass LDS radix sort -> radix tree -> refit.
+void build_lbvh(DeviceScene& scene, hipStream_t stream);
+// Parity dump helpers (device -> host, synchronous).
+void dump_bvh(DeviceScene& scene, hipStream_t stream, uint32_t* morton, uint32_t* perm,
+              uint32_t* children, uint32_t* parents_leaf, uint32_t* parents_internal,
+              float* bboxes);
+
+// Closest hit over a device ray batch (rays: 6 floats each).
+void trace_closest(const DeviceScene& scene, const float* d_rays, uint32_t n, uint32_t* d_id,
+                   float* d_t, float* d_uv, hipStream_t stream);
+// Same traversal, also counting inner iterations, box tests and triangle
+// tests into d_counters[3] (uint64, accumulated).
+void trace_closest_counted(const DeviceScene& scene, const float* d_rays, uint32_t n,
+                           uint32_t* d_id, float* d_t, float* d_uv,
+                           unsigned long long* d_counters, hipStream_t stream);
+
+// Progressive renderer state (wavefront queues + FP64 accumulator).
+struct Renderer {
+    int w = 0, h = 0;
+    uint32_t spp = 0;  // passes accumulated since the last reset
+    float dirs[12] = {0};
+    float eye[3] = {0, 0, 0};
+    pt_settings settings{1u, 0xFFFFFFFFu, 3.0f, 1u};
+    DevBuf accum;       // double[3*w*h]
+    DevBuf queues;      // ray/hit queues and per-pixel path state
+    DevBuf counters;    // queue counters
+    uint32_t lights = 0;  // directional lights in the scene (not rendered)
+    bool timed = false;   // a pass has been bracketed by the context events
+};
+
+void render_resize(Renderer& r, int w, int h, hipStream_t stream);
+void render_reset(Renderer& r, hipStream_t stream);
+void render_pass(Renderer& r, const DeviceScene& scene, uint64_t seed, hipStream_t stream);
+
+}  // namespace pt
+
+struct pt_context {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    pt::DeviceScene scene;
+    pt::Renderer renderer;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pt::DevBuf io_rays, io_id, io_t, io_uv;  // staging for host-pointer traces
+};

@@ -1,0 +1,485 @@
+// LBVH construction on gfx950 (reference: CUDA::BVHBuilder::build,
+// BVHBuilder.cu:207-269, after Karras 2012).
+//
+//   k_morton        30-bit Morton key + identity index per triangle
+//   k_sort_hist     \
+//   k_sort_rowscan   } x4: LSD radix sort, 8-bit digits, stable
+//   k_sort_scatter  /     (wave64 ballot match + LDS staging per 4096-key tile)
+//   k_radix_tree    Karras split per internal node, parents, counter reset
+//   k_refit         sorted triangle records + bottom-up AABBs (agent-scope
+//                   acquire/release counters), written straight into the
+//                   traversal node records
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "context.hpp"
+#include "devmath.hpp"
+
+namespace pt {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortKeysPerThread = 16;
+constexpr int kSortTile = kSortThreads * kSortKeysPerThread;  // 4096 keys
+
+// ---- Morton keys ------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t spread10(uint32_t a) {
+    a &= 0x3FFu;
+    a = (a | (a << 16)) & 0x030000FFu;
+    a = (a | (a << 8)) & 0x0300F00Fu;
+    a = (a | (a << 4)) & 0x030C30C3u;
+    a = (a | (a << 2)) & 0x09249249u;
+    return a;
+}
+
+// quantize (math_util.h:10-13) with CUDA's saturating float->int conversion.
+__device__ __forceinline__ uint32_t quantize1024(float x) {
+    float s = x * 1024.0f;
+    if (!(s == s)) return 0u;
+    if (s >= 1023.0f) return 1023u;
+    if (s < 1.0f) return 0u;
+    return static_cast<uint32_t>(static_cast<int32_t>(s));
+}
+
+// morton_encoder (BVHBuilder.cu:12-32): centroid ((v0 + v1) + v2) / 3, mapped
+// into the enclosing box with invScale computed on the host.
+__global__ __launch_bounds__(256) void k_morton(const pt_triangle* __restrict__ tris, uint32_t n,
+                                                float mnx, float mny, float mnz, float ivx,
+                                                float ivy, float ivz, uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* t = reinterpret_cast<const float*>(tris + i);
+    float cx = ((t[0] + t[3]) + t[6]) / 3.0f;
+    float cy = ((t[1] + t[4]) + t[7]) / 3.0f;
+    float cz = ((t[2] + t[5]) + t[8]) / 3.0f;
+    uint32_t qx = quantize1024((cx - mnx) * ivx);
+    uint32_t qy = quantize1024((cy - mny) * ivy);
+    uint32_t qz = quantize1024((cz - mnz) * ivz);
+    keys[i] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+    vals[i] = i;
+}
+
+// ---- radix sort ---------------------------------------------------------------
+
+// Exclusive scan of one value per thread over a 256-thread block (4 waves);
+// `total` receives the block sum.  Contains two barriers.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w,
+                                                        uint32_t& total) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pre += (k < wave) ? s_w[k] : 0u;
+    total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    __syncthreads();
+    return pre + x - v;
+}
+
+// Per-tile digit histogram: tile_hist[digit * num_tiles + tile].
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
+                                                            uint32_t n, int shift,
+                                                            uint32_t num_tiles,
+                                                            uint32_t* __restrict__ tile_hist) {
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortKeysPerThread; ++r) {
+        uint32_t i = base + r * kSortThreads + tid;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    tile_hist[static_cast<size_t>(tid) * num_tiles + blockIdx.x] = h[tid];
+}
+
+// One block per digit: exclusive scan of that digit's row of tile counts (in
+// place) and the digit's total.
+__global__ __launch_bounds__(kSortThreads) void k_sort_rowscan(uint32_t* __restrict__ tile_hist,
+                                                               uint32_t num_tiles,
+                                                               uint32_t* __restrict__ digit_total) {
+    __shared__ uint32_t s_w[4];
+    uint32_t* row = tile_hist + static_cast<size_t>(blockIdx.x) * num_tiles;
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < num_tiles; b += kSortThreads) {
+        uint32_t i = b + threadIdx.x;
+        uint32_t v = i < num_tiles ? row[i] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan_256(v, s_w, tot);
+        if (i < num_tiles) row[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
+}
+
+// Stable scatter of one 4096-key tile.  Each wave ranks 64 keys at a time
+// against the keys of its chunk so far: lanes holding the same digit are
+// found with eight ballots, their order is the popcount of lower peer lanes,
+// and a per-wave LDS counter per digit carries the count between rounds.
+// Keys are then staged in LDS in digit order and written out in runs.
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+    uint32_t num_tiles, const uint32_t* __restrict__ tile_hist,
+    const uint32_t* __restrict__ digit_total) {
+    __shared__ uint32_t s_wc[4][256];
+    __shared__ uint32_t s_dstart[256];
+    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_w[4];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s_wc[k][tid] = 0;
+    __syncthreads();
+
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * kSortTile;
+    const uint32_t count = min(static_cast<uint32_t>(kSortTile), n - base);
+    const uint64_t lt = (1ull << lane) - 1ull;
+
+    uint32_t key[kSortKeysPerThread], val[kSortKeysPerThread], rk[kSortKeysPerThread];
+#pragma unroll
+    for (int r = 0; r < kSortKeysPerThread; ++r) {
+        const uint32_t li = wave * (kSortKeysPerThread * 64) + r * 64 + lane;
+        const bool valid = li < count;
+        const uint32_t k = valid ? keys_in[base + li] : 0u;
+        const uint32_t v = valid ? vals_in[base + li] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = static_cast<uint32_t>(__popcll(peers & lt));
+        uint32_t old = 0;
+        if (valid) {
+            old = s_wc[wave][d];
+            if (rank == 0) s_wc[wave][d] = old + static_cast<uint32_t>(__popcll(peers));
+        }
+        key[r] = k;
+        val[r] = v;
+        rk[r] = old + rank;
+    }
+    __syncthreads();
+
+    // Digit `tid`: offsets of each wave's run, tile-local start, global base.
+    const uint32_t c0 = s_wc[0][tid], c1 = s_wc[1][tid], c2 = s_wc[2][tid], c3 = s_wc[3][tid];
+    __syncthreads();
+    s_wc[0][tid] = 0;
+    s_wc[1][tid] = c0;
+    s_wc[2][tid] = c0 + c1;
+    s_wc[3][tid] = c0 + c1 + c2;
+    uint32_t tmp;
+    const uint32_t dstart = block_excl_scan_256(c0 + c1 + c2 + c3, s_w, tmp);
+    const uint32_t gpre = block_excl_scan_256(digit_total[tid], s_w, tmp);
+    s_dstart[tid] = dstart;
+    s_gbase[tid] = gpre + tile_hist[static_cast<size_t>(tid) * num_tiles + tile];
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < kSortKeysPerThread; ++r) {
+        const uint32_t li = wave * (kSortKeysPerThread * 64) + r * 64 + lane;
+        if (li < count) {
+            const uint32_t d = (key[r] >> shift) & 255u;
+            const uint32_t pos = s_dstart[d] + s_wc[wave][d] + rk[r];
+            s_keys[pos] = key[r];
+            s_vals[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < count; i += kSortThreads) {
+        const uint32_t k = s_keys[i];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t g = s_gbase[d] + (i - s_dstart[d]);
+        keys_out[g] = k;
+        vals_out[g] = s_vals[i];
+    }
+}
+
+// ---- radix tree -----------------------------------------------------------------
+
+// delta (BVHBuilder.cu:116-133): common prefix length of keys i and j,
+// augmented with the index bits when the keys are equal; -1 outside [0, n).
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t n, int64_t i,
+                                     uint32_t ki, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t kj = keys[j];
+    if (ki == kj) return 32 + __clz(static_cast<int>(static_cast<uint32_t>(i ^ j)));
+    return __clz(static_cast<int>(ki ^ kj));
+}
+
+// radix_tree_construction_kernel (BVHBuilder.cu:135-196), one thread per
+// internal node; also clears the node's refit counter.
+__global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__ keys, uint32_t n,
+                                                    uint32_t* __restrict__ children,
+                                                    uint32_t* __restrict__ parents_leaf,
+                                                    uint32_t* __restrict__ parents_internal,
+                                                    uint32_t* __restrict__ counters,
+                                                    float4* __restrict__ nodes) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t N = n;
+    if (i >= N - 1) return;
+    const uint32_t ki = keys[i];
+    const int dr = delta(keys, N, i, ki, i + 1) - delta(keys, N, i, ki, i - 1);
+    const int64_t d = (dr > 0) - (dr < 0);
+    const int dmin = delta(keys, N, i, ki, i - d);
+    int64_t lmax = 2;
+    while (delta(keys, N, i, ki, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    while (lmax > 1) {
+        lmax /= 2;
+        if (delta(keys, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
+    }
+    const int64_t j = i + l * d;
+    const int dnode = delta(keys, N, i, ki, j);
+    int64_t s = 0;
+    while (l > 1) {
+        l = (l + 1) / 2;
+        if (delta(keys, N, i, ki, i + (s + l) * d) > dnode) s += l;
+    }
+    s = i + s * d + (d < 0 ? d : 0);
+    uint32_t type = 0;
+    if ((i < j ? i : j) != s) {
+        type |= 1u;
+        parents_internal[s] = static_cast<uint32_t>(i);
+    } else {
+        parents_leaf[s] = static_cast<uint32_t>(i);
+    }
+    if ((i > j ? i : j) != s + 1) {
+        type |= 2u;
+        parents_internal[s + 1] = static_cast<uint32_t>(i);
+    } else {
+        parents_leaf[s + 1] = static_cast<uint32_t>(i);
+    }
+    const uint32_t w = (static_cast<uint32_t>(s) << 2) | type;
+    children[i] = w;
+    nodes[4 * i + 3] = make_float4(__uint_as_float(w), 0.0f, 0.0f, 0.0f);
+    counters[i] = 0u;
+    if (i == 0) parents_internal[0] = kInvalid;  // BVHBuilder.cu:224
+}
+
+// ---- refit ------------------------------------------------------------------------
+
+struct Box {
+    float mnx, mxx, mny, mxy, mnz, mxz;
+};
+
+__device__ __forceinline__ void store_slot(float4* nodes, uint32_t p, bool left, const Box& b) {
+    float4* rec = nodes + 4 * static_cast<size_t>(p);
+    if (left) {
+        rec[0] = make_float4(b.mnx, b.mxx, b.mny, b.mxy);
+        reinterpret_cast<float2*>(rec + 1)[0] = make_float2(b.mnz, b.mxz);
+    } else {
+        reinterpret_cast<float2*>(rec + 1)[1] = make_float2(b.mnx, b.mxx);
+        rec[2] = make_float4(b.mny, b.mxy, b.mnz, b.mxz);
+    }
+}
+
+// compute_internal_node_bbox_kernel (BVHBuilder.cu:54-91), one thread per
+// sorted leaf.  The leaf also writes its sorted triangle record (the
+// reference instead permutes the 40-byte triangles inside the sort).  The
+// second thread to reach a node unites the two child boxes: the counter
+// increment is acquire-release at agent scope so the sibling's box stores,
+// made before its own increment, are visible across XCDs.
+__global__ __launch_bounds__(256) void k_refit(
+    const pt_triangle* __restrict__ in_tris, const uint32_t* __restrict__ perm,
+    const uint32_t* __restrict__ children, uint32_t* __restrict__ parents_leaf,
+    const uint32_t* __restrict__ parents_internal, uint32_t* __restrict__ counters,
+    float4* __restrict__ nodes, float4* __restrict__ tri_recs, float* __restrict__ root_box,
+    uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const pt_triangle* t = in_tris + perm[k];
+    const V3 v0 = v3(t->v0[0], t->v0[1], t->v0[2]);
+    const V3 v1 = v3(t->v1[0], t->v1[1], t->v1[2]);
+    const V3 v2 = v3(t->v2[0], t->v2[1], t->v2[2]);
+    const uint32_t id = t->id;
+    const V3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+    tri_recs[3 * static_cast<size_t>(k) + 0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(id));
+    tri_recs[3 * static_cast<size_t>(k) + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+    tri_recs[3 * static_cast<size_t>(k) + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    // BBox() then unite({v0, v1, v2}) (BBox.cpp:4-6, 14-24).
+    Box b;
+    b.mnx = gmin(gmin(gmin(kFltMax, v0.x), v1.x), v2.x);
+    b.mny = gmin(gmin(gmin(kFltMax, v0.y), v1.y), v2.y);
+    b.mnz = gmin(gmin(gmin(kFltMax, v0.z), v1.z), v2.z);
+    b.mxx = gmax(gmax(gmax(-kFltMax, v0.x), v1.x), v2.x);
+    b.mxy = gmax(gmax(gmax(-kFltMax, v0.y), v1.y), v2.y);
+    b.mxz = gmax(gmax(gmax(-kFltMax, v0.z), v1.z), v2.z);
+    if (n == 1) {
+        parents_leaf[0] = kInvalid;
+        root_box[0] = b.mnx; root_box[1] = b.mny; root_box[2] = b.mnz;
+        root_box[3] = b.mxx; root_box[4] = b.mxy; root_box[5] = b.mxz;
+        return;
+    }
+    uint32_t p = parents_leaf[k];
+    store_slot(nodes, p, (children[p] >> 2) == k, b);
+    for (;;) {
+        const uint32_t old =
+            __hip_atomic_fetch_add(counters + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) return;
+        const float4* rec = nodes + 4 * static_cast<size_t>(p);
+        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        // leftBBox.unite(rightBBox) (BVHBuilder.cu:85, BBox.cpp:30-36).
+        b.mnx = gmin(r0.x, r1.z);
+        b.mxx = gmax(r0.y, r1.w);
+        b.mny = gmin(r0.z, r2.x);
+        b.mxy = gmax(r0.w, r2.y);
+        b.mnz = gmin(r1.x, r2.z);
+        b.mxz = gmax(r1.y, r2.w);
+        const uint32_t pp = parents_internal[p];
+        if (pp == kInvalid) {
+            root_box[0] = b.mnx; root_box[1] = b.mny; root_box[2] = b.mnz;
+            root_box[3] = b.mxx; root_box[4] = b.mxy; root_box[5] = b.mxz;
+            return;
+        }
+        store_slot(nodes, pp, (children[pp] >> 2) == p, b);
+        p = pp;
+    }
+}
+
+// Parity dump: internal node boxes in the reference's BBox layout
+// (min.xyz, max.xyz), read back from the parents' records.
+__global__ __launch_bounds__(256) void k_dump_boxes(const float4* __restrict__ nodes,
+                                                    const uint32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ parents_internal,
+                                                    const float* __restrict__ root_box,
+                                                    uint32_t n, float* __restrict__ out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c + 1 >= n) return;
+    float* o = out + 6 * static_cast<size_t>(c);
+    if (c == 0) {
+        for (int k = 0; k < 6; ++k) o[k] = root_box[k];
+        return;
+    }
+    const uint32_t p = parents_internal[c];
+    const float4* rec = nodes + 4 * static_cast<size_t>(p);
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    if ((children[p] >> 2) == c) {
+        o[0] = r0.x; o[1] = r0.z; o[2] = r1.x; o[3] = r0.y; o[4] = r0.w; o[5] = r1.y;
+    } else {
+        o[0] = r1.z; o[1] = r2.x; o[2] = r2.z; o[3] = r1.w; o[4] = r2.y; o[5] = r2.w;
+    }
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
+
+}  // namespace
+
+void build_lbvh(DeviceScene& s, hipStream_t st) {
+    DeviceBvh& b = s.bvh;
+    const uint32_t n = s.n;
+    b.n = n;
+    if (n == 0) return;
+    if (n >= (1u << 30)) throw Error(PT_E_INVALID_ARG, "LBVH supports fewer than 2^30 triangles");
+    const size_t n4 = static_cast<size_t>(n) * 4;
+    const size_t m = n > 1 ? n - 1 : 1;
+    b.keys.reserve(n4);
+    b.vals.reserve(n4);
+    b.keys_alt.reserve(n4);
+    b.vals_alt.reserve(n4);
+    b.children.reserve(m * 4);
+    b.parents_leaf.reserve(n4);
+    b.parents_internal.reserve(m * 4);
+    b.counters.reserve(m * 4);
+    b.nodes.reserve(m * 64);
+    b.tris.reserve(static_cast<size_t>(n) * 48);
+    b.root_box.reserve(6 * sizeof(float));
+    const uint32_t num_tiles = div_up(n, kSortTile);
+    b.tile_hist.reserve(static_cast<size_t>(num_tiles) * 256 * 4);
+    b.digit_total.reserve(256 * 4);
+
+    // invScale = 1 / scale on the host (BVHBuilder.cu:15-17).
+    float inv[3];
+    for (int k = 0; k < 3; ++k) inv[k] = 1.0f / (s.bmax[k] - s.bmin[k]);
+    hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, st,
+                       s.tris.as<pt_triangle>(), n, s.bmin[0], s.bmin[1], s.bmin[2], inv[0],
+                       inv[1], inv[2], b.keys.as<uint32_t>(), b.vals.as<uint32_t>());
+    PT_HIP(hipGetLastError());
+
+    if (n >= 2) {
+        uint32_t* kin = b.keys.as<uint32_t>();
+        uint32_t* vin = b.vals.as<uint32_t>();
+        uint32_t* kout = b.keys_alt.as<uint32_t>();
+        uint32_t* vout = b.vals_alt.as<uint32_t>();
+        for (int pass = 0; pass < 4; ++pass) {
+            const int shift = 8 * pass;
+            hipLaunchKernelGGL(k_sort_hist, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, n,
+                               shift, num_tiles, b.tile_hist.as<uint32_t>());
+            hipLaunchKernelGGL(k_sort_rowscan, dim3(256), dim3(kSortThreads), 0, st,
+                               b.tile_hist.as<uint32_t>(), num_tiles, b.digit_total.as<uint32_t>());
+            hipLaunchKernelGGL(k_sort_scatter, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                               kout, vout, n, shift, num_tiles, b.tile_hist.as<uint32_t>(),
+                               b.digit_total.as<uint32_t>());
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
+        PT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n - 1, 256)), dim3(256), 0, st,
+                           b.keys.as<uint32_t>(), n, b.children.as<uint32_t>(),
+                           b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
+                           b.counters.as<uint32_t>(), b.nodes.as<float4>());
+        PT_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_refit, dim3(div_up(n, 256)), dim3(256), 0, st, s.tris.as<pt_triangle>(),
+                       b.vals.as<uint32_t>(), b.children.as<uint32_t>(),
+                       b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
+                       b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.tris.as<float4>(),
+                       b.root_box.as<float>(), n);
+    PT_HIP(hipGetLastError());
+}
+
+void dump_bvh(DeviceScene& s, hipStream_t st, uint32_t* morton, uint32_t* perm,
+              uint32_t* children, uint32_t* parents_leaf, uint32_t* parents_internal,
+              float* bboxes) {
+    const DeviceBvh& b = s.bvh;
+    const uint32_t n = b.n;
+    if (n == 0) return;
+    const size_t m = n - 1;
+    if (morton) PT_HIP(hipMemcpyAsync(morton, b.keys.as<void>(), n * 4ull, hipMemcpyDeviceToHost, st));
+    if (perm) PT_HIP(hipMemcpyAsync(perm, b.vals.as<void>(), n * 4ull, hipMemcpyDeviceToHost, st));
+    if (parents_leaf)
+        PT_HIP(hipMemcpyAsync(parents_leaf, b.parents_leaf.as<void>(), n * 4ull,
+                              hipMemcpyDeviceToHost, st));
+    if (m > 0) {
+        if (children)
+            PT_HIP(hipMemcpyAsync(children, b.children.as<void>(), m * 4, hipMemcpyDeviceToHost, st));
+        if (parents_internal)
+            PT_HIP(hipMemcpyAsync(parents_internal, b.parents_internal.as<void>(), m * 4,
+                                  hipMemcpyDeviceToHost, st));
+        if (bboxes) {
+            DevBuf tmp;
+            tmp.reserve(m * 24);
+            hipLaunchKernelGGL(k_dump_boxes, dim3(div_up(m, 256)), dim3(256), 0, st,
+                               b.nodes.as<float4>(), b.children.as<uint32_t>(),
+                               b.parents_internal.as<uint32_t>(), b.root_box.as<float>(), n,
+                               tmp.as<float>());
+            PT_HIP(hipGetLastError());
+            PT_HIP(hipMemcpyAsync(bboxes, tmp.as<void>(), m * 24, hipMemcpyDeviceToHost, st));
+            PT_HIP(hipStreamSynchronize(st));
+            return;
+        }
+    }
+    PT_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace pt

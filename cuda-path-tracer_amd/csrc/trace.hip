@@ -1,0 +1,71 @@
+// Batch closest-hit queries (the reference's CUDA::trace, trace.h:73-169,
+// exposed as a ray-batch entry point).
+#include <hip/hip_runtime.h>
+
+#include "context.hpp"
+#include "traverse.hpp"
+
+namespace pt {
+namespace {
+
+constexpr int kTraceThreads = 256;
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceThreads) void k_trace(
+    const float4* __restrict__ nodes, const float4* __restrict__ tris, uint32_t ntri,
+    const float* __restrict__ rays, uint32_t nrays, uint32_t* __restrict__ out_id,
+    float* __restrict__ out_t, float2* __restrict__ out_uv,
+    unsigned long long* __restrict__ counters) {
+    TraceCount cnt{0, 0, 0};
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nrays; i += stride) {
+        const float* r = rays + 6 * static_cast<size_t>(i);
+        const V3 o = v3(r[0], r[1], r[2]);
+        const V3 d = v3(r[3], r[4], r[5]);
+        const TraceHit h = trace_ray<COUNT>(nodes, tris, ntri, o, d, cnt);
+        if (out_id) out_id[i] = h.id;
+        if (out_t) out_t[i] = h.t;
+        if (out_uv) out_uv[i] = make_float2(h.u, h.v);
+    }
+    if (COUNT) {
+        // Wave-reduce the per-lane counts, one atomic per wave and counter.
+        unsigned long long a = cnt.iters, b = cnt.boxes, c = cnt.tris;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_down(a, off, 64);
+            b += __shfl_down(b, off, 64);
+            c += __shfl_down(c, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(counters + 0, a);
+            atomicAdd(counters + 1, b);
+            atomicAdd(counters + 2, c);
+        }
+    }
+}
+
+template <bool COUNT>
+void launch_trace(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
+                  float* d_t, float* d_uv, unsigned long long* d_counters, hipStream_t st) {
+    if (n == 0) return;
+    const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(n) + kTraceThreads - 1) /
+                                                  kTraceThreads);
+    hipLaunchKernelGGL(k_trace<COUNT>, dim3(blocks), dim3(kTraceThreads), 0, st,
+                       s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), s.bvh.n, d_rays, n, d_id,
+                       d_t, reinterpret_cast<float2*>(d_uv), d_counters);
+    PT_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void trace_closest(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
+                   float* d_t, float* d_uv, hipStream_t st) {
+    launch_trace<false>(s, d_rays, n, d_id, d_t, d_uv, nullptr, st);
+}
+
+void trace_closest_counted(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
+                           float* d_t, float* d_uv, unsigned long long* d_counters,
+                           hipStream_t st) {
+    launch_trace<true>(s, d_rays, n, d_id, d_t, d_uv, d_counters, st);
+}
+
+}  // namespace pt

@@ -1,0 +1,142 @@
+// Closest-hit traversal over the 64-byte node records (reference: CUDA::trace,
+// trace.h:73-169).  Visiting order is the reference's: test both children,
+// descend into the nearer (left on ties), defer the farther internal child on
+// a stack.  For the lowest-id tie rule the culling keeps children at exactly
+// the best distance, and a triangle replaces the best hit when nearer, or
+// equally near with a lower original id.
+#pragma once
+
+#include "devmath.hpp"
+
+namespace pt {
+
+struct TraceHit {
+    float t;        // FLT_MAX on a miss
+    uint32_t id;    // original triangle id, kInvalid on a miss
+    float u, v;     // barycentrics of v1, v2
+};
+
+struct TraceCount {
+    uint32_t iters, boxes, tris;
+};
+
+// Stack bound: every deferred node is the sibling of a distinct ancestor of
+// the current node, and a radix tree over < 2^30 augmented 62-bit keys is at
+// most 63 internal levels deep.
+constexpr int kStackMax = 64;
+
+template <bool COUNT, bool FINITE>
+__device__ __forceinline__ float child_box(V3 o, V3 dinv, float a, float b, float c, float d,
+                                           float e, float f) {
+    return FINITE ? slab_test_finite(o, dinv, a, b, c, d, e, f) : slab_test(o, dinv, a, b, c, d, e, f);
+}
+
+__device__ __forceinline__ float tri_hit(const float4* __restrict__ tris, uint32_t k, V3 o, V3 d,
+                                         float& u, float& v, uint32_t& id) {
+    const float4 a = tris[3 * static_cast<size_t>(k)];
+    const float4 b = tris[3 * static_cast<size_t>(k) + 1];
+    const float4 c = tris[3 * static_cast<size_t>(k) + 2];
+    id = __float_as_uint(a.w);
+    return mt_test(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), u, v);
+}
+
+template <bool COUNT, bool FINITE>
+__device__ __forceinline__ TraceHit traverse_impl(const float4* __restrict__ nodes,
+                                                  const float4* __restrict__ tris, V3 o, V3 d,
+                                                  V3 dinv, TraceCount& cnt) {
+    float minT = kMinTInit;
+    uint32_t best = kInvalid;
+    float bu = 0.0f, bv = 0.0f;
+    uint32_t stack[kStackMax];
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        for (;;) {
+            const float4* rec = nodes + 4 * static_cast<size_t>(node);
+            const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+            const uint32_t w = __float_as_uint(r3.x);
+            const uint32_t l = w >> 2;
+            const bool in0 = (w & 1u) != 0, in1 = (w & 2u) != 0;
+            float h0, h1, u0 = 0.0f, v0 = 0.0f, u1 = 0.0f, v1 = 0.0f;
+            uint32_t id0 = kInvalid, id1 = kInvalid;
+            if (COUNT) cnt.iters++;
+            if (in0) {
+                if (COUNT) cnt.boxes++;
+                h0 = child_box<COUNT, FINITE>(o, dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+            } else {
+                if (COUNT) cnt.tris++;
+                h0 = tri_hit(tris, l, o, d, u0, v0, id0);
+            }
+            if (in1) {
+                if (COUNT) cnt.boxes++;
+                h1 = child_box<COUNT, FINITE>(o, dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
+            } else {
+                if (COUNT) cnt.tris++;
+                h1 = tri_hit(tris, l + 1, o, d, u1, v1, id1);
+            }
+            if (!(h0 <= minT) && !(h1 <= minT)) break;
+            const bool sw = h1 < h0;  // left child wins ties (trace.h:125)
+            const float hc = sw ? h1 : h0, hf = sw ? h0 : h1;
+            const uint32_t cc = sw ? l + 1 : l, cf = sw ? l : l + 1;
+            const bool ic = sw ? in1 : in0, jf = sw ? in0 : in1;
+            const float uc = sw ? u1 : u0, vc = sw ? v1 : v0, uf = sw ? u0 : u1, vf = sw ? v0 : v1;
+            const uint32_t idc = sw ? id1 : id0, idf = sw ? id0 : id1;
+            if (ic) {
+                node = cc;
+                if (hf <= minT) {
+                    if (jf) {
+                        if (sp < kStackMax) stack[sp++] = cf;
+                    } else if (hf < minT || idf < best) {
+                        minT = hf; best = idf; bu = uf; bv = vf;
+                    }
+                }
+                continue;
+            }
+            if (hc < minT || idc < best) {
+                minT = hc; best = idc; bu = uc; bv = vc;
+            }
+            if (hf <= minT) {
+                if (jf) {
+                    node = cf;
+                    continue;
+                }
+                if (hf < minT || idf < best) {
+                    minT = hf; best = idf; bu = uf; bv = vf;
+                }
+            }
+            break;
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    TraceHit hit;
+    hit.id = best;
+    hit.t = best == kInvalid ? kFltMax : minT;
+    hit.u = best == kInvalid ? 0.0f : bu;
+    hit.v = best == kInvalid ? 0.0f : bv;
+    return hit;
+}
+
+// Entry for one ray over a BVH of ntri triangles.
+template <bool COUNT>
+__device__ __forceinline__ TraceHit trace_ray(const float4* __restrict__ nodes,
+                                              const float4* __restrict__ tris, uint32_t ntri, V3 o,
+                                              V3 d, TraceCount& cnt) {
+    TraceHit miss{kFltMax, kInvalid, 0.0f, 0.0f};
+    if (ntri == 0) return miss;
+    if (ntri == 1) {
+        float u = 0.0f, v = 0.0f;
+        uint32_t id;
+        if (COUNT) cnt.tris++;
+        const float h = tri_hit(tris, 0, o, d, u, v, id);
+        if (h <= kMinTInit) return TraceHit{h, id, u, v};
+        return miss;
+    }
+    const V3 dinv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool finite = __builtin_fabsf(dinv.x) <= kFltMax && __builtin_fabsf(dinv.y) <= kFltMax &&
+                        __builtin_fabsf(dinv.z) <= kFltMax;
+    if (finite) return traverse_impl<COUNT, true>(nodes, tris, o, d, dinv, cnt);
+    return traverse_impl<COUNT, false>(nodes, tris, o, d, dinv, cnt);
+}
+
+}  // namespace pt

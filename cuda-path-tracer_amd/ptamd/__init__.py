@@ -1,0 +1,302 @@
+"""ctypes binding of libptamd.so (the C-ABI in include/pt_api.h and
+include/pt_scenes.h) for tests and benchmarks.
+
+The library is loaded from ``cuda-path-tracer_amd/lib/libptamd.so``; importing
+this module raises ImportError when it has not been built.  GPU entry points
+raise ``PtError`` with the library's message when a call fails (for example
+PT_E_NO_DEVICE on a machine without a GPU).  Arrays are numpy arrays with the
+C layouts below; device-pointer entry points take integer addresses (for
+instance ``torch.Tensor.data_ptr()``) and a stream handle.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libptamd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libptamd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+PT_OK = 0
+PT_E_INVALID_ARG = -1
+PT_E_HIP = -2
+PT_E_NO_DEVICE = -3
+PT_E_NO_SCENE = -4
+PT_E_UNSUPPORTED = -5
+PT_E_OOM = -6
+PT_E_INTERNAL = -7
+INVALID = 0xFFFFFFFF
+FLT_MAX = float(np.finfo(np.float32).max)
+
+# pt_triangle (40 B) and pt_triangle_info (64 B).
+TRI_DTYPE = np.dtype([("v0", "<f4", (3,)), ("v1", "<f4", (3,)), ("v2", "<f4", (3,)), ("id", "<u4")])
+INFO_DTYPE = np.dtype([("n0", "<f4", (3,)), ("n1", "<f4", (3,)), ("n2", "<f4", (3,)),
+                       ("albedo", "<f4", (3,)), ("emission", "<f4", (3,)), ("specularity", "<f4")])
+assert TRI_DTYPE.itemsize == 40 and INFO_DTYPE.itemsize == 64
+
+
+class Settings(ctypes.Structure):
+    """pt_settings: bounces, rr_start (INVALID = off), indirect_intensity, stratified."""
+    _fields_ = [("bounces", ctypes.c_uint32), ("rr_start", ctypes.c_uint32),
+                ("indirect_intensity", ctypes.c_float), ("stratified", ctypes.c_uint32)]
+
+
+class PtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pt error {code}: {msg}")
+        self.code = code
+
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i = ctypes.c_int
+_f = ctypes.c_float
+
+_SIGS = {
+    "pt_create": [ctypes.POINTER(_vp), _i],
+    "pt_destroy": [_vp],
+    "pt_last_error": [_vp, ctypes.c_char_p, ctypes.c_size_t],
+    "pt_set_stream": [_vp, _vp],
+    "pt_synchronize": [_vp],
+    "pt_upload_scene": [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp],
+    "pt_set_settings": [_vp, _vp],
+    "pt_build_bvh": [_vp, _vp],
+    "pt_get_bvh": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pt_get_scene_info": [_vp, _vp, _vp, _vp],
+    "pt_trace_closest": [_vp, _vp, _u32, _vp, _vp, _vp],
+    "pt_trace_closest_device": [_vp, _vp, _u32, _vp, _vp, _vp, _vp],
+    "pt_trace_closest_counted": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp],
+    "pt_set_primary_rays": [_vp, _vp, _vp],
+    "pt_render_pass": [_vp, _i, _i, _u64],
+    "pt_reset_frames": [_vp],
+    "pt_read_accumulator": [_vp, _vp, _vp],
+    "pt_write_accumulator": [_vp, _i, _i, _vp, _u32],
+    "pt_read_image": [_vp, _vp],
+    "pt_last_render_ms": [_vp, _vp],
+    "pt_scene_bounds": [_vp, _u32, _vp, _vp],
+    "pt_gen_soup": [_u64, _u32, _i, _u32, _vp, _vp],
+    "pt_gen_sphere": [_u64, _u32, _i, _vp, _vp],
+    "pt_gen_hairball": [_u64, _u32, _u32, _i, _vp, _vp],
+    "pt_gen_shell_rays": [_u64, _vp, _vp, _f, _u32, _vp],
+    "pt_gen_batch_counts": [_u64, _u32, _u32, _u32, _vp],
+    "pt_camera_corners": [_vp, _vp, _vp, _f, _f, _f, _f, _vp],
+    "pt_pass_jitter": [_u64, _u32, _i, _i, _vp, _vp],
+}
+for _name, _args in _SIGS.items():
+    _fn = getattr(_lib, _name)
+    _fn.argtypes = _args
+    _fn.restype = _i
+
+EXPORTED = tuple(_SIGS)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} floats")
+    return a
+
+
+def _check(rc, ctx=None):
+    if rc < 0:
+        buf = ctypes.create_string_buffer(512)
+        _lib.pt_last_error(ctx, buf, 512)
+        raise PtError(rc, buf.value.decode(errors="replace"))
+    return rc
+
+
+# ---- generators (host only) ---------------------------------------------------
+
+def scene_bounds(tris):
+    bmin = np.zeros(3, np.float32)
+    bmax = np.zeros(3, np.float32)
+    _check(_lib.pt_scene_bounds(_ptr(tris), len(tris), _ptr(bmin), _ptr(bmax)))
+    return bmin, bmax
+
+
+def gen_soup(n, seed, emitter=True, first_id=0):
+    cap = n + 2
+    tris = np.zeros(cap, TRI_DTYPE)
+    infos = np.zeros(cap, INFO_DTYPE)
+    k = _check(_lib.pt_gen_soup(seed, n, 1 if emitter else 0, first_id, _ptr(tris), _ptr(infos)))
+    return tris[:k].copy(), infos[:k].copy()
+
+
+def gen_sphere(bands, seed, room=True):
+    cap = 2 * bands * bands + 14
+    tris = np.zeros(cap, TRI_DTYPE)
+    infos = np.zeros(cap, INFO_DTYPE)
+    k = _check(_lib.pt_gen_sphere(seed, bands, 1 if room else 0, _ptr(tris), _ptr(infos)))
+    return tris[:k].copy(), infos[:k].copy()
+
+
+def gen_hairball(strands, steps, seed, light=True):
+    cap = 2 * strands * steps + 2
+    tris = np.zeros(cap, TRI_DTYPE)
+    infos = np.zeros(cap, INFO_DTYPE)
+    k = _check(_lib.pt_gen_hairball(seed, strands, steps, 1 if light else 0, _ptr(tris), _ptr(infos)))
+    return tris[:k].copy(), infos[:k].copy()
+
+
+def gen_shell_rays(n, seed, bmin, bmax, enlarge=1.5):
+    rays = np.zeros((n, 6), np.float32)
+    bmin, bmax = _f32(bmin, 3), _f32(bmax, 3)
+    _check(_lib.pt_gen_shell_rays(seed, _ptr(bmin), _ptr(bmax), enlarge, n, _ptr(rays)))
+    return rays
+
+
+def batch_counts(nscenes, seed, lo=1024, hi=262144):
+    c = np.zeros(nscenes, np.uint32)
+    _check(_lib.pt_gen_batch_counts(seed, nscenes, lo, hi, _ptr(c)))
+    return c
+
+
+def camera_corners(eye, target, up=(0.0, 1.0, 0.0), fovy_deg=60.0, aspect=1.0, jx=0.0, jy=0.0):
+    d = np.zeros(12, np.float32)
+    eye, target, up = _f32(eye, 3), _f32(target, 3), _f32(up, 3)
+    _check(_lib.pt_camera_corners(_ptr(eye), _ptr(target), _ptr(up), fovy_deg, aspect, jx, jy, _ptr(d)))
+    return d
+
+
+def pass_jitter(seed, pass_index, w, h):
+    jx = np.zeros(1, np.float32)
+    jy = np.zeros(1, np.float32)
+    _check(_lib.pt_pass_jitter(seed, pass_index, w, h, _ptr(jx), _ptr(jy)))
+    return float(jx[0]), float(jy[0])
+
+
+# ---- device context -------------------------------------------------------------
+
+class Context:
+    """One pt_context (one device, one scene, one accumulator)."""
+
+    def __init__(self, device=0):
+        h = _vp()
+        _check(_lib.pt_create(ctypes.byref(h), device))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            _lib.pt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        return _check(rc, self._h)
+
+    def upload_scene(self, tris, infos, bmin=None, bmax=None, settings=None, lights=None):
+        tris = np.ascontiguousarray(tris, TRI_DTYPE)
+        infos = np.ascontiguousarray(infos, INFO_DTYPE)
+        bmin_a = None if bmin is None else _f32(bmin, 3)
+        bmax_a = None if bmax is None else _f32(bmax, 3)
+        st = None if settings is None else ctypes.byref(settings)
+        nl = 0 if lights is None else len(lights)
+        la = None if lights is None else np.ascontiguousarray(lights, np.float32)
+        self._c(_lib.pt_upload_scene(self._h, _ptr(tris), len(tris), _ptr(infos), len(infos),
+                                     _ptr(la), nl, _ptr(bmin_a), _ptr(bmax_a), st))
+        self.n = len(tris)
+
+    def set_settings(self, settings):
+        self._c(_lib.pt_set_settings(self._h, ctypes.byref(settings)))
+
+    def build_bvh(self, timed=True):
+        ms = np.zeros(1, np.float32)
+        self._c(_lib.pt_build_bvh(self._h, _ptr(ms) if timed else None))
+        return float(ms[0]) if timed else None
+
+    def get_bvh(self):
+        n = self.n
+        m = max(n - 1, 0)
+        out = dict(morton=np.zeros(n, np.uint32), perm=np.zeros(n, np.uint32),
+                   children=np.zeros(m, np.uint32), parents_leaf=np.zeros(n, np.uint32),
+                   parents_internal=np.zeros(m, np.uint32), bboxes=np.zeros((m, 6), np.float32))
+        self._c(_lib.pt_get_bvh(self._h, _ptr(out["morton"]), _ptr(out["perm"]), _ptr(out["children"]),
+                                _ptr(out["parents_leaf"]), _ptr(out["parents_internal"]),
+                                _ptr(out["bboxes"])))
+        return out
+
+    def scene_info(self):
+        n = np.zeros(1, np.uint32)
+        bmin = np.zeros(3, np.float32)
+        bmax = np.zeros(3, np.float32)
+        self._c(_lib.pt_get_scene_info(self._h, _ptr(n), _ptr(bmin), _ptr(bmax)))
+        return int(n[0]), bmin, bmax
+
+    def trace_closest(self, rays):
+        rays = _f32(rays).reshape(-1, 6)
+        n = len(rays)
+        ids = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        uv = np.zeros((n, 2), np.float32)
+        self._c(_lib.pt_trace_closest(self._h, _ptr(rays), n, _ptr(ids), _ptr(t), _ptr(uv)))
+        return ids, t, uv
+
+    def trace_closest_device(self, d_rays, n, d_id, d_t, d_uv=0, stream=0):
+        self._c(_lib.pt_trace_closest_device(self._h, d_rays, n, d_id or None, d_t or None,
+                                             d_uv or None, stream or None))
+
+    def trace_closest_counted(self, d_rays, n, d_id=0, d_t=0, d_uv=0, stream=0):
+        c = np.zeros(3, np.uint64)
+        self._c(_lib.pt_trace_closest_counted(self._h, d_rays, n, d_id or None, d_t or None,
+                                              d_uv or None, _ptr(c), stream or None))
+        return c
+
+    def set_primary_rays(self, dirs, eye):
+        dirs, eye = _f32(dirs, 12), _f32(eye, 3)
+        self._c(_lib.pt_set_primary_rays(self._h, _ptr(dirs), _ptr(eye)))
+
+    def render_pass(self, w, h, seed):
+        self._c(_lib.pt_render_pass(self._h, w, h, seed))
+
+    def reset_frames(self):
+        self._c(_lib.pt_reset_frames(self._h))
+
+    def read_accumulator(self, w, h):
+        acc = np.zeros((h, w, 3), np.float64)
+        spp = np.zeros(1, np.uint32)
+        self._c(_lib.pt_read_accumulator(self._h, _ptr(acc), _ptr(spp)))
+        return acc, int(spp[0])
+
+    def write_accumulator(self, acc, spp):
+        acc = np.ascontiguousarray(acc, np.float64)
+        h, w, _ = acc.shape
+        self._c(_lib.pt_write_accumulator(self._h, w, h, _ptr(acc), spp))
+
+    def read_image(self, w, h):
+        img = np.zeros((h, w, 4), np.float32)
+        self._c(_lib.pt_read_image(self._h, _ptr(img)))
+        return img
+
+    def last_render_ms(self):
+        ms = np.zeros(1, np.float32)
+        self._c(_lib.pt_last_render_ms(self._h, _ptr(ms)))
+        return float(ms[0])
+
+    def synchronize(self):
+        self._c(_lib.pt_synchronize(self._h))
+
+    def set_stream(self, stream):
+        self._c(_lib.pt_set_stream(self._h, stream or None))

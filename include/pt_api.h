@@ -1,0 +1,170 @@
+/*
+ * pt_api.h -- C-ABI of the MI355X-native LBVH path tracer (libptamd.so).
+ *
+ * Drop-in boundary for the reference's CUDA path
+ * (compix/CUDA-Path-Tracer, source/application/cuda):
+ *   CUDA::BVHBuilder::build            BVHBuilder.h:15, BVHBuilder.cu:207-269
+ *   CUDA::Scene::upload                Scene.h:24, Scene.cu:28-55
+ *   CUDAPathTracer::{upload, setPrimaryRays, setEyePos, raytrace, resetFrames}
+ *                                      cuda_path_tracer.h:18-28, cuda_path_tracer.cu:366-470
+ *   CUDA::trace (closest hit, device)  trace.h:73-169
+ *
+ * Conventions (all entry points):
+ *   - every function returns int status: PT_OK (0) or a negative PT_E* code;
+ *     nothing throws across the ABI; pt_last_error() returns the message.
+ *   - plain pointers and sizes only.  Functions without a `_device` suffix take
+ *     HOST pointers and are host-synchronous.  `_device` variants take device
+ *     pointers plus an optional hipStream_t (passed as void*, NULL = the
+ *     context's own stream) and are asynchronous on that stream.
+ *   - one context per host thread; a context owns one device, its scene,
+ *     its BVH, its accumulator and its scratch.
+ */
+#ifndef PT_API_H
+#define PT_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PT_OK 0
+#define PT_E_INVALID_ARG -1
+#define PT_E_HIP -2
+#define PT_E_NO_DEVICE -3
+#define PT_E_NO_SCENE -4
+#define PT_E_UNSUPPORTED -5
+#define PT_E_OOM -6
+#define PT_E_INTERNAL -7
+
+#define PT_INVALID_INDEX 0xFFFFFFFFu
+
+/* Triangle as the caller hands it over: three fp32 vertices plus the caller's
+ * triangle id (reference: CUDA::Triangle, geometry.h:20-30 -- v0, v1, v2,
+ * infoIdx; 40 bytes).  `id` indexes the shading records and is the "original
+ * ID" reported by traces and used to break equal-distance ties. */
+typedef struct pt_triangle {
+    float v0[3];
+    float v1[3];
+    float v2[3];
+    uint32_t id;
+} pt_triangle;
+
+/* Per-triangle shading record (reference: CUDA::TriangleInfo + Material,
+ * TriangleInfo.h:10-75, reduced to what the bounce loop reads):
+ * vertex normals (interpolated with the hit barycentrics and normalised,
+ * TriangleInfo.h:44-47), albedo used as-is (the texture branch,
+ * cuda_path_tracer.cu:127-131), emission (cuda_path_tracer.cu:194-199). */
+typedef struct pt_triangle_info {
+    float n0[3];
+    float n1[3];
+    float n2[3];
+    float albedo[3];
+    float emission[3];
+    float specularity;
+} pt_triangle_info;
+
+/* Directional light (reference: CUDA::DirectionalLight, Lights.h:6-11).
+ * Accepted by the ABI; rendering with lights returns PT_E_UNSUPPORTED. */
+typedef struct pt_dir_light {
+    float color[3];
+    float direction[3];
+    float intensity;
+} pt_dir_light;
+
+/* Render settings (reference: PathTracerSettings::GI, PathTracingSettings.h:19-29,
+ * read at upload in Scene.cu:43-54).  rr_start = PT_INVALID_INDEX disables
+ * Russian roulette (the reference has none). */
+typedef struct pt_settings {
+    uint32_t bounces;            /* indirectBounceCount                       */
+    uint32_t rr_start;           /* first bounce index with Russian roulette   */
+    float indirect_intensity;    /* indirectIntensity                         */
+    uint32_t stratified;         /* useStratifiedSampling (first bounce only)  */
+} pt_settings;
+
+typedef struct pt_context pt_context;
+
+/* ---- context ------------------------------------------------------------ */
+int pt_create(pt_context** out, int device);
+int pt_destroy(pt_context* ctx);
+/* Copies the last error message of ctx (or of the library when ctx is NULL). */
+int pt_last_error(const pt_context* ctx, char* buf, size_t len);
+/* Use the caller's hipStream_t for all further work (NULL restores the
+ * context's own stream). */
+int pt_set_stream(pt_context* ctx, void* stream);
+int pt_synchronize(pt_context* ctx);
+
+/* ---- scene + LBVH --------------------------------------------------------- */
+/* Reference: CUDAPathTracer::upload -> CUDA::Scene::upload -> BVHBuilder::build
+ * (cuda_path_tracer.cu:466-470, Scene.cu:28-55, BVHBuilder.cu:207-269).
+ * Copies triangles/infos to the device and builds the LBVH.  infos[k] belongs
+ * to the triangle whose id == k (ninfos >= max id + 1).  bmin/bmax is the
+ * enclosing box (HostSceneDescription::enclosingBBox, HostSceneDescription.h:14);
+ * NULL computes it as the union of the triangle boxes.  settings may be NULL
+ * (reference defaults: 1 bounce, no RR, intensity 3, stratified). */
+int pt_upload_scene(pt_context* ctx, const pt_triangle* tris, uint32_t n,
+                    const pt_triangle_info* infos, uint32_t ninfos,
+                    const pt_dir_light* lights, uint32_t nlights,
+                    const float* bmin, const float* bmax,
+                    const pt_settings* settings);
+int pt_set_settings(pt_context* ctx, const pt_settings* settings);
+
+/* Rebuild the LBVH of the uploaded scene (BVHBuilder::build, BVHBuilder.cu:207).
+ * Asynchronous on the context stream; ms_out (may be NULL) receives the
+ * HIP-event time of the build kernels and forces a synchronisation. */
+int pt_build_bvh(pt_context* ctx, float* ms_out);
+
+/* Parity dump of the last build, in the reference's layout:
+ *   morton[n]             sorted 30-bit Morton keys (BVHBuilder.cu:229-234)
+ *   perm[n]               input position of the k-th sorted triangle
+ *   children[n-1]         Node words (Node.h:8-30): (left<<2)|leftInternal|rightInternal<<1
+ *   parents_leaf[n]       parents[0] (BVHBuilder.cu:180,192)
+ *   parents_internal[n-1] parents[1], [0] = PT_INVALID_INDEX (BVHBuilder.cu:224)
+ *   bboxes[6*(n-1)]       internal node boxes min.xyz,max.xyz (BVHBuilder.cu:86)
+ * Any pointer may be NULL. */
+int pt_get_bvh(pt_context* ctx, uint32_t* morton, uint32_t* perm, uint32_t* children,
+               uint32_t* parents_leaf, uint32_t* parents_internal, float* bboxes);
+int pt_get_scene_info(pt_context* ctx, uint32_t* ntris, float* bmin, float* bmax);
+
+/* ---- closest hit ---------------------------------------------------------- */
+/* Batch form of CUDA::trace (trace.h:73-169): rays[i] = origin.xyz, dir.xyz.
+ * Outputs (any may be NULL): id = original triangle id of the closest hit
+ * (PT_INVALID_INDEX on miss, equal distances -> lowest id), t (FLT_MAX on
+ * miss), uv = barycentrics of v1, v2 (math_util.h:19-25). */
+int pt_trace_closest(pt_context* ctx, const float* rays, uint32_t n,
+                     uint32_t* id, float* t, float* uv);
+int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n,
+                            uint32_t* d_id, float* d_t, float* d_uv, void* stream);
+/* Measurement variant (no reference counterpart): the same traversal, also
+ * counting inner-loop iterations, box tests and triangle tests into
+ * counters[3] (host, uint64).  Host-synchronous. */
+int pt_trace_closest_counted(pt_context* ctx, const float* d_rays, uint32_t n,
+                             uint32_t* d_id, float* d_t, float* d_uv,
+                             uint64_t* counters, void* stream);
+
+/* ---- progressive render --------------------------------------------------- */
+/* CUDAPathTracer::setPrimaryRays + setEyePos (cuda_path_tracer.cu:366-372,
+ * cuda_path_tracer.h:19).  dirs = 4 corner directions, 12 floats, in the
+ * reference order bottom-left, bottom-right, top-right, top-left. */
+int pt_set_primary_rays(pt_context* ctx, const float* dirs, const float* eye);
+/* CUDAPathTracer::raytrace (cuda_path_tracer.cu:423-464): one 1-spp pass over
+ * w x h pixels, added to the FP64 accumulator.  The accumulator is reset when
+ * w/h change.  pass_seed replaces the per-pass host random seed. */
+int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed);
+/* CUDAPathTracer::resetFrames (cuda_path_tracer.h:28): zero the accumulator. */
+int pt_reset_frames(pt_context* ctx);
+/* Accumulator access: rgb_sum[3*w*h] FP64 sums (row 0 = bottom), spp passes. */
+int pt_read_accumulator(pt_context* ctx, double* rgb_sum, uint32_t* spp);
+int pt_write_accumulator(pt_context* ctx, int w, int h, const double* rgb_sum, uint32_t spp);
+/* Running-mean image as the reference's RGBA32F surface holds it
+ * (cuda_path_tracer.cu:413-420): rgba[4*w*h] = sum/spp, alpha 1. */
+int pt_read_image(pt_context* ctx, float* rgba);
+/* Wall time in ms of the last pt_render_pass (HIP events). */
+int pt_last_render_ms(pt_context* ctx, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_API_H */

@@ -1,0 +1,88 @@
+/*
+ * oracle.h -- CPU restatement of the reference's hot path.  TEST
+ * INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so; the product (libptamd.so) never does.
+ *
+ * Every function follows the reference file:line named beside it.  Build
+ * contract decisions where the reference is unspecified or undefined:
+ *   - sort is stable (equal Morton keys keep ascending input order);
+ *   - the float->int quantisation saturates (NaN -> 0), as CUDA's cvt.rzi does;
+ *   - equal-distance hits resolve to the lowest original id, which needs
+ *     non-strict culling (t <= best) where trace.h:119,134 use strict tests;
+ *   - n == 0 builds an empty BVH (every ray misses); n == 1 has no internal
+ *     node and the single triangle is tested directly.
+ * Pinning of each function (which test checks it against what) is listed in
+ * tests/test_oracle_*.py headers.
+ */
+#ifndef PT_ORACLE_H
+#define PT_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/pt_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a1: Morton keys (BVHBuilder.cu:12-32, math_util.h:10-13, morton.h:84-102). */
+void or_morton(const pt_triangle* tris, uint32_t n, const float* bmin, const float* bmax,
+               uint32_t* keys);
+uint32_t or_encode30(uint32_t x, uint32_t y, uint32_t z);
+uint32_t or_quantize(float x, uint32_t levels);
+
+/* a2: stable key sort (BVHBuilder.cu:234 with the stability contract). */
+void or_sort(const uint32_t* keys, uint32_t n, uint32_t* sorted_keys, uint32_t* perm);
+
+/* a3: Karras radix tree (BVHBuilder.cu:107-201, parents[1][0] from :224). */
+void or_radix_tree(const uint32_t* sorted_keys, uint32_t n, uint32_t* children,
+                   uint32_t* parents_leaf, uint32_t* parents_internal);
+
+/* a4: bottom-up refit (BVHBuilder.cu:39-98, BBox.cpp:4-36). boxes[6*(n-1)]. */
+void or_refit(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+              const uint32_t* parents_leaf, const uint32_t* parents_internal, uint32_t n,
+              float* boxes);
+
+/* a5: whole build (BVHBuilder.cu:207-269).  Returns 0. */
+int or_build(const pt_triangle* tris, uint32_t n, const float* bmin, const float* bmax,
+             uint32_t* sorted_keys, uint32_t* perm, uint32_t* children,
+             uint32_t* parents_leaf, uint32_t* parents_internal, float* boxes);
+
+/* a8 / a9 single tests, for known-answer tests. */
+int or_ray_triangle(const float* o, const float* d, const float* p0, const float* p1,
+                    const float* p2, float* u, float* v, float* t);
+int or_ray_aabb(const float* o, const float* d, const float* bmin, const float* bmax,
+                float* t);
+
+/* a7: closest hit in the reference's traversal order (trace.h:73-169) with the
+ * lowest-id tie rule.  Outputs may be NULL.  counters (may be NULL) receives
+ * 3 uint64 totals: inner-loop iterations, box tests, triangle tests. */
+void or_trace(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+              const float* boxes, uint32_t n, const float* rays, uint32_t nrays,
+              uint32_t* id, float* t, float* uv, uint64_t* counters, int threads);
+
+/* Brute force over all triangles with the same Moller-Trumbore test and the
+ * same tie rule (no BVH). */
+void or_brute(const pt_triangle* tris, uint32_t n, const float* rays, uint32_t nrays,
+              uint32_t* id, float* t, float* uv, int threads);
+
+/* a11/a12/a15: one progressive 1-spp pass over w x h pixels
+ * (cuda_path_tracer.cu:245-333, 385-421), added into accum[3*w*h] (FP64).
+ * infos are indexed by original triangle id.  seg_counts (may be NULL)
+ * receives the number of traced segments. */
+void or_render_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                    const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                    const pt_settings* settings, int w, int h, const float* dirs,
+                    const float* eye, uint32_t pass, uint64_t seed, double* accum,
+                    uint64_t* seg_counts, int threads);
+
+/* Sampling helpers shared by the render, exposed for tests. */
+void or_pcg4d(const uint32_t* in, uint32_t* out);
+void or_sincos2pi(float u, float* s, float* c);
+void or_to_hemisphere(const float* v, const float* n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_ORACLE_H */

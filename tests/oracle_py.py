@@ -1,0 +1,166 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU restatement of the
+reference hot path.  Test infrastructure: only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg load it."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"oracle not built ({LIB_PATH}); run __graft_entry__.build()")
+_lib = ctypes.CDLL(LIB_PATH)
+
+_vp, _u32, _u64, _i, _f = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, ctypes.c_float
+_SIGS = {
+    "or_morton": ([_vp, _u32, _vp, _vp, _vp], None),
+    "or_encode30": ([_u32, _u32, _u32], _u32),
+    "or_quantize": ([_f, _u32], _u32),
+    "or_sort": ([_vp, _u32, _vp, _vp], None),
+    "or_radix_tree": ([_vp, _u32, _vp, _vp, _vp], None),
+    "or_refit": ([_vp, _vp, _vp, _vp, _vp, _u32, _vp], None),
+    "or_build": ([_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "or_ray_triangle": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "or_ray_aabb": ([_vp, _vp, _vp, _vp, _vp], _i),
+    "or_trace": ([_vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _i], None),
+    "or_brute": ([_vp, _u32, _vp, _u32, _vp, _vp, _vp, _i], None),
+    "or_render_pass": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp, _vp, _i], None),
+    "or_pcg4d": ([_vp, _vp], None),
+    "or_sincos2pi": ([_f, _vp, _vp], None),
+    "or_to_hemisphere": ([_vp, _vp, _vp], None),
+}
+for _n, (_a, _r) in _SIGS.items():
+    _fn = getattr(_lib, _n)
+    _fn.argtypes = _a
+    _fn.restype = _r
+
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(x):
+    return np.ascontiguousarray(np.asarray(x, np.float32))
+
+
+def encode30(x, y, z):
+    return int(_lib.or_encode30(x, y, z))
+
+
+def quantize(x, levels=1024):
+    return int(_lib.or_quantize(float(np.float32(x)), levels))
+
+
+def morton(tris, bmin, bmax):
+    keys = np.zeros(len(tris), np.uint32)
+    bmin, bmax = _f32(bmin), _f32(bmax)
+    _lib.or_morton(_p(tris), len(tris), _p(bmin), _p(bmax), _p(keys))
+    return keys
+
+
+def sort(keys):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    sk = np.zeros_like(keys)
+    perm = np.zeros_like(keys)
+    _lib.or_sort(_p(keys), len(keys), _p(sk), _p(perm))
+    return sk, perm
+
+
+def radix_tree(sorted_keys):
+    sk = np.ascontiguousarray(sorted_keys, np.uint32)
+    n = len(sk)
+    m = max(n - 1, 0)
+    ch = np.zeros(m, np.uint32)
+    pl = np.zeros(n, np.uint32)
+    pi = np.zeros(m, np.uint32)
+    _lib.or_radix_tree(_p(sk), n, _p(ch), _p(pl), _p(pi))
+    return ch, pl, pi
+
+
+def build(tris, bmin, bmax):
+    n = len(tris)
+    m = max(n - 1, 0)
+    out = dict(morton=np.zeros(n, np.uint32), perm=np.zeros(n, np.uint32),
+               children=np.zeros(m, np.uint32), parents_leaf=np.zeros(n, np.uint32),
+               parents_internal=np.zeros(m, np.uint32), bboxes=np.zeros((m, 6), np.float32))
+    bmin, bmax = _f32(bmin), _f32(bmax)
+    _lib.or_build(_p(tris), n, _p(bmin), _p(bmax), _p(out["morton"]), _p(out["perm"]),
+                  _p(out["children"]), _p(out["parents_leaf"]), _p(out["parents_internal"]),
+                  _p(out["bboxes"]))
+    return out
+
+
+def ray_triangle(o, d, p0, p1, p2):
+    u = np.zeros(1, np.float32)
+    v = np.zeros(1, np.float32)
+    t = np.zeros(1, np.float32)
+    o, d, p0, p1, p2 = _f32(o), _f32(d), _f32(p0), _f32(p1), _f32(p2)
+    hit = _lib.or_ray_triangle(_p(o), _p(d), _p(p0), _p(p1), _p(p2), _p(u), _p(v), _p(t))
+    return bool(hit), float(u[0]), float(v[0]), float(t[0])
+
+
+def ray_aabb(o, d, bmin, bmax):
+    t = np.zeros(1, np.float32)
+    o, d, bmin, bmax = _f32(o), _f32(d), _f32(bmin), _f32(bmax)
+    hit = _lib.or_ray_aabb(_p(o), _p(d), _p(bmin), _p(bmax), _p(t))
+    return bool(hit), float(t[0])
+
+
+def trace(tris, bvh, rays, counters=False, threads=THREADS):
+    rays = _f32(rays).reshape(-1, 6)
+    n = len(rays)
+    ids = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
+    uv = np.zeros((n, 2), np.float32)
+    c = np.zeros(3, np.uint64) if counters else None
+    _lib.or_trace(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
+                  _p(rays), n, _p(ids), _p(t), _p(uv), _p(c), threads)
+    return (ids, t, uv, c) if counters else (ids, t, uv)
+
+
+def brute(tris, rays, threads=THREADS):
+    rays = _f32(rays).reshape(-1, 6)
+    n = len(rays)
+    ids = np.zeros(n, np.uint32)
+    t = np.zeros(n, np.float32)
+    uv = np.zeros((n, 2), np.float32)
+    _lib.or_brute(_p(tris), len(tris), _p(rays), n, _p(ids), _p(t), _p(uv), threads)
+    return ids, t, uv
+
+
+def render_pass(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, seed, accum=None,
+                threads=THREADS):
+    if accum is None:
+        accum = np.zeros((h, w, 3), np.float64)
+    segs = np.zeros(1, np.uint64)
+    dirs, eye = _f32(dirs), _f32(eye)
+    _lib.or_render_pass(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
+                        _p(infos), ctypes.byref(settings), w, h, _p(dirs), _p(eye),
+                        pass_index, seed, _p(accum), _p(segs), threads)
+    return accum, int(segs[0])
+
+
+def pcg4d(v):
+    a = np.ascontiguousarray(np.asarray(v, np.uint32))
+    out = np.zeros(4, np.uint32)
+    _lib.or_pcg4d(_p(a), _p(out))
+    return out
+
+
+def sincos2pi(u):
+    s = np.zeros(1, np.float32)
+    c = np.zeros(1, np.float32)
+    _lib.or_sincos2pi(float(np.float32(u)), _p(s), _p(c))
+    return float(s[0]), float(c[0])
+
+
+def to_hemisphere(v, n):
+    out = np.zeros(3, np.float32)
+    v, n = _f32(v), _f32(n)
+    _lib.or_to_hemisphere(_p(v), _p(n), _p(out))
+    return out

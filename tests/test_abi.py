@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every symbol the public headers
+declare (CPU only; no compute calls)."""
+import ctypes
+import os
+import re
+
+import ptamd
+from ptamd import hip
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in ("pt_api.h", "pt_scenes.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"^int\s+(pt_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared()
+    assert len(names) >= 25
+    lib = ctypes.CDLL(ptamd.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert names <= set(ptamd.EXPORTED)
+
+
+def test_struct_layouts():
+    assert ptamd.TRI_DTYPE.itemsize == 40
+    assert ptamd.INFO_DTYPE.itemsize == 64
+    assert ctypes.sizeof(ptamd.Settings) == 16
+
+
+def test_no_device_is_a_status_code():
+    if hip.device_count() > 0:
+        return
+    try:
+        ptamd.Context(0)
+    except ptamd.PtError as e:
+        assert e.code == ptamd.PT_E_NO_DEVICE
+    else:
+        raise AssertionError("pt_create succeeded without a device")
+
+
+def test_null_arguments_rejected():
+    assert ptamd._lib.pt_upload_scene(None, None, 0, None, 0, None, 0, None, None, None) == ptamd.PT_E_INVALID_ARG
+    assert ptamd._lib.pt_scene_bounds(None, 0, None, None) == ptamd.PT_E_INVALID_ARG

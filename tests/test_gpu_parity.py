@@ -1,0 +1,253 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the
+same seeded inputs.
+
+Bar (bit-exact unless stated):
+  build   Morton keys, sort permutation, children, parents[0], parents[1] and
+          internal node boxes (compared as bit patterns)
+  trace   closest original id, t and barycentrics bit-identical to the oracle's
+          reference-order traversal (the contract allows t within 1e-6
+          relative; checked too), visit counters identical
+  render  FP64 accumulators at equal seeds and passes: mean relative
+          per-channel error <= 1e-3 (the contract), and reported exactness
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+import ptamd
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check_build(ctx, tris, infos, bmin=None, bmax=None, tag=""):
+    say(f"[build parity] {tag} n={len(tris)}")
+    ctx.upload_scene(tris, infos, bmin, bmax)
+    g = ctx.get_bvh()
+    if bmin is None:
+        bmin, bmax = ptamd.scene_bounds(tris)
+    o = orc.build(tris, bmin, bmax)
+    for k in ("morton", "perm", "children", "parents_leaf", "parents_internal"):
+        assert np.array_equal(g[k], o[k]), f"{tag}: {k} differs"
+    assert np.array_equal(g["bboxes"].view(np.uint32), o["bboxes"].view(np.uint32)), f"{tag}: boxes differ"
+    return o
+
+
+def soup_dups(n):
+    tris, infos = ptamd.gen_soup(n, 31, emitter=False)
+    tris["v0"] = tris["v0"][0]
+    tris["v1"] = tris["v1"][0]
+    tris["v2"] = tris["v2"][0]
+    return tris, infos
+
+
+def soup_flat(n):
+    tris, infos = ptamd.gen_soup(n, 32, emitter=False)
+    for f in ("v0", "v1", "v2"):
+        tris[f][:, 1] = np.float32(0.1)
+    return tris, infos
+
+
+BUILD_CASES = {
+    "soup2": lambda: ptamd.gen_soup(2, 1, emitter=False),
+    "soup3": lambda: ptamd.gen_soup(3, 1, emitter=False),
+    "soup100": lambda: ptamd.gen_soup(98, 2),
+    "soup4095": lambda: ptamd.gen_soup(4095, 3, emitter=False),
+    "soup4096": lambda: ptamd.gen_soup(4096, 3, emitter=False),
+    "soup4097": lambda: ptamd.gen_soup(4097, 3, emitter=False),
+    "c0_soup": lambda: ptamd.gen_soup(65536, 0),
+    "soup100k": lambda: ptamd.gen_soup(100003, 4),
+    "dups": lambda: soup_dups(5000),
+    "flat": lambda: soup_flat(5000),
+    "sphere64": lambda: ptamd.gen_sphere(64, 5, room=True),
+    "hair": lambda: ptamd.gen_hairball(200, 60, 6, light=True),
+}
+
+
+@pytest.mark.parametrize("case", list(BUILD_CASES))
+def test_build_parity(ctx, case):
+    tris, infos = BUILD_CASES[case]()
+    check_build(ctx, tris, infos, tag=case)
+
+
+def test_build_single_and_empty(ctx):
+    tris, infos = ptamd.gen_soup(1, 9, emitter=False)
+    say("[build parity] n=1")
+    ctx.upload_scene(tris, infos)
+    g = ctx.get_bvh()
+    assert g["parents_leaf"][0] == ptamd.INVALID and g["perm"][0] == 0
+    rays = ptamd.gen_shell_rays(4000, 1, *ptamd.scene_bounds(tris), 1.2)
+    ids, t, uv = ctx.trace_closest(rays)
+    b = orc.build(tris, *ptamd.scene_bounds(tris))
+    oids, ot, ouv = orc.trace(tris, b, rays)
+    assert np.array_equal(ids, oids) and np.array_equal(t, ot)
+    say("[trace] n=0")
+    ctx.upload_scene(tris[:0], infos[:0], (0, 0, 0), (1, 1, 1))
+    ids, t, uv = ctx.trace_closest(rays)
+    assert np.all(ids == ptamd.INVALID) and np.all(t == ptamd.FLT_MAX)
+
+
+def test_build_given_bounds(ctx):
+    # a caller box larger than the scene changes the keys; parity must hold
+    tris, infos = ptamd.gen_soup(20000, 12)
+    check_build(ctx, tris, infos, np.array([-1, -2, -3], np.float32),
+                np.array([2, 3, 4], np.float32), tag="given-bounds")
+
+
+def test_repeated_builds_identical(ctx):
+    tris, infos = ptamd.gen_soup(1 << 20, 13, emitter=False)
+    say("[repeat builds] n=2^20")
+    ctx.upload_scene(tris, infos)
+    first = ctx.get_bvh()
+    for _ in range(4):
+        ctx.build_bvh()
+        again = ctx.get_bvh()
+        for k in first:
+            assert np.array_equal(first[k].view(np.uint32), again[k].view(np.uint32)), k
+
+
+def interior_rays(n, seed, bmin, bmax):
+    rng = np.random.default_rng(seed)
+    o = bmin + (bmax - bmin) * rng.random((n, 3), dtype=np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], axis=1).astype(np.float32)
+
+
+def axis_rays(n, seed, bmin, bmax):
+    r = interior_rays(n, seed, bmin, bmax)
+    rng = np.random.default_rng(seed + 1)
+    k = rng.integers(0, 3, n)
+    r[np.arange(n), 3 + k] = 0.0
+    two = rng.random(n) < 0.5
+    r[two, 3:] = 0.0
+    r[two, 3 + k[two]] = 1.0
+    return r
+
+
+def check_trace(ctx, tris, infos, rays, tag):
+    say(f"[trace parity] {tag} n={len(tris)} rays={len(rays)}")
+    o = check_build(ctx, tris, infos, tag=tag)
+    ids, t, uv = ctx.trace_closest(rays)
+    oids, ot, ouv, oc = orc.trace(tris, o, rays, counters=True)
+    assert np.array_equal(ids, oids), f"{tag}: {np.count_nonzero(ids != oids)} ids differ"
+    hit = oids != ptamd.INVALID
+    assert np.all(np.abs(t[hit] - ot[hit]) <= 1e-6 * np.abs(ot[hit]))
+    assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(uv.view(np.uint32), ouv.view(np.uint32))
+    # visit counters of the GPU traversal equal the oracle's
+    from ptamd import hip
+    d_rays = hip.DeviceArray.from_host(rays)
+    c = ctx.trace_closest_counted(d_rays.ptr, len(rays))
+    d_rays.free()
+    assert np.array_equal(c, oc), (c, oc)
+    return hit.mean()
+
+
+TRACE_CASES = {
+    "c0_soup": lambda: ptamd.gen_soup(65536, 0),
+    "sphere128": lambda: ptamd.gen_sphere(128, 1, room=True),
+    "hair": lambda: ptamd.gen_hairball(300, 144, 2, light=False),
+    "dups": lambda: soup_dups(3000),
+}
+
+
+@pytest.mark.parametrize("case", list(TRACE_CASES))
+def test_trace_parity(ctx, case):
+    tris, infos = TRACE_CASES[case]()
+    bmin, bmax = ptamd.scene_bounds(tris)
+    rays = np.concatenate([ptamd.gen_shell_rays(60000, 3, bmin, bmax, 1.5),
+                           interior_rays(60000, 4, bmin, bmax), axis_rays(8000, 5, bmin, bmax)])
+    check_trace(ctx, tris, infos, rays, case)
+
+
+def test_trace_parity_c1_full_mesh(ctx):
+    # config 1 microbench mesh at full size: 871,200 triangles
+    tris, infos = ptamd.gen_sphere(660, 2024, room=False)
+    bmin, bmax = ptamd.scene_bounds(tris)
+    rays = ptamd.gen_shell_rays(400000, 77, bmin, bmax, 1.5)
+    rate = check_trace(ctx, tris, infos, rays, "c1-mesh")
+    assert rate > 0.05
+
+
+def test_build_parity_c2_full_hairball(ctx):
+    tris, infos = ptamd.gen_hairball(10000, 144, 2025, light=False)
+    assert len(tris) == 2_880_000
+    o = check_build(ctx, tris, infos, tag="c2-hair")
+    bmin, bmax = ptamd.scene_bounds(tris)
+    assert np.array_equal(o["bboxes"][0], np.concatenate([bmin, bmax]))
+
+
+def render_both(ctx, tris, infos, st, w, h, passes, eye, target, seed=11):
+    ctx.upload_scene(tris, infos, settings=st)
+    bmin, bmax = ptamd.scene_bounds(tris)
+    b = orc.build(tris, bmin, bmax)
+    oacc = np.zeros((h, w, 3), np.float64)
+    for p in range(passes):
+        jx, jy = ptamd.pass_jitter(seed, p, w, h)
+        dirs = ptamd.camera_corners(eye, target, (0, 1, 0), 60.0, w / h, jx, jy)
+        ctx.set_primary_rays(dirs, eye)
+        ctx.render_pass(w, h, seed + p)
+        orc.render_pass(tris, b, infos, st, w, h, dirs, eye, p, seed + p, oacc)
+    gacc, spp = ctx.read_accumulator(w, h)
+    assert spp == passes
+    return gacc, oacc
+
+
+def rel_err(g, o):
+    return [float(np.abs(g[..., c] - o[..., c]).sum() / max(np.abs(o[..., c]).sum(), 1e-30)) for c in range(3)]
+
+
+def test_render_parity_c0(ctx):
+    say("[render parity] c0 soup 96x96 x4 passes, 4 bounces")
+    tris, infos = ptamd.gen_soup(65536, 0)
+    st = ptamd.Settings(4, ptamd.INVALID, 1.0, 1)
+    g, o = render_both(ctx, tris, infos, st, 96, 96, 4, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0))
+    assert o.sum() > 0
+    err = rel_err(g, o)
+    say(f"  mean relative per-channel error {err}, exact={np.array_equal(g, o)}")
+    assert max(err) <= 1e-3
+    assert np.array_equal(g, o)
+
+
+def test_render_parity_sphere_room_rr(ctx):
+    say("[render parity] sphere room 64x48 x3 passes, 8 bounces, RR from 3")
+    tris, infos = ptamd.gen_sphere(48, 3, room=True)
+    st = ptamd.Settings(8, 3, 1.0, 1)
+    g, o = render_both(ctx, tris, infos, st, 64, 48, 3, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))
+    assert o.sum() > 0
+    err = rel_err(g, o)
+    say(f"  mean relative per-channel error {err}, exact={np.array_equal(g, o)}")
+    assert max(err) <= 1e-3
+
+
+def test_accumulator_roundtrip_and_image(ctx):
+    say("[accumulator] write/read/image")
+    tris, infos = ptamd.gen_soup(5000, 1)
+    ctx.upload_scene(tris, infos)
+    acc = np.random.default_rng(0).random((20, 30, 3))
+    ctx.write_accumulator(acc, 4)
+    back, spp = ctx.read_accumulator(30, 20)
+    assert spp == 4 and np.array_equal(back, acc)
+    img = ctx.read_image(30, 20)
+    assert np.array_equal(img[..., :3], (acc / 4).astype(np.float32)) and np.all(img[..., 3] == 1)
+    ctx.reset_frames()
+    back, spp = ctx.read_accumulator(30, 20)
+    assert spp == 0 and not back.any()
+
+
+def test_errors_are_status_codes(ctx):
+    say("[errors]")
+    tris, infos = ptamd.gen_soup(100, 1)
+    with pytest.raises(ptamd.PtError) as e:
+        ctx.upload_scene(tris, infos[:10])  # missing shading records
+    assert e.value.code == ptamd.PT_E_INVALID_ARG
+    with pytest.raises(ptamd.PtError) as e:
+        ptamd.Context(99)
+    assert e.value.code == ptamd.PT_E_NO_DEVICE
