@@ -256,12 +256,12 @@ int pt_gen_sphere(uint64_t seed, uint32_t bands, int with_room, pt_triangle* tri
                    {-s, -s, s},  {s, -s, s},  {s, s, s},  {-s, s, s}};
         // Faces as quads (a, b, c, d) wound so that the normal points inward.
         const int q[6][4] = {
-            {0, 1, 5, 4},  // floor y=-2, normal +y
-            {3, 7, 6, 2},  // ceiling y=+2, normal -y
-            {0, 4, 7, 3},  // x=-2, normal +x
-            {1, 2, 6, 5},  // x=+2, normal -x
-            {0, 3, 2, 1},  // z=-2, normal +z
-            {4, 5, 6, 7},  // z=+2, normal -z
+            {0, 4, 5, 1},  // floor y=-2, normal +y
+            {3, 2, 6, 7},  // ceiling y=+2, normal -y
+            {0, 3, 7, 4},  // x=-2, normal +x
+            {1, 5, 6, 2},  // x=+2, normal -x
+            {0, 1, 2, 3},  // z=-2, normal +z
+            {4, 7, 6, 5},  // z=+2, normal -z
         };
         for (int f = 0; f < 6; ++f) {
             V3 a = p[q[f][0]], b = p[q[f][1]], c = p[q[f][2]], d = p[q[f][3]];
