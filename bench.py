@@ -101,6 +101,32 @@ def main():
     build_ms_1m = float(np.median([ctx.build_bvh() for _ in range(10)]))
     log(f"[bench] LBVH build: {build_ms_1m:.3f} ms @ 2^20 soup, {build_ms_mesh:.3f} ms @ 871,200 mesh")
 
+    # Progressive render legs (spp/s): config 0 and config 1 scenes.
+    render = {}
+    for name, (rt, ri), st, (w, h), passes, eye, tgt in (
+        ("c0_soup_256x256_b4", ptamd.gen_soup(65536, 0), ptamd.Settings(4, ptamd.INVALID, 1.0, 1),
+         (256, 256), 16, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)),
+        ("c1_room_1024x1024_b8_rr3", ptamd.gen_sphere(660, seed_mesh, room=True), ptamd.Settings(8, 3, 1.0, 1),
+         (1024, 1024), 4 if args.profile else 8, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))):
+        ctx.upload_scene(rt, ri, settings=st)
+        dirs = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h)
+        ctx.set_primary_rays(dirs, eye)
+        ctx.render_pass(w, h, 1)  # warm-up pass (allocates queues)
+        ctx.reset_frames()
+        stream.synchronize()
+        r0, r1 = hip.Event(), hip.Event()
+        r0.record(stream)
+        for p in range(passes):
+            jx, jy = ptamd.pass_jitter(5, p, w, h)
+            ctx.set_primary_rays(ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, jx, jy), eye)
+            ctx.render_pass(w, h, 1000 + p)
+        r1.record(stream)
+        r1.synchronize()
+        rms = r0.elapsed_ms(r1)
+        render[name] = {"ms_per_pass": round(rms / passes, 3), "spp_per_s": round(passes / (rms * 1e-3), 2),
+                        "Msamples_per_s": round(w * h * passes / (rms * 1e-3) / 1e6, 2), "passes": passes}
+        log(f"[bench] render {name}: {rms / passes:.3f} ms/pass")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -142,6 +168,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
         "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
+        "render": render,
         "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),
                       "tri_tests_per_ray": round(tri_tests / n, 3),
                       "algorithmic_bytes_per_ray": round(bytes_per_launch / n, 1), "hit_rate": round(hit_rate, 4)},

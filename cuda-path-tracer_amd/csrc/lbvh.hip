@@ -5,10 +5,12 @@
 //   k_sort_hist     \
 //   k_sort_rowscan   } x4: LSD radix sort, 8-bit digits, stable
 //   k_sort_scatter  /     (wave64 ballot match + LDS staging per 4096-key tile)
-//   k_radix_tree    Karras split per internal node, parents, counter reset
-//   k_refit         sorted triangle records + bottom-up AABBs (agent-scope
-//                   acquire/release counters), written straight into the
-//                   traversal node records
+//   k_radix_tree    Karras split per internal node, parents, counter reset,
+//                   list of nodes whose leaf range crosses a refit block
+//   k_refit_local   sorted triangle records + bottom-up AABBs inside each
+//                   1024-leaf block (LDS counters/boxes, workgroup scope)
+//   k_refit_top     the remaining block-spanning nodes, one workgroup
+// Boxes are written straight into the parents' traversal node records.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +25,11 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kSortKeysPerThread = 16;
 constexpr int kSortTile = kSortThreads * kSortKeysPerThread;  // 4096 keys
+// Refit leaf block: nodes whose leaf range lies inside one block of this many
+// sorted leaves are refit inside one workgroup (LDS counters and boxes); the
+// "spanning" nodes above them are finished by a single workgroup afterwards.
+constexpr int kRefitBlock = 1024;
+constexpr uint32_t kSpanFlag = 0x80000000u;
 
 // ---- Morton keys ------------------------------------------------------------
 
@@ -49,8 +56,10 @@ __device__ __forceinline__ uint32_t quantize1024(float x) {
 __global__ __launch_bounds__(256) void k_morton(const pt_triangle* __restrict__ tris, uint32_t n,
                                                 float mnx, float mny, float mnz, float ivx,
                                                 float ivy, float ivz, uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
+                                                uint32_t* __restrict__ vals,
+                                                uint32_t* __restrict__ span_count) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *span_count = 0u;
     if (i >= n) return;
     const float* t = reinterpret_cast<const float*>(tris + i);
     float cx = ((t[0] + t[3]) + t[6]) / 3.0f;
@@ -228,13 +237,17 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
 }
 
 // radix_tree_construction_kernel (BVHBuilder.cu:135-196), one thread per
-// internal node; also clears the node's refit counter.
+// internal node.  Also initialises the node's refit counter: bit 31 marks a
+// node whose leaf range [min(i,j), max(i,j)] crosses a refit block boundary,
+// and such nodes are appended to span_list.
 __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__ keys, uint32_t n,
                                                     uint32_t* __restrict__ children,
                                                     uint32_t* __restrict__ parents_leaf,
                                                     uint32_t* __restrict__ parents_internal,
                                                     uint32_t* __restrict__ counters,
-                                                    float4* __restrict__ nodes) {
+                                                    float4* __restrict__ nodes,
+                                                    uint32_t* __restrict__ span_list,
+                                                    uint32_t* __restrict__ span_count) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t N = n;
     if (i >= N - 1) return;
@@ -273,7 +286,10 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
     const uint32_t w = (static_cast<uint32_t>(s) << 2) | type;
     children[i] = w;
     nodes[4 * i + 3] = make_float4(__uint_as_float(w), 0.0f, 0.0f, 0.0f);
-    counters[i] = 0u;
+    const int64_t lo = i < j ? i : j, hi = i > j ? i : j;
+    const bool spanning = (lo / kRefitBlock) != (hi / kRefitBlock);
+    counters[i] = spanning ? kSpanFlag : 0u;
+    if (spanning) span_list[atomicAdd(span_count, 1u)] = static_cast<uint32_t>(i);
     if (i == 0) parents_internal[0] = kInvalid;  // BVHBuilder.cu:224
 }
 
@@ -294,19 +310,58 @@ __device__ __forceinline__ void store_slot(float4* nodes, uint32_t p, bool left,
     }
 }
 
-// compute_internal_node_bbox_kernel (BVHBuilder.cu:54-91), one thread per
-// sorted leaf.  The leaf also writes its sorted triangle record (the
-// reference instead permutes the 40-byte triangles inside the sort).  The
-// second thread to reach a node unites the two child boxes: the counter
-// increment is acquire-release at agent scope so the sibling's box stores,
-// made before its own increment, are visible across XCDs.
-__global__ __launch_bounds__(256) void k_refit(
+__device__ __forceinline__ Box unite(const Box& a, const Box& b) {
+    // leftBBox.unite(rightBBox) (BVHBuilder.cu:85, BBox.cpp:30-36).
+    Box o;
+    o.mnx = gmin(a.mnx, b.mnx);
+    o.mxx = gmax(a.mxx, b.mxx);
+    o.mny = gmin(a.mny, b.mny);
+    o.mxy = gmax(a.mxy, b.mxy);
+    o.mnz = gmin(a.mnz, b.mnz);
+    o.mxz = gmax(a.mxz, b.mxz);
+    return o;
+}
+
+__device__ __forceinline__ void store_root(float* root_box, const Box& b) {
+    root_box[0] = b.mnx; root_box[1] = b.mny; root_box[2] = b.mnz;
+    root_box[3] = b.mxx; root_box[4] = b.mxy; root_box[5] = b.mxz;
+}
+
+// LDS box array in structure-of-arrays form (one float plane per bound).
+struct LdsBoxes {
+    float v[6][kRefitBlock];
+    __device__ __forceinline__ void put(int k, const Box& b) {
+        v[0][k] = b.mnx; v[1][k] = b.mxx; v[2][k] = b.mny;
+        v[3][k] = b.mxy; v[4][k] = b.mnz; v[5][k] = b.mxz;
+    }
+    __device__ __forceinline__ Box get(int k) const {
+        return Box{v[0][k], v[1][k], v[2][k], v[3][k], v[4][k], v[5][k]};
+    }
+};
+
+// Refit, phase 1 (compute_internal_node_bbox_kernel, BVHBuilder.cu:54-91):
+// one thread per sorted leaf, one workgroup per block of kRefitBlock leaves.
+// The leaf writes its sorted triangle record (the reference instead permutes
+// the 40-byte triangles inside the sort), then climbs: the first thread to
+// reach a node stops, the second unites the two child boxes.  Every node
+// below the spanning ones has its whole leaf range -- hence both children --
+// inside the block, so counters and boxes live in LDS and the hand-off needs
+// only workgroup-scope ordering.  Each finished box is also stored into the
+// parent's traversal record.
+__global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     const pt_triangle* __restrict__ in_tris, const uint32_t* __restrict__ perm,
     const uint32_t* __restrict__ children, uint32_t* __restrict__ parents_leaf,
-    const uint32_t* __restrict__ parents_internal, uint32_t* __restrict__ counters,
+    const uint32_t* __restrict__ parents_internal, const uint32_t* __restrict__ counters,
     float4* __restrict__ nodes, float4* __restrict__ tri_recs, float* __restrict__ root_box,
     uint32_t n) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_cnt[kRefitBlock];
+    __shared__ LdsBoxes s_leaf;
+    __shared__ LdsBoxes s_node;
+    const int tid = threadIdx.x;
+    const uint32_t base = blockIdx.x * kRefitBlock;
+    const uint32_t k = base + tid;
+    s_cnt[tid] = 0u;
+    __syncthreads();
     if (k >= n) return;
     const pt_triangle* t = in_tris + perm[k];
     const V3 v0 = v3(t->v0[0], t->v0[1], t->v0[2]);
@@ -327,33 +382,76 @@ __global__ __launch_bounds__(256) void k_refit(
     b.mxz = gmax(gmax(gmax(-kFltMax, v0.z), v1.z), v2.z);
     if (n == 1) {
         parents_leaf[0] = kInvalid;
-        root_box[0] = b.mnx; root_box[1] = b.mny; root_box[2] = b.mnz;
-        root_box[3] = b.mxx; root_box[4] = b.mxy; root_box[5] = b.mxz;
+        store_root(root_box, b);
         return;
     }
+    s_leaf.put(tid, b);
     uint32_t p = parents_leaf[k];
-    store_slot(nodes, p, (children[p] >> 2) == k, b);
+    if (counters[p] & kSpanFlag) {
+        store_slot(nodes, p, (children[p] >> 2) == k, b);  // finished by k_refit_top
+        return;
+    }
     for (;;) {
-        const uint32_t old =
-            __hip_atomic_fetch_add(counters + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t old = __hip_atomic_fetch_add(s_cnt + (p - base), 1u, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0u) return;
-        const float4* rec = nodes + 4 * static_cast<size_t>(p);
-        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-        // leftBBox.unite(rightBBox) (BVHBuilder.cu:85, BBox.cpp:30-36).
-        b.mnx = gmin(r0.x, r1.z);
-        b.mxx = gmax(r0.y, r1.w);
-        b.mny = gmin(r0.z, r2.x);
-        b.mxy = gmax(r0.w, r2.y);
-        b.mnz = gmin(r1.x, r2.z);
-        b.mxz = gmax(r1.y, r2.w);
+        const uint32_t w = children[p];
+        const uint32_t l = w >> 2, r = l + 1;
+        const Box lb = (w & 1u) ? s_node.get(l - base) : s_leaf.get(l - base);
+        const Box rb = (w & 2u) ? s_node.get(r - base) : s_leaf.get(r - base);
+        b = unite(lb, rb);
+        s_node.put(p - base, b);
         const uint32_t pp = parents_internal[p];
         if (pp == kInvalid) {
-            root_box[0] = b.mnx; root_box[1] = b.mny; root_box[2] = b.mnz;
-            root_box[3] = b.mxx; root_box[4] = b.mxy; root_box[5] = b.mxz;
+            store_root(root_box, b);
             return;
         }
         store_slot(nodes, pp, (children[pp] >> 2) == p, b);
+        if (counters[pp] & kSpanFlag) return;
         p = pp;
+    }
+}
+
+// Refit, phase 2: the spanning nodes (leaf range across refit blocks), all in
+// one workgroup so the counter hand-off is workgroup-scope again.  Every child
+// box of a node sits in the node's own record: local children were stored by
+// phase 1, spanning children are stored here before their parent's counter
+// is incremented.  Each local child of a spanning node is one "arrival".
+__global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ parents_internal,
+                                                    uint32_t* __restrict__ counters,
+                                                    float4* __restrict__ nodes,
+                                                    float* __restrict__ root_box,
+                                                    const uint32_t* __restrict__ span_list,
+                                                    const uint32_t* __restrict__ span_count) {
+    const uint32_t count = *span_count;
+    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
+        const uint32_t s = span_list[idx];
+        const uint32_t w = children[s];
+        const uint32_t l = w >> 2, r = l + 1;
+        int arrivals = 0;
+        if (!(w & 1u) || !(counters[l] & kSpanFlag)) ++arrivals;
+        if (!(w & 2u) || !(counters[r] & kSpanFlag)) ++arrivals;
+        for (int a = 0; a < arrivals; ++a) {
+            uint32_t p = s;
+            for (;;) {
+                const uint32_t old = __hip_atomic_fetch_add(counters + p, 1u, __ATOMIC_ACQ_REL,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                if ((old & ~kSpanFlag) == 0u) break;
+                const float4* rec = nodes + 4 * static_cast<size_t>(p);
+                const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+                const Box lb{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y};
+                const Box rb{r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+                const Box b = unite(lb, rb);
+                const uint32_t pp = parents_internal[p];
+                if (pp == kInvalid) {
+                    store_root(root_box, b);
+                    break;
+                }
+                store_slot(nodes, pp, (children[pp] >> 2) == p, b);
+                p = pp;
+            }
+        }
     }
 }
 
@@ -407,13 +505,16 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     const uint32_t num_tiles = div_up(n, kSortTile);
     b.tile_hist.reserve(static_cast<size_t>(num_tiles) * 256 * 4);
     b.digit_total.reserve(256 * 4);
+    b.span_list.reserve(m * 4);
+    b.span_count.reserve(4);
 
     // invScale = 1 / scale on the host (BVHBuilder.cu:15-17).
     float inv[3];
     for (int k = 0; k < 3; ++k) inv[k] = 1.0f / (s.bmax[k] - s.bmin[k]);
     hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, st,
                        s.tris.as<pt_triangle>(), n, s.bmin[0], s.bmin[1], s.bmin[2], inv[0],
-                       inv[1], inv[2], b.keys.as<uint32_t>(), b.vals.as<uint32_t>());
+                       inv[1], inv[2], b.keys.as<uint32_t>(), b.vals.as<uint32_t>(),
+                       b.span_count.as<uint32_t>());
     PT_HIP(hipGetLastError());
 
     if (n >= 2) {
@@ -437,14 +538,21 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
         hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n - 1, 256)), dim3(256), 0, st,
                            b.keys.as<uint32_t>(), n, b.children.as<uint32_t>(),
                            b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
-                           b.counters.as<uint32_t>(), b.nodes.as<float4>());
+                           b.counters.as<uint32_t>(), b.nodes.as<float4>(),
+                           b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>());
         PT_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_refit, dim3(div_up(n, 256)), dim3(256), 0, st, s.tris.as<pt_triangle>(),
-                       b.vals.as<uint32_t>(), b.children.as<uint32_t>(),
+    hipLaunchKernelGGL(k_refit_local, dim3(div_up(n, kRefitBlock)), dim3(kRefitBlock), 0, st,
+                       s.tris.as<pt_triangle>(), b.vals.as<uint32_t>(), b.children.as<uint32_t>(),
                        b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                        b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.tris.as<float4>(),
                        b.root_box.as<float>(), n);
+    if (n > static_cast<uint32_t>(kRefitBlock)) {
+        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
+                           b.parents_internal.as<uint32_t>(), b.counters.as<uint32_t>(),
+                           b.nodes.as<float4>(), b.root_box.as<float>(),
+                           b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>());
+    }
     PT_HIP(hipGetLastError());
 }
 
