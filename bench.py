@@ -90,6 +90,15 @@ def main():
     ms_per_step = ms / args.steps
     mrays = n / (ms_per_step * 1e-3) / 1e6
     achieved = bytes_per_launch / (ms_per_step * 1e-3) / 1e9
+    if args.profile:
+        # Calibration for the HBM byte counters: a plain 256 MiB device copy.
+        a = hip.DeviceArray(256 << 20, np.uint8)
+        b = hip.DeviceArray(256 << 20, np.uint8)
+        a.zero()
+        b.copy_from(a)
+        hip.synchronize()
+        a.free()
+        b.free()
     hits = d_id.to_host()
     hit_rate = float((hits != ptamd.INVALID).mean())
     log(f"[bench] {mrays:.1f} Mrays/s ({ms_per_step:.3f} ms/launch, wall {wall:.2f}s), hit rate {hit_rate:.3f}")

@@ -100,7 +100,7 @@ struct DeviceBvh {
     DevBuf children, parents_leaf, parents_internal, counters;
     DevBuf nodes, tris, root_box;
     DevBuf tile_hist, digit_total;  // radix sort scratch
-    DevBuf span_list, span_count;   // refit: nodes whose leaf range crosses refit blocks
+    DevBuf span_list, span_top, span_count;  // refit: per-level node lists
 };
 
 struct DeviceScene {

@@ -9,7 +9,8 @@
 //                   list of nodes whose leaf range crosses a refit block
 //   k_refit_local   sorted triangle records + bottom-up AABBs inside each
 //                   1024-leaf block (LDS counters/boxes, workgroup scope)
-//   k_refit_top     the remaining block-spanning nodes, one workgroup
+//   k_refit_mid     nodes inside each 65536-leaf block, one workgroup each
+//   k_refit_top     the remaining nodes, one workgroup
 // Boxes are written straight into the parents' traversal node records.
 #include <hip/hip_runtime.h>
 
@@ -25,11 +26,22 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kSortKeysPerThread = 16;
 constexpr int kSortTile = kSortThreads * kSortKeysPerThread;  // 4096 keys
-// Refit leaf block: nodes whose leaf range lies inside one block of this many
-// sorted leaves are refit inside one workgroup (LDS counters and boxes); the
-// "spanning" nodes above them are finished by a single workgroup afterwards.
+// Refit levels.  A node's level is the smallest k whose block of sorted
+// leaves contains the node's whole leaf range:
+//   level 0: inside a 1024-leaf block    -> k_refit_local (LDS, one workgroup per block)
+//   level 1: inside a 65536-leaf block   -> k_refit_mid (one workgroup per block)
+//   level 2: anything larger             -> k_refit_top (one workgroup)
+// A node's children have a level no higher than its own, so each phase only
+// needs the previous phases' boxes.  Levels are kept in bits 30-31 of the
+// node's refit counter.
 constexpr int kRefitBlock = 1024;
-constexpr uint32_t kSpanFlag = 0x80000000u;
+constexpr uint32_t kMidBlock = 65536;
+// Level-1 nodes of one 65536-leaf block: each crosses one of its 63 inner
+// 1024-leaf boundaries, and a boundary is crossed only by the <= 64 ancestors
+// of its two leaves' lowest common ancestor.
+constexpr uint32_t kMidCap = 63 * 64;
+constexpr uint32_t kLevelShift = 30;
+constexpr uint32_t kCountMask = (1u << kLevelShift) - 1u;
 
 // ---- Morton keys ------------------------------------------------------------
 
@@ -57,9 +69,10 @@ __global__ __launch_bounds__(256) void k_morton(const pt_triangle* __restrict__ 
                                                 float mnx, float mny, float mnz, float ivx,
                                                 float ivy, float ivz, uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ vals,
-                                                uint32_t* __restrict__ span_count) {
+                                                uint32_t* __restrict__ span_count,
+                                                uint32_t n_span_counts) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *span_count = 0u;
+    if (i < n_span_counts) span_count[i] = 0u;
     if (i >= n) return;
     const float* t = reinterpret_cast<const float*>(tris + i);
     float cx = ((t[0] + t[3]) + t[6]) / 3.0f;
@@ -237,9 +250,11 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
 }
 
 // radix_tree_construction_kernel (BVHBuilder.cu:135-196), one thread per
-// internal node.  Also initialises the node's refit counter: bit 31 marks a
-// node whose leaf range [min(i,j), max(i,j)] crosses a refit block boundary,
-// and such nodes are appended to span_list.
+// internal node.  Also initialises the node's refit counter with its refit
+// level (from its leaf range [min(i,j), max(i,j)]) and appends level-1 nodes
+// to their 65536-block's list (span_list[block * kMidCap ...], count in
+// span_count[1 + block]) and level-2 nodes to the top list (span_top, count
+// in span_count[0]).
 __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__ keys, uint32_t n,
                                                     uint32_t* __restrict__ children,
                                                     uint32_t* __restrict__ parents_leaf,
@@ -247,6 +262,7 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ counters,
                                                     float4* __restrict__ nodes,
                                                     uint32_t* __restrict__ span_list,
+                                                    uint32_t* __restrict__ span_top,
                                                     uint32_t* __restrict__ span_count) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t N = n;
@@ -287,9 +303,16 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
     children[i] = w;
     nodes[4 * i + 3] = make_float4(__uint_as_float(w), 0.0f, 0.0f, 0.0f);
     const int64_t lo = i < j ? i : j, hi = i > j ? i : j;
-    const bool spanning = (lo / kRefitBlock) != (hi / kRefitBlock);
-    counters[i] = spanning ? kSpanFlag : 0u;
-    if (spanning) span_list[atomicAdd(span_count, 1u)] = static_cast<uint32_t>(i);
+    uint32_t level = 0;
+    if ((lo / kRefitBlock) != (hi / kRefitBlock)) level = ((lo / kMidBlock) != (hi / kMidBlock)) ? 2u : 1u;
+    counters[i] = level << kLevelShift;
+    if (level == 1) {
+        const uint32_t blk = static_cast<uint32_t>(lo / kMidBlock);
+        const uint32_t slot = atomicAdd(span_count + 1 + blk, 1u);
+        if (slot < kMidCap) span_list[static_cast<size_t>(blk) * kMidCap + slot] = static_cast<uint32_t>(i);
+    } else if (level == 2) {
+        span_top[atomicAdd(span_count, 1u)] = static_cast<uint32_t>(i);
+    }
     if (i == 0) parents_internal[0] = kInvalid;  // BVHBuilder.cu:224
 }
 
@@ -387,8 +410,8 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     }
     s_leaf.put(tid, b);
     uint32_t p = parents_leaf[k];
-    if (counters[p] & kSpanFlag) {
-        store_slot(nodes, p, (children[p] >> 2) == k, b);  // finished by k_refit_top
+    if (counters[p] >> kLevelShift) {
+        store_slot(nodes, p, (children[p] >> 2) == k, b);  // finished by a later phase
         return;
     }
     for (;;) {
@@ -407,37 +430,38 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
             return;
         }
         store_slot(nodes, pp, (children[pp] >> 2) == p, b);
-        if (counters[pp] & kSpanFlag) return;
+        if (counters[pp] >> kLevelShift) return;
         p = pp;
     }
 }
 
-// Refit, phase 2: the spanning nodes (leaf range across refit blocks), all in
-// one workgroup so the counter hand-off is workgroup-scope again.  Every child
-// box of a node sits in the node's own record: local children were stored by
-// phase 1, spanning children are stored here before their parent's counter
-// is incremented.  Each local child of a spanning node is one "arrival".
-__global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ children,
-                                                    const uint32_t* __restrict__ parents_internal,
-                                                    uint32_t* __restrict__ counters,
-                                                    float4* __restrict__ nodes,
-                                                    float* __restrict__ root_box,
-                                                    const uint32_t* __restrict__ span_list,
-                                                    const uint32_t* __restrict__ span_count) {
-    const uint32_t count = *span_count;
+// Refit of the level-L nodes listed in `list` (count at *count), all inside
+// one workgroup so the counter hand-off needs only workgroup-scope ordering.
+// Every child box of a node sits in the node's own record: lower-level
+// children were stored by earlier phases, same-level children are stored here
+// before their parent's counter is incremented.  Each lower-level child of a
+// listed node is one "arrival"; the thread completing a node climbs on while
+// the parent is of the same level.
+__device__ __forceinline__ void refit_level(const uint32_t* __restrict__ children,
+                                            const uint32_t* __restrict__ parents_internal,
+                                            uint32_t* __restrict__ counters,
+                                            float4* __restrict__ nodes,
+                                            float* __restrict__ root_box,
+                                            const uint32_t* __restrict__ list, uint32_t count,
+                                            uint32_t level) {
     for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
-        const uint32_t s = span_list[idx];
+        const uint32_t s = list[idx];
         const uint32_t w = children[s];
         const uint32_t l = w >> 2, r = l + 1;
         int arrivals = 0;
-        if (!(w & 1u) || !(counters[l] & kSpanFlag)) ++arrivals;
-        if (!(w & 2u) || !(counters[r] & kSpanFlag)) ++arrivals;
+        if (!(w & 1u) || (counters[l] >> kLevelShift) < level) ++arrivals;
+        if (!(w & 2u) || (counters[r] >> kLevelShift) < level) ++arrivals;
         for (int a = 0; a < arrivals; ++a) {
             uint32_t p = s;
             for (;;) {
                 const uint32_t old = __hip_atomic_fetch_add(counters + p, 1u, __ATOMIC_ACQ_REL,
                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
-                if ((old & ~kSpanFlag) == 0u) break;
+                if ((old & kCountMask) == 0u) break;
                 const float4* rec = nodes + 4 * static_cast<size_t>(p);
                 const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
                 const Box lb{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y};
@@ -449,10 +473,35 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
                     break;
                 }
                 store_slot(nodes, pp, (children[pp] >> 2) == p, b);
+                if ((counters[pp] >> kLevelShift) != level) break;
                 p = pp;
             }
         }
     }
+}
+
+// Level-1 nodes: one workgroup per 65536-leaf block.
+__global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ parents_internal,
+                                                    uint32_t* __restrict__ counters,
+                                                    float4* __restrict__ nodes,
+                                                    float* __restrict__ root_box,
+                                                    const uint32_t* __restrict__ span_list,
+                                                    const uint32_t* __restrict__ span_count) {
+    const uint32_t blk = blockIdx.x;
+    refit_level(children, parents_internal, counters, nodes, root_box,
+                span_list + static_cast<size_t>(blk) * kMidCap, min(span_count[1 + blk], kMidCap), 1u);
+}
+
+// Level-2 nodes: one workgroup.
+__global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ parents_internal,
+                                                    uint32_t* __restrict__ counters,
+                                                    float4* __restrict__ nodes,
+                                                    float* __restrict__ root_box,
+                                                    const uint32_t* __restrict__ span_top,
+                                                    const uint32_t* __restrict__ span_count) {
+    refit_level(children, parents_internal, counters, nodes, root_box, span_top, span_count[0], 2u);
 }
 
 // Parity dump: internal node boxes in the reference's BBox layout
@@ -505,16 +554,18 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     const uint32_t num_tiles = div_up(n, kSortTile);
     b.tile_hist.reserve(static_cast<size_t>(num_tiles) * 256 * 4);
     b.digit_total.reserve(256 * 4);
-    b.span_list.reserve(m * 4);
-    b.span_count.reserve(4);
+    const uint32_t mid_blocks = div_up(n, kMidBlock);
+    b.span_list.reserve(static_cast<size_t>(mid_blocks) * kMidCap * 4);
+    b.span_top.reserve(m * 4);
+    b.span_count.reserve((1 + static_cast<size_t>(mid_blocks)) * 4);
 
     // invScale = 1 / scale on the host (BVHBuilder.cu:15-17).
     float inv[3];
     for (int k = 0; k < 3; ++k) inv[k] = 1.0f / (s.bmax[k] - s.bmin[k]);
-    hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_morton, dim3(div_up(std::max<uint64_t>(n, 1 + mid_blocks), 256)), dim3(256), 0, st,
                        s.tris.as<pt_triangle>(), n, s.bmin[0], s.bmin[1], s.bmin[2], inv[0],
                        inv[1], inv[2], b.keys.as<uint32_t>(), b.vals.as<uint32_t>(),
-                       b.span_count.as<uint32_t>());
+                       b.span_count.as<uint32_t>(), 1 + mid_blocks);
     PT_HIP(hipGetLastError());
 
     if (n >= 2) {
@@ -539,7 +590,8 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                            b.keys.as<uint32_t>(), n, b.children.as<uint32_t>(),
                            b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                            b.counters.as<uint32_t>(), b.nodes.as<float4>(),
-                           b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>());
+                           b.span_list.as<uint32_t>(), b.span_top.as<uint32_t>(),
+                           b.span_count.as<uint32_t>());
         PT_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_refit_local, dim3(div_up(n, kRefitBlock)), dim3(kRefitBlock), 0, st,
@@ -548,10 +600,16 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                        b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.tris.as<float4>(),
                        b.root_box.as<float>(), n);
     if (n > static_cast<uint32_t>(kRefitBlock)) {
-        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
+        hipLaunchKernelGGL(k_refit_mid, dim3(mid_blocks), dim3(1024), 0, st, b.children.as<uint32_t>(),
                            b.parents_internal.as<uint32_t>(), b.counters.as<uint32_t>(),
                            b.nodes.as<float4>(), b.root_box.as<float>(),
                            b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>());
+    }
+    if (n > kMidBlock) {
+        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
+                           b.parents_internal.as<uint32_t>(), b.counters.as<uint32_t>(),
+                           b.nodes.as<float4>(), b.root_box.as<float>(),
+                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>());
     }
     PT_HIP(hipGetLastError());
 }

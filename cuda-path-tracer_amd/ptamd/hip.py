@@ -81,6 +81,10 @@ class DeviceArray:
         _ok(_hip.hipMemcpy(a.ctypes.data, self.ptr, self.nbytes, _D2H), "hipMemcpy D2H")
         return a
 
+    def copy_from(self, other):
+        """Device-to-device copy of min(nbytes) bytes (the runtime's blit kernel)."""
+        _ok(_hip.hipMemcpy(self.ptr, other.ptr, min(self.nbytes, other.nbytes), _D2D), "hipMemcpy D2D")
+
     def zero(self):
         _ok(_hip.hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
