@@ -157,6 +157,14 @@ def main():
                          f"OpenMP {threads} threads"}
         log(f"[bench] CPU oracle: {cpu['value']} Mrays/s on {threads} threads ({m} rays, {dt:.1f}s)")
 
+    # HBM bytes per launch of the same kernel from the committed rocprofv3
+    # FETCH_SIZE/WRITE_SIZE runs (calibrated against a 256 MiB copy).
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "closest_hit_c1_16M_hbm_bytes.json")
+    if os.path.exists(tf) and n == 16 * 1024 * 1024:
+        with open(tf) as f:
+            traffic = int(json.load(f)["bytes_per_launch"])
+
     out = {
         "metric": "closest-hit Mrays/s @871K tris",
         "value": round(mrays, 2),
@@ -174,7 +182,8 @@ def main():
                                "vs 660x660-band fBm UV sphere, 871,200 fp32 triangles",
                    "rays": n, "triangles": int(len(tris)), "seed_mesh": seed_mesh, "seed_rays": seed_rays},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": int(bytes_per_launch)},
         "cpu_baseline": cpu,
         "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
         "render": render,
