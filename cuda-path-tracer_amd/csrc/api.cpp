@@ -205,6 +205,7 @@ int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n, ui
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         require(n == 0 || d_rays != nullptr, "rays is NULL");
+        require(n < (1u << 31), "at most 2^31 - 1 rays per call");
         pt::trace_closest(ctx->scene, d_rays, n, d_id, d_t, d_uv, pick(ctx, stream));
     });
 }
@@ -215,6 +216,7 @@ int pt_trace_closest(pt_context* ctx, const float* rays, uint32_t n, uint32_t* i
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         require(n == 0 || rays != nullptr, "rays is NULL");
+        require(n < (1u << 31), "at most 2^31 - 1 rays per call");
         if (n == 0) return;
         hipStream_t st = ctx->stream;
         const size_t nn = n;
@@ -238,6 +240,7 @@ int pt_trace_closest_counted(pt_context* ctx, const float* d_rays, uint32_t n, u
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         require(n == 0 || d_rays != nullptr, "rays is NULL");
+        require(n < (1u << 31), "at most 2^31 - 1 rays per call");
         hipStream_t st = pick(ctx, stream);
         pt::DevBuf c;
         c.reserve(3 * sizeof(unsigned long long));
@@ -331,6 +334,30 @@ int pt_last_render_ms(pt_context* ctx, float* ms) {
         PT_HIP(hipEventSynchronize(ctx->ev1));
         PT_HIP(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
     });
+}
+
+namespace {
+uint32_t* tune_slot(const char* name) {
+    if (!name) return nullptr;
+    const std::string k(name);
+    if (k == "trace.blocks") return &pt::trace_tuning().blocks;
+    if (k == "trace.refill") return &pt::trace_tuning().refill;
+    return nullptr;
+}
+}  // namespace
+
+int pt_tune_set(const char* name, uint32_t value) {
+    uint32_t* slot = tune_slot(name);
+    if (!slot || value == 0) return PT_E_INVALID_ARG;
+    *slot = value;
+    return PT_OK;
+}
+
+int pt_tune_get(const char* name, uint32_t* value) {
+    uint32_t* slot = tune_slot(name);
+    if (!slot || !value) return PT_E_INVALID_ARG;
+    *value = *slot;
+    return PT_OK;
 }
 
 }  // extern "C"

@@ -101,6 +101,7 @@ struct DeviceBvh {
     DevBuf nodes, tris, root_box;
     DevBuf tile_hist, digit_total;  // radix sort scratch
     DevBuf span_list, span_top, span_count;  // refit: per-level node lists
+    DevBuf ray_counter;                      // persistent traversal: next ray
 };
 
 struct DeviceScene {
@@ -124,6 +125,13 @@ void build_lbvh(DeviceScene& scene, hipStream_t stream);
 void dump_bvh(DeviceScene& scene, hipStream_t stream, uint32_t* morton, uint32_t* perm,
               uint32_t* children, uint32_t* parents_leaf, uint32_t* parents_internal,
               float* bboxes);
+
+// Launch shape of the persistent closest-hit kernel (tunable).
+struct TraceTuning {
+    uint32_t blocks = 2048;  // persistent workgroups of 256 threads
+    uint32_t refill = 48;    // refill idle lanes when fewer than this many are busy
+};
+TraceTuning& trace_tuning();
 
 // Closest hit over a device ray batch (rays: 6 floats each).
 void trace_closest(const DeviceScene& scene, const float* d_rays, uint32_t n, uint32_t* d_id,

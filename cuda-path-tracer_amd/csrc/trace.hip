@@ -2,6 +2,8 @@
 // exposed as a ray-batch entry point).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "context.hpp"
 #include "traverse.hpp"
 
@@ -43,19 +45,134 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
     }
 }
 
+// Persistent closest-hit kernel (ntri >= 2): fixed grid, rays handed out in
+// pools of kRayPool per wave from one global counter; lanes refill from the
+// pool when fewer than `refill` lanes are busy.
+constexpr uint32_t kRayPool = 256;
+
+template <bool COUNT>
+__global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
+    const float4* __restrict__ nodes, const float4* __restrict__ tris, const float* __restrict__ rays,
+    uint32_t nrays, uint32_t* __restrict__ out_id, float* __restrict__ out_t,
+    float2* __restrict__ out_uv, uint32_t* __restrict__ ray_counter, uint32_t refill,
+    unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
+    uint32_t spill[kStackMax - kLdsStack];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    TraceCount cnt{0, 0, 0};
+    RayState r;
+    bool active = false;
+    uint32_t ray = 0;
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
+    bool drained = false;                  // wave-uniform: global counter exhausted
+    for (;;) {
+        uint64_t busy = __ballot(active);
+        if (!drained && static_cast<uint32_t>(__popcll(busy)) < refill) {
+            // Hand rays to idle lanes, refilling the wave's pool as needed.
+            uint64_t idle = ~busy;
+            while (idle != 0ull) {
+                if (pool_next >= pool_end) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(ray_counter, kRayPool);
+                    base = __shfl(base, 0, 64);
+                    if (base >= nrays) {
+                        drained = true;
+                        break;
+                    }
+                    pool_next = base;
+                    pool_end = min(base + kRayPool, nrays);
+                }
+                const uint32_t avail = pool_end - pool_next;
+                const uint64_t lane_bit = 1ull << lane;
+                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
+                const bool take = (idle & lane_bit) && rank < avail;
+                const uint32_t took = min(static_cast<uint32_t>(__popcll(idle)), avail);
+                if (take) {
+                    ray = pool_next + rank;
+                    const float* rp = rays + 6 * static_cast<size_t>(ray);
+                    r.o = v3(rp[0], rp[1], rp[2]);
+                    r.d = v3(rp[3], rp[4], rp[5]);
+                    r.dinv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+                    r.finite = __builtin_fabsf(r.dinv.x) <= kFltMax &&
+                               __builtin_fabsf(r.dinv.y) <= kFltMax &&
+                               __builtin_fabsf(r.dinv.z) <= kFltMax;
+                    r.minT = kMinTInit;
+                    r.best = kInvalid;
+                    r.bu = 0.0f;
+                    r.bv = 0.0f;
+                    r.node = 0;
+                    r.sp = 0;
+                    active = true;
+                }
+                pool_next += took;
+                idle = ~__ballot(active);
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active && !visit_step<COUNT>(nodes, tris, r, s_stack, spill, cnt)) {
+            if (out_id) out_id[ray] = r.best;
+            if (out_t) out_t[ray] = r.best == kInvalid ? kFltMax : r.minT;
+            if (out_uv) out_uv[ray] = r.best == kInvalid ? make_float2(0.0f, 0.0f) : make_float2(r.bu, r.bv);
+            active = false;
+        }
+    }
+    if (COUNT) {
+        unsigned long long a = cnt.iters, b = cnt.boxes, c = cnt.tris;
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_down(a, off, 64);
+            b += __shfl_down(b, off, 64);
+            c += __shfl_down(c, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(counters + 0, a);
+            atomicAdd(counters + 1, b);
+            atomicAdd(counters + 2, c);
+        }
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
 template <bool COUNT>
 void launch_trace(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
                   float* d_t, float* d_uv, unsigned long long* d_counters, hipStream_t st) {
     if (n == 0) return;
-    const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(n) + kTraceThreads - 1) /
-                                                  kTraceThreads);
-    hipLaunchKernelGGL(k_trace<COUNT>, dim3(blocks), dim3(kTraceThreads), 0, st,
-                       s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), s.bvh.n, d_rays, n, d_id,
-                       d_t, reinterpret_cast<float2*>(d_uv), d_counters);
+    if (s.bvh.n < 2) {
+        const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(n) + kTraceThreads - 1) /
+                                                      kTraceThreads);
+        hipLaunchKernelGGL(k_trace<COUNT>, dim3(blocks), dim3(kTraceThreads), 0, st,
+                           s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), s.bvh.n, d_rays, n,
+                           d_id, d_t, reinterpret_cast<float2*>(d_uv), d_counters);
+        PT_HIP(hipGetLastError());
+        return;
+    }
+    const TraceTuning& tune = trace_tuning();
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s.bvh.ray_counter.as<uint32_t>());
+    const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
+    const uint32_t blocks =
+        static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
+    hipLaunchKernelGGL(k_trace_persistent<COUNT>, dim3(blocks), dim3(kPersistThreads), 0, st,
+                       s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), d_rays, n, d_id, d_t,
+                       reinterpret_cast<float2*>(d_uv), s.bvh.ray_counter.as<uint32_t>(),
+                       std::min(64u, std::max(1u, tune.refill)), d_counters);
     PT_HIP(hipGetLastError());
 }
 
 }  // namespace
+
+TraceTuning& trace_tuning() {
+    static TraceTuning t = [] {
+        TraceTuning v;
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            cus > 0)
+            v.blocks = static_cast<uint32_t>(cus) * 8u;
+        return v;
+    }();
+    return t;
+}
 
 void trace_closest(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
                    float* d_t, float* d_uv, hipStream_t st) {

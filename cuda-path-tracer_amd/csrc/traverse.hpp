@@ -139,4 +139,100 @@ __device__ __forceinline__ TraceHit trace_ray(const float4* __restrict__ nodes,
     return traverse_impl<COUNT, false>(nodes, tris, o, d, dinv, cnt);
 }
 
+// ---- persistent traversal -------------------------------------------------
+//
+// One lane = one ray at a time, advanced one node visit per loop iteration;
+// a lane whose ray has finished takes the next ray from its wave's pool, so
+// long rays do not hold a whole wave's lanes idle.  The visiting order per ray
+// is exactly traverse_impl's.  The deferred-node stack keeps its first
+// kLdsStack entries in LDS ([depth][thread] so a wave's lanes hit distinct
+// banks) and the rest in a private array (scratch), touched only by deep rays.
+
+constexpr int kLdsStack = 16;
+constexpr int kPersistThreads = 256;
+
+struct RayState {
+    V3 o, d, dinv;
+    float minT;
+    uint32_t best;
+    float bu, bv;
+    uint32_t node;
+    int sp;
+    bool finite;
+};
+
+template <bool COUNT>
+__device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
+                                           const float4* __restrict__ tris, RayState& r,
+                                           uint32_t (*s_stack)[kPersistThreads],
+                                           uint32_t* spill, TraceCount& cnt) {
+    const float4* rec = nodes + 4 * static_cast<size_t>(r.node);
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const uint32_t w = __float_as_uint(r3.x);
+    const uint32_t l = w >> 2;
+    const bool in0 = (w & 1u) != 0, in1 = (w & 2u) != 0;
+    float h0, h1, u0 = 0.0f, v0 = 0.0f, u1 = 0.0f, v1 = 0.0f;
+    uint32_t id0 = kInvalid, id1 = kInvalid;
+    if (COUNT) cnt.iters++;
+    if (in0) {
+        if (COUNT) cnt.boxes++;
+        h0 = r.finite ? slab_test_finite(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y)
+                      : slab_test(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+    } else {
+        if (COUNT) cnt.tris++;
+        h0 = tri_hit(tris, l, r.o, r.d, u0, v0, id0);
+    }
+    if (in1) {
+        if (COUNT) cnt.boxes++;
+        h1 = r.finite ? slab_test_finite(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w)
+                      : slab_test(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
+    } else {
+        if (COUNT) cnt.tris++;
+        h1 = tri_hit(tris, l + 1, r.o, r.d, u1, v1, id1);
+    }
+    if (h0 <= r.minT || h1 <= r.minT) {
+        const bool sw = h1 < h0;  // left child wins ties (trace.h:125)
+        const float hc = sw ? h1 : h0, hf = sw ? h0 : h1;
+        const uint32_t cc = sw ? l + 1 : l, cf = sw ? l : l + 1;
+        const bool ic = sw ? in1 : in0, jf = sw ? in0 : in1;
+        const float uc = sw ? u1 : u0, vc = sw ? v1 : v0, uf = sw ? u0 : u1, vf = sw ? v0 : v1;
+        const uint32_t idc = sw ? id1 : id0, idf = sw ? id0 : id1;
+        if (ic) {
+            r.node = cc;
+            if (hf <= r.minT) {
+                if (jf) {
+                    const int sp = r.sp;
+                    if (sp < kLdsStack) {
+                        s_stack[sp][threadIdx.x] = cf;
+                        r.sp = sp + 1;
+                    } else if (sp < kStackMax) {
+                        spill[sp - kLdsStack] = cf;
+                        r.sp = sp + 1;
+                    }
+                } else if (hf < r.minT || idf < r.best) {
+                    r.minT = hf; r.best = idf; r.bu = uf; r.bv = vf;
+                }
+            }
+            return true;
+        }
+        if (hc < r.minT || idc < r.best) {
+            r.minT = hc; r.best = idc; r.bu = uc; r.bv = vc;
+        }
+        if (hf <= r.minT) {
+            if (jf) {
+                r.node = cf;
+                return true;
+            }
+            if (hf < r.minT || idf < r.best) {
+                r.minT = hf; r.best = idf; r.bu = uf; r.bv = vf;
+            }
+        }
+    }
+    if (r.sp == 0) return false;
+    const int sp = r.sp - 1;
+    r.node = sp < kLdsStack ? s_stack[sp][threadIdx.x] : spill[sp - kLdsStack];
+    r.sp = sp;
+    return true;
+}
+
 }  // namespace pt

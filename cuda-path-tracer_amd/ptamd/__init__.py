@@ -88,6 +88,8 @@ _SIGS = {
     "pt_gen_batch_counts": [_u64, _u32, _u32, _u32, _vp],
     "pt_camera_corners": [_vp, _vp, _vp, _f, _f, _f, _f, _vp],
     "pt_pass_jitter": [_u64, _u32, _i, _i, _vp, _vp],
+    "pt_tune_set": [ctypes.c_char_p, _u32],
+    "pt_tune_get": [ctypes.c_char_p, _vp],
 }
 for _name, _args in _SIGS.items():
     _fn = getattr(_lib, _name)
@@ -174,6 +176,16 @@ def pass_jitter(seed, pass_index, w, h):
     jy = np.zeros(1, np.float32)
     _check(_lib.pt_pass_jitter(seed, pass_index, w, h, _ptr(jx), _ptr(jy)))
     return float(jx[0]), float(jy[0])
+
+
+def tune_set(name, value):
+    _check(_lib.pt_tune_set(name.encode(), int(value)))
+
+
+def tune_get(name):
+    v = np.zeros(1, np.uint32)
+    _check(_lib.pt_tune_get(name.encode(), _ptr(v)))
+    return int(v[0])
 
 
 # ---- device context -------------------------------------------------------------

@@ -163,6 +163,15 @@ int pt_read_image(pt_context* ctx, float* rgba);
 /* Wall time in ms of the last pt_render_pass (HIP events). */
 int pt_last_render_ms(pt_context* ctx, float* ms);
 
+/* ---- launch tuning (no reference counterpart) -------------------------------
+ * Process-wide launch-shape knobs read by the kernels' host launchers; results
+ * do not depend on them (tests check this).  Names:
+ *   "trace.blocks"  persistent closest-hit workgroups (256 threads each)
+ *   "trace.refill"  refill idle lanes when fewer than this many of 64 are busy
+ * Unknown names return PT_E_INVALID_ARG. */
+int pt_tune_set(const char* name, uint32_t value);
+int pt_tune_get(const char* name, uint32_t* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,16 @@ def test_no_device_is_a_status_code():
         raise AssertionError("pt_create succeeded without a device")
 
 
+def test_tuning_knobs():
+    for name in ("trace.blocks", "trace.refill"):
+        v = ptamd.tune_get(name)
+        ptamd.tune_set(name, v + 1)
+        assert ptamd.tune_get(name) == v + 1
+        ptamd.tune_set(name, v)
+    assert ptamd._lib.pt_tune_set(b"no.such.knob", 1) == ptamd.PT_E_INVALID_ARG
+    assert ptamd._lib.pt_tune_set(b"trace.blocks", 0) == ptamd.PT_E_INVALID_ARG
+
+
 def test_null_arguments_rejected():
     assert ptamd._lib.pt_upload_scene(None, None, 0, None, 0, None, 0, None, None, None) == ptamd.PT_E_INVALID_ARG
     assert ptamd._lib.pt_scene_bounds(None, 0, None, None) == ptamd.PT_E_INVALID_ARG

@@ -167,6 +167,29 @@ def test_trace_parity(ctx, case):
     check_trace(ctx, tris, infos, rays, case)
 
 
+def test_trace_invariant_under_tuning(ctx):
+    """Launch-shape knobs (what the autotuner varies) never change results."""
+    tris, infos = ptamd.gen_soup(65536, 0)
+    bmin, bmax = ptamd.scene_bounds(tris)
+    rays = np.concatenate([ptamd.gen_shell_rays(30000, 8, bmin, bmax, 1.5),
+                           interior_rays(30000, 9, bmin, bmax)])
+    ctx.upload_scene(tris, infos)
+    b = orc.build(tris, bmin, bmax)
+    oids, ot, ouv = orc.trace(tris, b, rays)
+    saved = (ptamd.tune_get("trace.blocks"), ptamd.tune_get("trace.refill"))
+    try:
+        for blocks, refill in ((1, 1), (3, 64), (64, 8), (256, 32), (100000, 48), (2048, 63)):
+            say(f"[tuning invariance] blocks={blocks} refill={refill}")
+            ptamd.tune_set("trace.blocks", blocks)
+            ptamd.tune_set("trace.refill", refill)
+            ids, t, uv = ctx.trace_closest(rays)
+            assert np.array_equal(ids, oids) and np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+            assert np.array_equal(uv.view(np.uint32), ouv.view(np.uint32))
+    finally:
+        ptamd.tune_set("trace.blocks", saved[0])
+        ptamd.tune_set("trace.refill", saved[1])
+
+
 def test_trace_parity_c1_full_mesh(ctx):
     # config 1 microbench mesh at full size: 871,200 triangles
     tris, infos = ptamd.gen_sphere(660, 2024, room=False)
