@@ -36,13 +36,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported (this build runs on one GPU; got {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
     if args.profile:
-        args.steps, args.warmup, args.no_cpu = 3, 2, True
+        args.steps, args.warmup, args.no_cpu, args.no_tune = 3, 2, True, True
 
     import numpy as np
 
@@ -73,15 +74,29 @@ def main():
     log(f"[bench] per ray: {iters / n:.2f} node visits, {boxes / n:.2f} box tests, "
         f"{tri_tests / n:.2f} triangle tests -> {bytes_per_launch / n:.0f} B/ray")
 
-    for _ in range(args.warmup):
+    def step():
         ctx.trace_closest_device(d_rays.ptr, n, d_id.ptr, d_t.ptr, 0, stream.handle)
+
+    # Autotune the launch shape on this workload (before the timed region).
+    tuning = None
+    if not args.no_tune:
+        from ptamd import autotune
+
+        best, table = autotune.tune(step, stream, autotune.trace_space(), reps=2)
+        tuning = {"chosen": best, "candidates": len(table),
+                  "best_ms": round(min(ms for _, ms in table), 4),
+                  "worst_ms": round(max(ms for _, ms in table), 4)}
+        log(f"[bench] autotune: {best} ({len(table)} candidates, "
+            f"{tuning['best_ms']:.3f}..{tuning['worst_ms']:.3f} ms)")
+    for _ in range(args.warmup):
+        step()
     stream.synchronize()
     hip.synchronize()
     e0, e1 = hip.Event(), hip.Event()
     w0 = time.time()
     e0.record(stream)
     for _ in range(args.steps):
-        ctx.trace_closest_device(d_rays.ptr, n, d_id.ptr, d_t.ptr, 0, stream.handle)
+        step()
     e1.record(stream)
     e1.synchronize()
     hip.synchronize()
@@ -185,6 +200,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": int(bytes_per_launch)},
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
         "render": render,
         "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),

@@ -1,0 +1,57 @@
+"""Launch-shape autotuner.
+
+Times candidate settings of the library's tuning knobs (pt_tune_set) on the
+caller's real workload with HIP events on the caller's stream and keeps the
+fastest.  Results never depend on these knobs (tests/test_gpu_parity.py
+checks the closest-hit knobs), so tuning is safe to run anywhere before a
+timed region.
+"""
+from __future__ import annotations
+
+import itertools
+
+from . import hip, tune_get, tune_set
+
+
+def _time(fn, stream, reps):
+    e0, e1 = hip.Event(), hip.Event()
+    fn()  # warm-up at this setting
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    e0.destroy()
+    e1.destroy()
+    return ms
+
+
+def tune(run, stream, space, reps=3):
+    """Grid search over `space` ({knob: [values]}).  `run()` launches one
+    step of the workload on `stream`.  Leaves the best setting applied and
+    returns (best: {knob: value}, table: [(setting, ms)])."""
+    names = list(space)
+    saved = {k: tune_get(k) for k in names}
+    table = []
+    best, best_ms = None, float("inf")
+    try:
+        for values in itertools.product(*(space[k] for k in names)):
+            setting = dict(zip(names, values))
+            for k, v in setting.items():
+                tune_set(k, v)
+            ms = _time(run, stream, reps)
+            table.append((setting, ms))
+            if ms < best_ms:
+                best, best_ms = setting, ms
+    finally:
+        for k, v in (best or saved).items():
+            tune_set(k, v)
+    return best, table
+
+
+def trace_space(num_cus=256):
+    """Default closest-hit search space: persistent workgroups per CU and the
+    lane-refill threshold."""
+    return {"trace.blocks": [num_cus * k for k in (2, 4, 8, 12, 16)],
+            "trace.refill": [16, 32, 48, 56, 64]}
