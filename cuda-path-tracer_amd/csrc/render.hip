@@ -11,6 +11,7 @@
 // pass; the kernels loop over their queue with a fixed grid.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "context.hpp"
@@ -52,6 +53,7 @@ constexpr int kThreads = 256;
 __global__ __launch_bounds__(kThreads) void k_generate(const FrameParams fp_,
                                                        RayQueue q, PathState ps,
                                                        uint32_t* __restrict__ counters,
+                                                       uint32_t* __restrict__ ray_counters,
                                                        uint32_t nlevels) {
     const FrameParams* fp = &fp_;
     const int w = fp->w, h = fp->h;
@@ -60,6 +62,7 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameParams fp_,
     if (gid == 0) {
         counters[0] = npix;
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
+        for (uint32_t k = 0; k <= nlevels; ++k) ray_counters[k] = 0u;
     }
     const V3 d00 = v3(fp->dirs[0], fp->dirs[1], fp->dirs[2]);
     const V3 d10 = v3(fp->dirs[3], fp->dirs[4], fp->dirs[5]);
@@ -85,22 +88,36 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameParams fp_,
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_extend(const float4* __restrict__ nodes,
-                                                     const float4* __restrict__ tris,
-                                                     uint32_t ntri, RayQueue q, HitQueue hq,
-                                                     const uint32_t* __restrict__ counters,
-                                                     uint32_t level) {
+// Closest hit for every ray of queue `level` (count on the device): the
+// persistent traversal, rays handed out from ray_counters[level].
+__global__ __launch_bounds__(kPersistThreads) void k_extend(const float4* __restrict__ nodes,
+                                                            const float4* __restrict__ tris,
+                                                            uint32_t ntri, RayQueue q, HitQueue hq,
+                                                            const uint32_t* __restrict__ counters,
+                                                            uint32_t* __restrict__ ray_counters,
+                                                            uint32_t level, uint32_t refill) {
+    __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
     const uint32_t count = counters[level];
     TraceCount cnt{0, 0, 0};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const V3 o = v3(q.ox[i], q.oy[i], q.oz[i]);
-        const V3 d = v3(q.dx[i], q.dy[i], q.dz[i]);
-        const TraceHit h = trace_ray<false>(nodes, tris, ntri, o, d, cnt);
+    auto load = [&](uint32_t i, V3& o, V3& d) {
+        o = v3(q.ox[i], q.oy[i], q.oz[i]);
+        d = v3(q.dx[i], q.dy[i], q.dz[i]);
+    };
+    auto store = [&](uint32_t i, const TraceHit& h) {
         hq.t[i] = h.t;
         hq.id[i] = h.id;
         hq.u[i] = h.u;
         hq.v[i] = h.v;
+    };
+    if (ntri < 2) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+            V3 o, d;
+            load(i, o, d);
+            store(i, trace_ray<false>(nodes, tris, ntri, o, d, cnt));
+        }
+        return;
     }
+    persistent_trace<false>(nodes, tris, count, ray_counters + level, refill, s_stack, load, store, cnt);
 }
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
@@ -294,7 +311,7 @@ void render_resize(Renderer& r, int w, int h, hipStream_t st) {
     r.h = h;
     r.accum.reserve(np * 3 * sizeof(double));
     r.queues.reserve(queue_bytes(np));
-    r.counters.reserve(64 * sizeof(uint32_t));
+    r.counters.reserve(128 * sizeof(uint32_t));  // queue lengths, then ray hand-out counters
     render_reset(r, st);
 }
 
@@ -326,12 +343,17 @@ void render_pass(Renderer& r, const DeviceScene& scene, uint64_t seed, hipStream
 
     Layout L = carve(r);
     uint32_t* cnt = r.counters.as<uint32_t>();
+    uint32_t* rcnt = cnt + 64;
+    const TraceTuning& tune = trace_tuning();
+    const uint32_t pgrid = std::max(1u, tune.blocks);
+    const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
     const uint32_t nlev = r.settings.bounces + 1;
     const uint32_t grid = grid_for(np);
-    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fd, L.q[0], L.ps, cnt, nlev);
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fd, L.q[0], L.ps, cnt, rcnt, nlev);
     for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
-        hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kThreads), 0, st, scene.bvh.nodes.as<float4>(),
-                           scene.bvh.tris.as<float4>(), scene.bvh.n, L.q[lvl & 1], L.hq, cnt, lvl);
+        hipLaunchKernelGGL(k_extend, dim3(std::min(pgrid, grid)), dim3(kPersistThreads), 0, st,
+                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.bvh.n,
+                           L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
         hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, fd,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl);

@@ -45,11 +45,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
     }
 }
 
-// Persistent closest-hit kernel (ntri >= 2): fixed grid, rays handed out in
-// pools of kRayPool per wave from one global counter; lanes refill from the
-// pool when fewer than `refill` lanes are busy.
-constexpr uint32_t kRayPool = 256;
-
+// Persistent closest-hit kernel over an AoS ray batch (ntri >= 2).
 template <bool COUNT>
 __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
     const float4* __restrict__ nodes, const float4* __restrict__ tris, const float* __restrict__ rays,
@@ -57,66 +53,20 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
     float2* __restrict__ out_uv, uint32_t* __restrict__ ray_counter, uint32_t refill,
     unsigned long long* __restrict__ counters) {
     __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
-    uint32_t spill[kStackMax - kLdsStack];
-    const int lane = threadIdx.x & 63;
-    const uint64_t lt = (1ull << lane) - 1ull;
     TraceCount cnt{0, 0, 0};
-    RayState r;
-    bool active = false;
-    uint32_t ray = 0;
-    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
-    bool drained = false;                  // wave-uniform: global counter exhausted
-    for (;;) {
-        uint64_t busy = __ballot(active);
-        if (!drained && static_cast<uint32_t>(__popcll(busy)) < refill) {
-            // Hand rays to idle lanes, refilling the wave's pool as needed.
-            uint64_t idle = ~busy;
-            while (idle != 0ull) {
-                if (pool_next >= pool_end) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(ray_counter, kRayPool);
-                    base = __shfl(base, 0, 64);
-                    if (base >= nrays) {
-                        drained = true;
-                        break;
-                    }
-                    pool_next = base;
-                    pool_end = min(base + kRayPool, nrays);
-                }
-                const uint32_t avail = pool_end - pool_next;
-                const uint64_t lane_bit = 1ull << lane;
-                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
-                const bool take = (idle & lane_bit) && rank < avail;
-                const uint32_t took = min(static_cast<uint32_t>(__popcll(idle)), avail);
-                if (take) {
-                    ray = pool_next + rank;
-                    const float* rp = rays + 6 * static_cast<size_t>(ray);
-                    r.o = v3(rp[0], rp[1], rp[2]);
-                    r.d = v3(rp[3], rp[4], rp[5]);
-                    r.dinv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-                    r.finite = __builtin_fabsf(r.dinv.x) <= kFltMax &&
-                               __builtin_fabsf(r.dinv.y) <= kFltMax &&
-                               __builtin_fabsf(r.dinv.z) <= kFltMax;
-                    r.minT = kMinTInit;
-                    r.best = kInvalid;
-                    r.bu = 0.0f;
-                    r.bv = 0.0f;
-                    r.node = 0;
-                    r.sp = 0;
-                    active = true;
-                }
-                pool_next += took;
-                idle = ~__ballot(active);
-            }
-        }
-        if (__ballot(active) == 0ull) break;
-        if (active && !visit_step<COUNT>(nodes, tris, r, s_stack, spill, cnt)) {
-            if (out_id) out_id[ray] = r.best;
-            if (out_t) out_t[ray] = r.best == kInvalid ? kFltMax : r.minT;
-            if (out_uv) out_uv[ray] = r.best == kInvalid ? make_float2(0.0f, 0.0f) : make_float2(r.bu, r.bv);
-            active = false;
-        }
-    }
+    persistent_trace<COUNT>(
+        nodes, tris, nrays, ray_counter, refill, s_stack,
+        [&](uint32_t i, V3& o, V3& d) {
+            const float* rp = rays + 6 * static_cast<size_t>(i);
+            o = v3(rp[0], rp[1], rp[2]);
+            d = v3(rp[3], rp[4], rp[5]);
+        },
+        [&](uint32_t i, const TraceHit& h) {
+            if (out_id) out_id[i] = h.id;
+            if (out_t) out_t[i] = h.t;
+            if (out_uv) out_uv[i] = make_float2(h.u, h.v);
+        },
+        cnt);
     if (COUNT) {
         unsigned long long a = cnt.iters, b = cnt.boxes, c = cnt.tris;
         for (int off = 32; off > 0; off >>= 1) {
@@ -124,7 +74,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
             b += __shfl_down(b, off, 64);
             c += __shfl_down(c, off, 64);
         }
-        if (lane == 0) {
+        if ((threadIdx.x & 63) == 0) {
             atomicAdd(counters + 0, a);
             atomicAdd(counters + 1, b);
             atomicAdd(counters + 2, c);

@@ -235,4 +235,77 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
     return true;
 }
 
+// Persistent closest-hit loop over rays [0, nrays), handed out in pools of
+// kRayPool per wave from *ray_counter (zeroed by the caller); idle lanes are
+// refilled when fewer than `refill` (1..64) lanes of the wave are busy.
+// load(i, o, d) fetches ray i; store(i, hit) writes its result.
+constexpr uint32_t kRayPool = 256;
+
+template <bool COUNT, class Load, class Store>
+__device__ __forceinline__ void persistent_trace(const float4* __restrict__ nodes,
+                                                 const float4* __restrict__ tris, uint32_t nrays,
+                                                 uint32_t* __restrict__ ray_counter,
+                                                 uint32_t refill,
+                                                 uint32_t (*s_stack)[kPersistThreads], Load load,
+                                                 Store store, TraceCount& cnt) {
+    uint32_t spill[kStackMax - kLdsStack];
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    RayState r;
+    bool active = false;
+    uint32_t ray = 0;
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
+    bool drained = false;                  // wave-uniform: no rays left to hand out
+    for (;;) {
+        const uint64_t busy = __ballot(active);
+        if (!drained && static_cast<uint32_t>(__popcll(busy)) < refill) {
+            uint64_t idle = ~busy;
+            while (idle != 0ull) {
+                if (pool_next >= pool_end) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(ray_counter, kRayPool);
+                    base = __shfl(base, 0, 64);
+                    if (base >= nrays) {
+                        drained = true;
+                        break;
+                    }
+                    pool_next = base;
+                    pool_end = min(base + kRayPool, nrays);
+                }
+                const uint32_t avail = pool_end - pool_next;
+                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
+                const bool take = ((idle >> lane) & 1ull) && rank < avail;
+                const uint32_t took = min(static_cast<uint32_t>(__popcll(idle)), avail);
+                if (take) {
+                    ray = pool_next + rank;
+                    load(ray, r.o, r.d);
+                    r.dinv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+                    r.finite = __builtin_fabsf(r.dinv.x) <= kFltMax &&
+                               __builtin_fabsf(r.dinv.y) <= kFltMax &&
+                               __builtin_fabsf(r.dinv.z) <= kFltMax;
+                    r.minT = kMinTInit;
+                    r.best = kInvalid;
+                    r.bu = 0.0f;
+                    r.bv = 0.0f;
+                    r.node = 0;
+                    r.sp = 0;
+                    active = true;
+                }
+                pool_next += took;
+                idle = ~__ballot(active);
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active && !visit_step<COUNT>(nodes, tris, r, s_stack, spill, cnt)) {
+            TraceHit h;
+            h.id = r.best;
+            h.t = r.best == kInvalid ? kFltMax : r.minT;
+            h.u = r.best == kInvalid ? 0.0f : r.bu;
+            h.v = r.best == kInvalid ? 0.0f : r.bv;
+            store(ray, h);
+            active = false;
+        }
+    }
+}
+
 }  // namespace pt
