@@ -125,31 +125,35 @@ def main():
     build_ms_1m = float(np.median([ctx.build_bvh() for _ in range(10)]))
     log(f"[bench] LBVH build: {build_ms_1m:.3f} ms @ 2^20 soup, {build_ms_mesh:.3f} ms @ 871,200 mesh")
 
-    # Progressive render legs (spp/s): config 0 and config 1 scenes.
+    # Progressive render legs (spp/s): config 0 (16 passes per frame) and
+    # config 1 (one pass per frame), frames replayed as captured hipGraphs.
     render = {}
-    for name, (rt, ri), st, (w, h), passes, eye, tgt in (
+    for name, (rt, ri), st, (w, h), frames, per_frame, eye, tgt in (
         ("c0_soup_256x256_b4", ptamd.gen_soup(65536, 0), ptamd.Settings(4, ptamd.INVALID, 1.0, 1),
-         (256, 256), 16, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)),
+         (256, 256), 2, 16, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)),
         ("c1_room_1024x1024_b8_rr3", ptamd.gen_sphere(660, seed_mesh, room=True), ptamd.Settings(8, 3, 1.0, 1),
-         (1024, 1024), 4 if args.profile else 8, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))):
+         (1024, 1024), 4 if args.profile else 8, 1, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))):
         ctx.upload_scene(rt, ri, settings=st)
-        dirs = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h)
-        ctx.set_primary_rays(dirs, eye)
-        ctx.render_pass(w, h, 1)  # warm-up pass (allocates queues)
+        ctx.set_primary_rays(ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h), eye)
+        passes = frames * per_frame
+        dirs = np.stack([ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, *ptamd.pass_jitter(5, p, w, h))
+                         for p in range(passes)])
+        seeds = np.arange(1000, 1000 + passes, dtype=np.uint64)
+        ctx.render_passes(w, h, seeds[:per_frame], dirs[:per_frame])  # warm-up frame (allocation, capture)
         ctx.reset_frames()
         stream.synchronize()
         r0, r1 = hip.Event(), hip.Event()
         r0.record(stream)
-        for p in range(passes):
-            jx, jy = ptamd.pass_jitter(5, p, w, h)
-            ctx.set_primary_rays(ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, jx, jy), eye)
-            ctx.render_pass(w, h, 1000 + p)
+        for f in range(frames):
+            sl = slice(f * per_frame, (f + 1) * per_frame)
+            ctx.render_passes(w, h, seeds[sl], dirs[sl])
         r1.record(stream)
         r1.synchronize()
         rms = r0.elapsed_ms(r1)
         render[name] = {"ms_per_pass": round(rms / passes, 3), "spp_per_s": round(passes / (rms * 1e-3), 2),
-                        "Msamples_per_s": round(w * h * passes / (rms * 1e-3) / 1e6, 2), "passes": passes}
-        log(f"[bench] render {name}: {rms / passes:.3f} ms/pass")
+                        "Msamples_per_s": round(w * h * passes / (rms * 1e-3) / 1e6, 2), "passes": passes,
+                        "passes_per_frame": per_frame}
+        log(f"[bench] render {name}: {rms / passes:.3f} ms/pass ({per_frame} passes/frame)")
 
     cpu = None
     if not args.no_cpu:

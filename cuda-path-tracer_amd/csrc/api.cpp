@@ -270,7 +270,23 @@ int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed) {
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
         PT_HIP(hipEventRecord(ctx->ev0, st));
-        pt::render_pass(ctx->renderer, ctx->scene, pass_seed, st);
+        pt::render_passes(ctx->renderer, ctx->scene, 1, nullptr, &pass_seed, st);
+        PT_HIP(hipEventRecord(ctx->ev1, st));
+        ctx->renderer.timed = true;
+    });
+}
+
+int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const float* dirs,
+                     const uint64_t* seeds) {
+    if (!ctx || !seeds) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        if (ctx->renderer.lights != 0)
+            throw pt::Error(PT_E_UNSUPPORTED, "directional lights are not supported by the renderer");
+        hipStream_t st = ctx->stream;
+        pt::render_resize(ctx->renderer, w, h, st);
+        PT_HIP(hipEventRecord(ctx->ev0, st));
+        pt::render_passes(ctx->renderer, ctx->scene, npasses, dirs, seeds, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
         ctx->renderer.timed = true;
     });
@@ -342,13 +358,14 @@ uint32_t* tune_slot(const char* name) {
     const std::string k(name);
     if (k == "trace.blocks") return &pt::trace_tuning().blocks;
     if (k == "trace.refill") return &pt::trace_tuning().refill;
+    if (k == "render.graphs") return &pt::render_tuning().graphs;
     return nullptr;
 }
 }  // namespace
 
 int pt_tune_set(const char* name, uint32_t value) {
     uint32_t* slot = tune_slot(name);
-    if (!slot || value == 0) return PT_E_INVALID_ARG;
+    if (!slot || (value == 0 && slot != &pt::render_tuning().graphs)) return PT_E_INVALID_ARG;
     *slot = value;
     return PT_OK;
 }

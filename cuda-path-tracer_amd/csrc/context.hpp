@@ -142,6 +142,17 @@ void trace_closest_counted(const DeviceScene& scene, const float* d_rays, uint32
                            uint32_t* d_id, float* d_t, float* d_uv,
                            unsigned long long* d_counters, hipStream_t stream);
 
+// Per-pass values of a frame, read by the render kernels from device memory.
+struct PassParams {
+    float dirs[12];  // corner directions: bottom-left, bottom-right, top-right, top-left
+    float eye[3];
+    uint32_t pass;   // pass index since the last reset (stratum, RNG key)
+    uint32_t seedw;  // 32-bit pass seed (RNG key)
+    uint32_t pad;
+};
+constexpr uint32_t kMaxPasses = 64;  // passes per frame
+constexpr int kParamRing = 4;        // pinned PassParams slots in flight
+
 // Progressive renderer state (wavefront queues + FP64 accumulator).
 struct Renderer {
     int w = 0, h = 0;
@@ -150,15 +161,34 @@ struct Renderer {
     float eye[3] = {0, 0, 0};
     pt_settings settings{1u, 0xFFFFFFFFu, 3.0f, 1u};
     DevBuf accum;       // double[3*w*h]
-    DevBuf queues;      // ray/hit queues and per-pixel path state
+    DevBuf queues;      // ray/hit queues and per-path state
     DevBuf counters;    // queue counters
+    DevBuf params;      // PassParams[kMaxPasses]
+    HostBuf params_host;
+    hipEvent_t ring_ev[kParamRing] = {};
+    bool ring_used[kParamRing] = {};
+    int ring_next = 0;
+    void* graph_exec = nullptr;   // hipGraphExec_t of the last captured frame
+    alignas(8) char graph_key[160] = {};
     uint32_t lights = 0;  // directional lights in the scene (not rendered)
     bool timed = false;   // a pass has been bracketed by the context events
+    Renderer() = default;
+    Renderer(const Renderer&) = delete;
+    Renderer& operator=(const Renderer&) = delete;
+    ~Renderer();
 };
+
+struct RenderTuning {
+    uint32_t graphs = 1;  // replay each frame's kernels as one captured hipGraph
+};
+RenderTuning& render_tuning();
 
 void render_resize(Renderer& r, int w, int h, hipStream_t stream);
 void render_reset(Renderer& r, hipStream_t stream);
-void render_pass(Renderer& r, const DeviceScene& scene, uint64_t seed, hipStream_t stream);
+// npasses 1-spp passes in one frame; dirs = 12 floats per pass (NULL: the
+// renderer's corner rays for every pass); seeds[npasses].
+void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, const float* dirs,
+                   const uint64_t* seeds, hipStream_t stream);
 
 }  // namespace pt
 

@@ -1,14 +1,17 @@
 // Progressive unidirectional path tracer as a wavefront pipeline
 // (reference: raytrace_kernel + blinnPhongShadingPathTrace,
 // cuda_path_tracer.cu:245-333, 385-421, which run one megakernel thread per
-// pixel).  Per pass:
-//   k_generate     primary rays for every pixel, queue 0, counters reset
-//   k_extend(L)    closest hit for every ray of queue L
+// pixel and one launch per 1-spp pass).  One frame renders P passes at once
+// (path = pass * pixels + pixel) so small images still fill the GPU:
+//   k_generate     primary rays of every path into queue 0, counters reset
+//   k_extend(L)    persistent closest hit for every ray of queue L
 //   k_shade(L)     shade the hits, append surviving paths to queue L+1
 //                  (wave-aggregated atomics), L = 0 .. bounces
-//   k_accumulate   FP64 per-pixel sums += this pass's path radiance
-// The queue lengths live on the device, so the host never waits inside a
-// pass; the kernels loop over their queue with a fixed grid.
+//   k_accumulate   per pixel, FP64 sums += each pass's radiance, in pass order
+// Queue lengths live on the device, so the host never waits inside a frame,
+// and a frame's kernel sequence is captured once as a hipGraph and replayed.
+// Per-pass values (corner rays, eye, pass index, seed) are read from device
+// memory, so replays only need a fresh copy of that small block.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,22 +24,21 @@
 namespace pt {
 namespace {
 
-struct FrameParams {
-    float dirs[12];
-    float eye[3];
-    uint32_t pass;
-    uint32_t seedw;
+// Fixed for a captured frame.
+struct FrameConst {
     int w, h;
+    uint32_t npix;
+    uint32_t npasses;
     uint32_t bounces;
     uint32_t rr_start;
     float intensity;
     uint32_t stratified;
 };
 
-// Structure-of-arrays queue of rays (origin, direction, pixel).
+// Structure-of-arrays queue of rays (origin, direction, path index).
 struct RayQueue {
     float *ox, *oy, *oz, *dx, *dy, *dz;
-    uint32_t* pix;
+    uint32_t* path;
 };
 struct HitQueue {
     float* t;
@@ -50,38 +52,39 @@ struct PathState {
 
 constexpr int kThreads = 256;
 
-__global__ __launch_bounds__(kThreads) void k_generate(const FrameParams fp_,
+__global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
+                                                       const PassParams* __restrict__ pp,
                                                        RayQueue q, PathState ps,
                                                        uint32_t* __restrict__ counters,
                                                        uint32_t* __restrict__ ray_counters,
                                                        uint32_t nlevels) {
-    const FrameParams* fp = &fp_;
-    const int w = fp->w, h = fp->h;
-    const uint32_t npix = static_cast<uint32_t>(w) * static_cast<uint32_t>(h);
+    const uint32_t npaths = fc.npix * fc.npasses;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
-        counters[0] = npix;
+        counters[0] = npaths;
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
         for (uint32_t k = 0; k <= nlevels; ++k) ray_counters[k] = 0u;
     }
-    const V3 d00 = v3(fp->dirs[0], fp->dirs[1], fp->dirs[2]);
-    const V3 d10 = v3(fp->dirs[3], fp->dirs[4], fp->dirs[5]);
-    const V3 d11 = v3(fp->dirs[6], fp->dirs[7], fp->dirs[8]);
-    const V3 d01 = v3(fp->dirs[9], fp->dirs[10], fp->dirs[11]);
-    for (uint32_t p = gid; p < npix; p += gridDim.x * blockDim.x) {
-        const int x = static_cast<int>(p % static_cast<uint32_t>(w));
-        const int y = static_cast<int>(p / static_cast<uint32_t>(w));
+    for (uint32_t p = gid; p < npaths; p += gridDim.x * blockDim.x) {
+        const PassParams& P = pp[p / fc.npix];
+        const uint32_t pix = p % fc.npix;
+        const int x = static_cast<int>(pix % static_cast<uint32_t>(fc.w));
+        const int y = static_cast<int>(pix / static_cast<uint32_t>(fc.w));
         // Pixel-corner blend of the corner directions (cuda_path_tracer.cu:404-407).
-        const float tx = static_cast<float>(x) / static_cast<float>(w);
-        const float ty = static_cast<float>(y) / static_cast<float>(h);
+        const float tx = static_cast<float>(x) / static_cast<float>(fc.w);
+        const float ty = static_cast<float>(y) / static_cast<float>(fc.h);
+        const V3 d00 = v3(P.dirs[0], P.dirs[1], P.dirs[2]);
+        const V3 d10 = v3(P.dirs[3], P.dirs[4], P.dirs[5]);
+        const V3 d11 = v3(P.dirs[6], P.dirs[7], P.dirs[8]);
+        const V3 d01 = v3(P.dirs[9], P.dirs[10], P.dirs[11]);
         const V3 d = mix(mix(d00, d10, tx), mix(d01, d11, tx), ty);
-        q.ox[p] = fp->eye[0];
-        q.oy[p] = fp->eye[1];
-        q.oz[p] = fp->eye[2];
+        q.ox[p] = P.eye[0];
+        q.oy[p] = P.eye[1];
+        q.oz[p] = P.eye[2];
         q.dx[p] = d.x;
         q.dy[p] = d.y;
         q.dz[p] = d.z;
-        q.pix[p] = p;
+        q.path[p] = p;
         ps.cr[p] = 0.0f;
         ps.cg[p] = 0.0f;
         ps.cb[p] = 0.0f;
@@ -122,16 +125,13 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const float4* __rest
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
-__global__ __launch_bounds__(kThreads) void k_shade(const FrameParams fp_,
+__global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
+                                                    const PassParams* __restrict__ pp,
                                                     const pt_triangle_info* __restrict__ infos,
                                                     RayQueue qin, HitQueue hq, RayQueue qout,
                                                     PathState ps, uint32_t* __restrict__ counters,
                                                     uint32_t level) {
-    const FrameParams* fp = &fp_;
     const uint32_t count = counters[level];
-    const uint32_t bounces = fp->bounces;
-    const uint32_t rr_start = fp->rr_start;
-    const float intensity = fp->intensity;
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -140,9 +140,11 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameParams fp_,
         const uint32_t i = base + threadIdx.x;
         bool cont = false;
         V3 no{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
-        uint32_t pix = 0;
+        uint32_t path = 0;
         if (i < count && hq.id[i] != kInvalid) {
-            pix = qin.pix[i];
+            path = qin.path[i];
+            const PassParams& P = pp[path / fc.npix];
+            const uint32_t pix = path % fc.npix;
             const V3 o = v3(qin.ox[i], qin.oy[i], qin.oz[i]);
             const V3 d = v3(qin.dx[i], qin.dy[i], qin.dz[i]);
             const float t = hq.t[i];
@@ -167,8 +169,8 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameParams fp_,
                 }
                 absorbed = albedo;
             } else {
-                absorbed = v3(ps.ar[pix], ps.ag[pix], ps.ab[pix]);
-                acc = v3(ps.cr[pix], ps.cg[pix], ps.cb[pix]);
+                absorbed = v3(ps.ar[path], ps.ag[path], ps.ab[path]);
+                acc = v3(ps.cr[path], ps.cg[path], ps.cb[path]);
                 V3 L = v3(0.0f, 0.0f, 0.0f);
                 if (emissive) {
                     // Emitter cosine (cuda_path_tracer.cu:304-312).
@@ -177,15 +179,15 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameParams fp_,
                     albedo = v3(1.0f, 1.0f, 1.0f);
                 }
                 absorbed = mul(absorbed, albedo);
-                acc = add(acc, scale(mul(absorbed, L), intensity));
+                acc = add(acc, scale(mul(absorbed, L), fc.intensity));
             }
             // Next bounce index = level (bounce loop, cuda_path_tracer.cu:267).
             const uint32_t b = level;
-            if (b < bounces) {
-                const U4 rv = pcg4d(U4{pix, fp->pass, b, fp->seedw});
+            if (b < fc.bounces) {
+                const U4 rv = pcg4d(U4{pix, P.pass, b, P.seedw});
                 float u1 = u24(rv.x), u2 = u24(rv.y);
                 cont = true;
-                if (b >= rr_start) {
+                if (b >= fc.rr_start) {
                     float p = gmax(gmax(albedo.x, albedo.y), albedo.z);
                     p = gmin(p, 1.0f);
                     if (!(u24(rv.z) < p)) {
@@ -196,11 +198,13 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameParams fp_,
                 }
                 if (cont) {
                     no = add(pos, scale(nrm, 1e-5f));
-                    if (fp->stratified != 0u && b == 0) {
-                        const uint32_t p1 = fp->pass % 64u;
+                    if (fc.stratified != 0u && b == 0) {
+                        // 8x8 strata, stratum = pass % 64 (cuda_path_tracer.cu:273-280).
+                        const uint32_t p1 = P.pass % 64u;
                         u1 = (static_cast<float>(p1 % 8u) + u1) / 8.0f;
                         u2 = (static_cast<float>(p1 / 8u) + u2) / 8.0f;
                     }
+                    // Cosine-weighted hemisphere sample (random.h:95-102).
                     const float r = __builtin_sqrtf(u1);
                     float sn, cs;
                     sincos2pi(u2, sn, cs);
@@ -208,12 +212,12 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameParams fp_,
                     nd = to_hemisphere(local, nrm);
                 }
             }
-            ps.ar[pix] = absorbed.x;
-            ps.ag[pix] = absorbed.y;
-            ps.ab[pix] = absorbed.z;
-            ps.cr[pix] = acc.x;
-            ps.cg[pix] = acc.y;
-            ps.cb[pix] = acc.z;
+            ps.ar[path] = absorbed.x;
+            ps.ag[path] = absorbed.y;
+            ps.ab[path] = absorbed.z;
+            ps.cr[path] = acc.x;
+            ps.cg[path] = acc.y;
+            ps.cb[path] = acc.z;
         }
         const uint64_t m = __ballot(cont);
         if (m != 0ull) {
@@ -229,20 +233,29 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameParams fp_,
                 qout.dx[s] = nd.x;
                 qout.dy[s] = nd.y;
                 qout.dz[s] = nd.z;
-                qout.pix[s] = pix;
+                qout.path[s] = path;
             }
         }
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_accumulate(const FrameParams fp_,
-                                                         PathState ps, double* __restrict__ accum) {
-    const FrameParams* fp = &fp_;
-    const uint32_t npix = static_cast<uint32_t>(fp->w) * static_cast<uint32_t>(fp->h);
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
-        accum[3 * static_cast<size_t>(p) + 0] += static_cast<double>(ps.cr[p]);
-        accum[3 * static_cast<size_t>(p) + 1] += static_cast<double>(ps.cg[p]);
-        accum[3 * static_cast<size_t>(p) + 2] += static_cast<double>(ps.cb[p]);
+// FP64 accumulation, pass by pass in order -- the same additions as one
+// accumulate per 1-spp pass.
+__global__ __launch_bounds__(kThreads) void k_accumulate(const FrameConst fc, PathState ps,
+                                                         double* __restrict__ accum) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < fc.npix; p += gridDim.x * blockDim.x) {
+        double r = accum[3 * static_cast<size_t>(p) + 0];
+        double g = accum[3 * static_cast<size_t>(p) + 1];
+        double b = accum[3 * static_cast<size_t>(p) + 2];
+        for (uint32_t k = 0; k < fc.npasses; ++k) {
+            const size_t s = static_cast<size_t>(k) * fc.npix + p;
+            r += static_cast<double>(ps.cr[s]);
+            g += static_cast<double>(ps.cg[s]);
+            b += static_cast<double>(ps.cb[s]);
+        }
+        accum[3 * static_cast<size_t>(p) + 0] = r;
+        accum[3 * static_cast<size_t>(p) + 1] = g;
+        accum[3 * static_cast<size_t>(p) + 2] = b;
     }
 }
 
@@ -258,40 +271,38 @@ struct Layout {
     PathState ps;
 };
 
-Layout carve(Renderer& r) {
-    const size_t np = static_cast<size_t>(r.w) * r.h;
-    char* base = r.queues.as<char>();
-    auto take = [&](size_t bytes) {
+constexpr int kQueueArrays = 2 * 7 + 4 + 6;
+
+size_t slab_bytes(size_t npaths) { return (npaths * 4 + 255) & ~size_t(255); }
+
+Layout carve(char* base, size_t npaths) {
+    const size_t a = slab_bytes(npaths);
+    auto take = [&]() {
         char* p = base;
-        base += (bytes + 255) & ~size_t(255);
+        base += a;
         return p;
     };
     Layout L;
     for (int k = 0; k < 2; ++k) {
-        L.q[k].ox = reinterpret_cast<float*>(take(np * 4));
-        L.q[k].oy = reinterpret_cast<float*>(take(np * 4));
-        L.q[k].oz = reinterpret_cast<float*>(take(np * 4));
-        L.q[k].dx = reinterpret_cast<float*>(take(np * 4));
-        L.q[k].dy = reinterpret_cast<float*>(take(np * 4));
-        L.q[k].dz = reinterpret_cast<float*>(take(np * 4));
-        L.q[k].pix = reinterpret_cast<uint32_t*>(take(np * 4));
+        L.q[k].ox = reinterpret_cast<float*>(take());
+        L.q[k].oy = reinterpret_cast<float*>(take());
+        L.q[k].oz = reinterpret_cast<float*>(take());
+        L.q[k].dx = reinterpret_cast<float*>(take());
+        L.q[k].dy = reinterpret_cast<float*>(take());
+        L.q[k].dz = reinterpret_cast<float*>(take());
+        L.q[k].path = reinterpret_cast<uint32_t*>(take());
     }
-    L.hq.t = reinterpret_cast<float*>(take(np * 4));
-    L.hq.id = reinterpret_cast<uint32_t*>(take(np * 4));
-    L.hq.u = reinterpret_cast<float*>(take(np * 4));
-    L.hq.v = reinterpret_cast<float*>(take(np * 4));
-    L.ps.ar = reinterpret_cast<float*>(take(np * 4));
-    L.ps.ag = reinterpret_cast<float*>(take(np * 4));
-    L.ps.ab = reinterpret_cast<float*>(take(np * 4));
-    L.ps.cr = reinterpret_cast<float*>(take(np * 4));
-    L.ps.cg = reinterpret_cast<float*>(take(np * 4));
-    L.ps.cb = reinterpret_cast<float*>(take(np * 4));
+    L.hq.t = reinterpret_cast<float*>(take());
+    L.hq.id = reinterpret_cast<uint32_t*>(take());
+    L.hq.u = reinterpret_cast<float*>(take());
+    L.hq.v = reinterpret_cast<float*>(take());
+    L.ps.ar = reinterpret_cast<float*>(take());
+    L.ps.ag = reinterpret_cast<float*>(take());
+    L.ps.ab = reinterpret_cast<float*>(take());
+    L.ps.cr = reinterpret_cast<float*>(take());
+    L.ps.cg = reinterpret_cast<float*>(take());
+    L.ps.cb = reinterpret_cast<float*>(take());
     return L;
-}
-
-size_t queue_bytes(size_t np) {
-    const size_t a = ((np * 4 + 255) & ~size_t(255));
-    return a * (2 * 7 + 4 + 6);
 }
 
 uint32_t grid_for(size_t work) {
@@ -301,16 +312,62 @@ uint32_t grid_for(size_t work) {
     return static_cast<uint32_t>(b);
 }
 
+// Everything a captured frame's kernel arguments depend on.
+struct GraphKey {
+    FrameConst fc;
+    const void* nodes;
+    const void* tris;
+    const void* infos;
+    uint32_t ntri;
+    const void* queues;
+    const void* params;
+    const void* accum;
+    const void* counters;
+    uint32_t blocks, refill;
+    hipStream_t stream;
+};
+
+void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, hipStream_t st) {
+    const size_t npaths = static_cast<size_t>(fc.npix) * fc.npasses;
+    Layout L = carve(r.queues.as<char>(), npaths);
+    uint32_t* cnt = r.counters.as<uint32_t>();
+    uint32_t* rcnt = cnt + 64;
+    const PassParams* pp = r.params.as<PassParams>();
+    const TraceTuning& tune = trace_tuning();
+    const uint32_t nlev = fc.bounces + 1;
+    const uint32_t grid = grid_for(npaths);
+    const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
+    const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt, nlev);
+    for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
+        hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st,
+                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.bvh.n,
+                           L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
+        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
+                           scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
+                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl);
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
+                       r.accum.as<double>());
+}
+
 }  // namespace
+
+Renderer::~Renderer() {
+    if (graph_exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec));
+    for (auto& e : ring_ev)
+        if (e) (void)hipEventDestroy(e);
+}
 
 void render_resize(Renderer& r, int w, int h, hipStream_t st) {
     if (w <= 0 || h <= 0) throw Error(PT_E_INVALID_ARG, "render size must be positive");
+    if (static_cast<uint64_t>(w) * static_cast<uint64_t>(h) > (1ull << 28))
+        throw Error(PT_E_INVALID_ARG, "at most 2^28 pixels");
     if (r.w == w && r.h == h) return;
     const size_t np = static_cast<size_t>(w) * h;
     r.w = w;
     r.h = h;
     r.accum.reserve(np * 3 * sizeof(double));
-    r.queues.reserve(queue_bytes(np));
     r.counters.reserve(128 * sizeof(uint32_t));  // queue lengths, then ray hand-out counters
     render_reset(r, st);
 }
@@ -323,44 +380,93 @@ void render_reset(Renderer& r, hipStream_t st) {
     PT_HIP(hipGetLastError());
 }
 
-void render_pass(Renderer& r, const DeviceScene& scene, uint64_t seed, hipStream_t st) {
+void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, const float* dirs,
+                   const uint64_t* seeds, hipStream_t st) {
     if (!scene.ready) throw Error(PT_E_NO_SCENE, "no scene uploaded");
     if (r.settings.bounces > 60) throw Error(PT_E_INVALID_ARG, "at most 60 bounces");
-    const size_t np = static_cast<size_t>(r.w) * r.h;
-    FrameParams frame;
-    FrameParams* fh = &frame;
-    std::memcpy(fh->dirs, r.dirs, sizeof fh->dirs);
-    std::memcpy(fh->eye, r.eye, sizeof fh->eye);
-    fh->pass = r.spp;
-    fh->seedw = static_cast<uint32_t>(seed) ^ static_cast<uint32_t>(seed >> 32);
-    fh->w = r.w;
-    fh->h = r.h;
-    fh->bounces = r.settings.bounces;
-    fh->rr_start = r.settings.rr_start;
-    fh->intensity = r.settings.indirect_intensity;
-    fh->stratified = r.settings.stratified;
-    const FrameParams& fd = frame;
+    if (npasses == 0 || npasses > kMaxPasses) throw Error(PT_E_INVALID_ARG, "1 .. 64 passes per call");
+    const size_t npix = static_cast<size_t>(r.w) * r.h;
+    if (npix * npasses >= (1ull << 31)) throw Error(PT_E_INVALID_ARG, "too many paths in one frame");
+    r.queues.reserve(slab_bytes(npix * npasses) * kQueueArrays);
+    r.params.reserve(sizeof(PassParams) * kMaxPasses);
+    r.params_host.reserve(sizeof(PassParams) * kMaxPasses * kParamRing);
+    if (!r.ring_ev[0])
+        for (auto& e : r.ring_ev) PT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
-    Layout L = carve(r);
-    uint32_t* cnt = r.counters.as<uint32_t>();
-    uint32_t* rcnt = cnt + 64;
-    const TraceTuning& tune = trace_tuning();
-    const uint32_t pgrid = std::max(1u, tune.blocks);
-    const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
-    const uint32_t nlev = r.settings.bounces + 1;
-    const uint32_t grid = grid_for(np);
-    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fd, L.q[0], L.ps, cnt, rcnt, nlev);
-    for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
-        hipLaunchKernelGGL(k_extend, dim3(std::min(pgrid, grid)), dim3(kPersistThreads), 0, st,
-                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.bvh.n,
-                           L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
-        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, fd,
-                           scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
-                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl);
+    // Per-pass block: through a ring of pinned slots, so a slot is rewritten
+    // only after its previous copy has left.
+    const int slot = r.ring_next;
+    r.ring_next = (r.ring_next + 1) % kParamRing;
+    if (r.ring_used[slot]) PT_HIP(hipEventSynchronize(r.ring_ev[slot]));
+    PassParams* hp = r.params_host.as<PassParams>() + static_cast<size_t>(slot) * kMaxPasses;
+    for (uint32_t k = 0; k < npasses; ++k) {
+        std::memcpy(hp[k].dirs, dirs ? dirs + 12 * k : r.dirs, sizeof hp[k].dirs);
+        std::memcpy(hp[k].eye, r.eye, sizeof hp[k].eye);
+        hp[k].pass = r.spp + k;
+        hp[k].seedw = static_cast<uint32_t>(seeds[k]) ^ static_cast<uint32_t>(seeds[k] >> 32);
+        hp[k].pad = 0;
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kThreads), 0, st, fd, L.ps, r.accum.as<double>());
-    PT_HIP(hipGetLastError());
-    r.spp += 1;
+    PT_HIP(hipMemcpyAsync(r.params.as<void>(), hp, sizeof(PassParams) * npasses, hipMemcpyHostToDevice, st));
+    PT_HIP(hipEventRecord(r.ring_ev[slot], st));
+    r.ring_used[slot] = true;
+
+    FrameConst fc;
+    fc.w = r.w;
+    fc.h = r.h;
+    fc.npix = static_cast<uint32_t>(npix);
+    fc.npasses = npasses;
+    fc.bounces = r.settings.bounces;
+    fc.rr_start = r.settings.rr_start;
+    fc.intensity = r.settings.indirect_intensity;
+    fc.stratified = r.settings.stratified;
+
+    if (!render_tuning().graphs) {
+        launch_frame(r, scene, fc, st);
+        PT_HIP(hipGetLastError());
+    } else {
+        GraphKey key;
+        std::memset(&key, 0, sizeof key);
+        key.fc = fc;
+        key.nodes = scene.bvh.nodes.as<void>();
+        key.tris = scene.bvh.tris.as<void>();
+        key.infos = scene.infos.as<void>();
+        key.ntri = scene.bvh.n;
+        key.queues = r.queues.as<void>();
+        key.params = r.params.as<void>();
+        key.accum = r.accum.as<void>();
+        key.counters = r.counters.as<void>();
+        key.blocks = trace_tuning().blocks;
+        key.refill = trace_tuning().refill;
+        key.stream = st;
+        static_assert(sizeof(GraphKey) <= sizeof(r.graph_key), "graph key storage");
+        if (!r.graph_exec || std::memcmp(r.graph_key, &key, sizeof key) != 0) {
+            if (r.graph_exec) {
+                PT_HIP(hipStreamSynchronize(st));
+                PT_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(r.graph_exec)));
+                r.graph_exec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            PT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            launch_frame(r, scene, fc, st);
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (le != hipSuccess) throw Error(PT_E_HIP, std::string("frame capture: ") + hipGetErrorString(le));
+            if (ce != hipSuccess) throw Error(PT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) throw Error(PT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            r.graph_exec = ex;
+            std::memcpy(r.graph_key, &key, sizeof key);
+        }
+        PT_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(r.graph_exec), st));
+    }
+    r.spp += npasses;
+}
+
+RenderTuning& render_tuning() {
+    static RenderTuning t;
+    return t;
 }
 
 }  // namespace pt

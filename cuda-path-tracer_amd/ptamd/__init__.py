@@ -75,6 +75,7 @@ _SIGS = {
     "pt_trace_closest_counted": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp],
     "pt_set_primary_rays": [_vp, _vp, _vp],
     "pt_render_pass": [_vp, _i, _i, _u64],
+    "pt_render_passes": [_vp, _i, _i, _u32, _vp, _vp],
     "pt_reset_frames": [_vp],
     "pt_read_accumulator": [_vp, _vp, _vp],
     "pt_write_accumulator": [_vp, _i, _i, _vp, _u32],
@@ -282,6 +283,12 @@ class Context:
 
     def render_pass(self, w, h, seed):
         self._c(_lib.pt_render_pass(self._h, w, h, seed))
+
+    def render_passes(self, w, h, seeds, dirs=None):
+        """len(seeds) passes in one frame; dirs: (npasses, 12) corner rays or None."""
+        seeds = np.ascontiguousarray(seeds, np.uint64)
+        d = None if dirs is None else _f32(dirs, 12 * len(seeds))
+        self._c(_lib.pt_render_passes(self._h, w, h, len(seeds), _ptr(d), _ptr(seeds)))
 
     def reset_frames(self):
         self._c(_lib.pt_reset_frames(self._h))

@@ -152,6 +152,13 @@ int pt_set_primary_rays(pt_context* ctx, const float* dirs, const float* eye);
  * w x h pixels, added to the FP64 accumulator.  The accumulator is reset when
  * w/h change.  pass_seed replaces the per-pass host random seed. */
 int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed);
+/* Several passes in one frame (no reference counterpart; the reference
+ * launches one pass per frame): the accumulator ends up exactly as after
+ * npasses calls of pt_render_pass with the same corner rays and seeds.
+ * dirs = 12 floats per pass (NULL: the corner rays of pt_set_primary_rays for
+ * every pass), seeds[npasses]; 1 <= npasses <= 64. */
+int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const float* dirs,
+                     const uint64_t* seeds);
 /* CUDAPathTracer::resetFrames (cuda_path_tracer.h:28): zero the accumulator. */
 int pt_reset_frames(pt_context* ctx);
 /* Accumulator access: rgb_sum[3*w*h] FP64 sums (row 0 = bottom), spp passes. */
@@ -168,6 +175,8 @@ int pt_last_render_ms(pt_context* ctx, float* ms);
  * do not depend on them (tests check this).  Names:
  *   "trace.blocks"  persistent closest-hit workgroups (256 threads each)
  *   "trace.refill"  refill idle lanes when fewer than this many of 64 are busy
+ *   "render.graphs" 1: replay each frame's kernels as one captured hipGraph,
+ *                   0: launch them one by one
  * Unknown names return PT_E_INVALID_ARG. */
 int pt_tune_set(const char* name, uint32_t value);
 int pt_tune_get(const char* name, uint32_t* value);

@@ -1,0 +1,115 @@
+"""GPU render paths: multi-pass frames, hipGraph replays and launch-shape
+knobs give accumulators bit-identical to sequential 1-spp passes and to the
+oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+import ptamd
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+W, H = 64, 48
+EYE, TGT = (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)
+
+
+def pass_dirs(p, seed=21):
+    jx, jy = ptamd.pass_jitter(seed, p, W, H)
+    return ptamd.camera_corners(EYE, TGT, (0, 1, 0), 60.0, W / H, jx, jy)
+
+
+@pytest.fixture(scope="module")
+def scene():
+    tris, infos = ptamd.gen_soup(65536, 0)
+    return tris, infos, ptamd.Settings(4, 2, 1.0, 1)
+
+
+def oracle_acc(tris, infos, st, passes, seeds):
+    b = orc.build(tris, *ptamd.scene_bounds(tris))
+    acc = np.zeros((H, W, 3))
+    for p in range(passes):
+        orc.render_pass(tris, b, infos, st, W, H, pass_dirs(p), EYE, p, int(seeds[p]), acc)
+    return acc
+
+
+def test_multi_pass_frame_equals_sequential(ctx, scene):
+    tris, infos, st = scene
+    ctx.upload_scene(tris, infos, settings=st)
+    n = 6
+    seeds = np.arange(100, 100 + n, dtype=np.uint64)
+    dirs = np.stack([pass_dirs(p) for p in range(n)])
+    o = oracle_acc(tris, infos, st, n, seeds)
+    results = {}
+    for graphs in (1, 0):
+        ptamd.tune_set("render.graphs", graphs)
+        say(f"[render] sequential passes, graphs={graphs}")
+        ctx.reset_frames()
+        for p in range(n):
+            ctx.set_primary_rays(dirs[p], EYE)
+            ctx.render_pass(W, H, int(seeds[p]))
+        results[("seq", graphs)] = ctx.read_accumulator(W, H)
+        say(f"[render] one {n}-pass frame, graphs={graphs}")
+        ctx.reset_frames()
+        ctx.render_passes(W, H, seeds, dirs)
+        results[("frame", graphs)] = ctx.read_accumulator(W, H)
+        say(f"[render] 2+4 pass frames, graphs={graphs}")
+        ctx.reset_frames()
+        ctx.render_passes(W, H, seeds[:2], dirs[:2])
+        ctx.render_passes(W, H, seeds[2:], dirs[2:])
+        results[("split", graphs)] = ctx.read_accumulator(W, H)
+    ptamd.tune_set("render.graphs", 1)
+    assert o.sum() > 0
+    for k, (acc, spp) in results.items():
+        assert spp == n, k
+        assert np.array_equal(acc, o), k
+
+
+def test_graph_replays_match_eager(ctx, scene):
+    tris, infos, st = scene
+    ctx.upload_scene(tris, infos, settings=st)
+    frames = 5
+    seeds = np.arange(7, 7 + 2 * frames, dtype=np.uint64)
+    dirs = np.stack([pass_dirs(p) for p in range(2 * frames)])
+    per_mode = {}
+    for graphs in (0, 1):
+        ptamd.tune_set("render.graphs", graphs)
+        ctx.reset_frames()
+        accs = []
+        for f in range(frames):
+            say(f"[graph replay] graphs={graphs} frame {f}")
+            ctx.render_passes(W, H, seeds[2 * f:2 * f + 2], dirs[2 * f:2 * f + 2])
+            ctx.synchronize()
+            accs.append(ctx.read_accumulator(W, H)[0])
+        per_mode[graphs] = accs
+    ptamd.tune_set("render.graphs", 1)
+    for f in range(frames):
+        assert np.array_equal(per_mode[0][f], per_mode[1][f]), f"frame {f}"
+
+
+def test_render_invariant_under_tuning(ctx, scene):
+    tris, infos, st = scene
+    ctx.upload_scene(tris, infos, settings=st)
+    seeds = np.arange(3, dtype=np.uint64)
+    dirs = np.stack([pass_dirs(p) for p in range(3)])
+    o = oracle_acc(tris, infos, st, 3, seeds)
+    saved = (ptamd.tune_get("trace.blocks"), ptamd.tune_get("trace.refill"))
+    try:
+        for blocks, refill in ((1, 64), (17, 5), (4096, 40)):
+            say(f"[render tuning] blocks={blocks} refill={refill}")
+            ptamd.tune_set("trace.blocks", blocks)
+            ptamd.tune_set("trace.refill", refill)
+            ctx.reset_frames()
+            ctx.render_passes(W, H, seeds, dirs)
+            acc, spp = ctx.read_accumulator(W, H)
+            assert spp == 3 and np.array_equal(acc, o)
+    finally:
+        ptamd.tune_set("trace.blocks", saved[0])
+        ptamd.tune_set("trace.refill", saved[1])
