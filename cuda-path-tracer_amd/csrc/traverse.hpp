@@ -235,11 +235,14 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
     return true;
 }
 
-// Persistent closest-hit loop over rays [0, nrays), handed out in pools of
-// kRayPool per wave from *ray_counter (zeroed by the caller); idle lanes are
-// refilled when fewer than `refill` (1..64) lanes of the wave are busy.
-// load(i, o, d) fetches ray i; store(i, hit) writes its result.
+// Persistent closest-hit loop over rays [0, nrays), handed out in per-wave
+// pools from *ray_counter (zeroed by the caller): 256 rays when there are at
+// least four per lane of the grid, else 64 so that a short queue still
+// spreads over every wave.  Idle lanes are refilled when fewer than `refill`
+// (1..64) lanes of the wave are busy.  load(i, o, d) fetches ray i;
+// store(i, hit) writes its result.
 constexpr uint32_t kRayPool = 256;
+constexpr uint32_t kRayPoolSmall = 64;
 
 template <bool COUNT, class Load, class Store>
 __device__ __forceinline__ void persistent_trace(const float4* __restrict__ nodes,
@@ -256,6 +259,8 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
     uint32_t ray = 0;
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool drained = false;                  // wave-uniform: no rays left to hand out
+    const uint32_t pool =
+        nrays / 4u >= gridDim.x * blockDim.x ? kRayPool : kRayPoolSmall;
     for (;;) {
         const uint64_t busy = __ballot(active);
         if (!drained && static_cast<uint32_t>(__popcll(busy)) < refill) {
@@ -263,14 +268,14 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
             while (idle != 0ull) {
                 if (pool_next >= pool_end) {
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(ray_counter, kRayPool);
+                    if (lane == 0) base = atomicAdd(ray_counter, pool);
                     base = __shfl(base, 0, 64);
                     if (base >= nrays) {
                         drained = true;
                         break;
                     }
                     pool_next = base;
-                    pool_end = min(base + kRayPool, nrays);
+                    pool_end = min(base + pool, nrays);
                 }
                 const uint32_t avail = pool_end - pool_next;
                 const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
