@@ -37,13 +37,14 @@ def main():
     ap.add_argument("--profile", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported (this build runs on one GPU; got {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
     if args.profile:
-        args.steps, args.warmup, args.no_cpu, args.no_tune = 3, 2, True, True
+        args.steps, args.warmup, args.no_cpu, args.no_tune, args.no_batch = 3, 2, True, True, True
 
     import numpy as np
 
@@ -155,6 +156,57 @@ def main():
                         "passes_per_frame": per_frame}
         log(f"[bench] render {name}: {rms / passes:.3f} ms/pass ({per_frame} passes/frame)")
 
+    # Config 3: 512 soups (log-uniform 1,024..262,144 triangles), one batched
+    # build vs one build per scene; one frame tracing every scene (a
+    # 256x256-path pass each, 8 bounces, RR from 3) vs one frame per scene.
+    batched = None
+    if not args.no_batch:
+        bt, bi, boffs = ptamd.gen_batch(512, 3)
+        bi["emission"][boffs[1] - 2:boffs[1]] = 10.0  # a few emitting triangles
+        ctx.upload_scenes(bt, boffs, bi, settings=ptamd.Settings(8, 3, 1.0, 1))
+        ctx.build_bvh()
+        b_ms = float(np.median([ctx.build_bvh() for _ in range(3)]))
+        eye, tgt = (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)
+        bw = bh = 256
+        cdirs = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, 1.0)
+        ctx.set_primary_rays(cdirs, eye)
+        scenes = np.arange(512, dtype=np.uint32)
+        seeds = np.arange(512, dtype=np.uint64)
+        ctx.render_passes(bw, bh, seeds[:1], None, scenes[:1])  # allocate queues
+        ctx.reset_frames()
+        f0, f1 = hip.Event(), hip.Event()
+        f0.record(stream)
+        ctx.render_passes(bw, bh, seeds, None, scenes)
+        f1.record(stream)
+        f1.synchronize()
+        frame_ms = f0.elapsed_ms(f1)
+        rays_b = ctx.last_frame_rays()
+        # one scene per frame (launch sequence per scene)
+        ctx.reset_frames()
+        f0.record(stream)
+        for s in range(0, 512, 8):
+            ctx.render_passes(bw, bh, seeds[s:s + 1], None, scenes[s:s + 1])
+        f1.record(stream)
+        f1.synchronize()
+        single_ms = f0.elapsed_ms(f1) * 8  # every 8th scene, scaled to 512
+        # one build per scene
+        sb = []
+        for s in range(0, 512, 16):
+            t = bt[boffs[s]:boffs[s + 1]]
+            ctx.upload_scene(t, bi)
+            sb.append(ctx.build_bvh())
+        single_build_ms = float(np.sum(sb)) * 16
+        batched = {"scenes": 512, "triangles": int(len(bt)),
+                   "build_ms": round(b_ms, 3), "scenes_per_s_built": round(512 / (b_ms * 1e-3), 1),
+                   "one_scene_per_launch_build_ms": round(single_build_ms, 3),
+                   "frame_ms": round(frame_ms, 3), "frame_rays": rays_b,
+                   "aggregate_Mrays_per_s": round(rays_b / (frame_ms * 1e-3) / 1e6, 1),
+                   "one_scene_per_frame_ms": round(single_ms, 3),
+                   "one_scene_per_frame_Mrays_per_s_est": round(rays_b / (single_ms * 1e-3) / 1e6, 1)}
+        log(f"[bench] batched: build {b_ms:.2f} ms ({512 / (b_ms * 1e-3):.0f} scenes/s) vs "
+            f"{single_build_ms:.2f} ms one per launch; frame {frame_ms:.2f} ms "
+            f"({batched['aggregate_Mrays_per_s']} Mrays/s) vs {single_ms:.2f} ms one scene per frame")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -207,6 +259,7 @@ def main():
         "tuning": tuning,
         "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
         "render": render,
+        "batched_scenes": batched,
         "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),
                       "tri_tests_per_ray": round(tri_tests / n, 3),
                       "algorithmic_bytes_per_ray": round(bytes_per_launch / n, 1), "hit_rate": round(hit_rate, 4)},

@@ -107,54 +107,96 @@ int pt_synchronize(pt_context* ctx) {
     return guarded(ctx, [&] { PT_HIP(hipStreamSynchronize(ctx->stream)); });
 }
 
+namespace {
+// Scene::upload (Scene.cu:28-55) for one or more scenes: deep copy, per-scene
+// enclosing boxes, then the LBVH build.  boxes: 6 floats per scene (min.xyz,
+// max.xyz) or NULL for the union of each scene's triangle boxes.
+void upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* offs, uint32_t nscenes,
+                   const pt_triangle_info* infos, uint32_t ninfos, const pt_dir_light* lights,
+                   uint32_t nlights, const float* boxes, const pt_settings* settings) {
+    require(nscenes >= 1 && offs != nullptr, "at least one scene");
+    require(offs[0] == 0, "scene offsets must start at 0");
+    for (uint32_t k = 0; k < nscenes; ++k) require(offs[k] <= offs[k + 1], "scene offsets must not decrease");
+    const uint32_t n = offs[nscenes];
+    require(n == 0 || tris != nullptr, "tris is NULL");
+    require(ninfos == 0 || infos != nullptr, "infos is NULL");
+    require(nlights == 0 || lights != nullptr, "lights is NULL");
+    require(n < (1u << 30), "at most 2^30 - 1 triangles");
+    uint32_t max_id = 0;
+    for (uint32_t i = 0; i < n; ++i) max_id = std::max(max_id, tris[i].id);
+    require(n == 0 || (infos != nullptr && ninfos > max_id),
+            "infos must hold a record for every triangle id");
+    pt::DeviceScene& s = ctx->scene;
+    hipStream_t st = ctx->stream;
+    PT_HIP(hipStreamSynchronize(st));
+    s.ready = false;
+    s.n = n;
+    s.ninfos = ninfos;
+    s.off_host.assign(offs, offs + nscenes + 1);
+    s.box_host.assign(static_cast<size_t>(nscenes) * 6, 0.0f);
+    std::vector<float> mbox(static_cast<size_t>(nscenes) * 6);
+    for (uint32_t k = 0; k < nscenes; ++k) {
+        float* b = &s.box_host[6 * static_cast<size_t>(k)];
+        if (boxes) {
+            std::memcpy(b, boxes + 6 * static_cast<size_t>(k), 6 * sizeof(float));
+        } else {
+            pt_scene_bounds(tris + offs[k], offs[k + 1] - offs[k], b, b + 3);
+        }
+        // invScale = 1 / scale on the host (BVHBuilder.cu:15-17).
+        for (int a = 0; a < 3; ++a) {
+            mbox[6 * static_cast<size_t>(k) + a] = b[a];
+            mbox[6 * static_cast<size_t>(k) + 3 + a] = 1.0f / (b[3 + a] - b[a]);
+        }
+    }
+    s.off.reserve((nscenes + 1) * sizeof(uint32_t));
+    s.morton_box.reserve(mbox.size() * sizeof(float));
+    PT_HIP(hipMemcpyAsync(s.off.as<void>(), offs, (nscenes + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    PT_HIP(hipMemcpyAsync(s.morton_box.as<void>(), mbox.data(), mbox.size() * sizeof(float),
+                          hipMemcpyHostToDevice, st));
+    if (n) {
+        s.tris.reserve(sizeof(pt_triangle) * static_cast<size_t>(n));
+        PT_HIP(hipMemcpyAsync(s.tris.as<void>(), tris, sizeof(pt_triangle) * static_cast<size_t>(n),
+                              hipMemcpyHostToDevice, st));
+    }
+    if (ninfos) {
+        s.infos.reserve(sizeof(pt_triangle_info) * static_cast<size_t>(ninfos));
+        PT_HIP(hipMemcpyAsync(s.infos.as<void>(), infos, sizeof(pt_triangle_info) * static_cast<size_t>(ninfos),
+                              hipMemcpyHostToDevice, st));
+    }
+    if (settings) ctx->renderer.settings = *settings;
+    ctx->renderer.lights = nlights;
+    pt::build_lbvh(s, st);
+    PT_HIP(hipStreamSynchronize(st));
+    s.ready = true;
+    // CUDAPathTracer::upload resets the frame counter (cuda_path_tracer.cu:469).
+    pt::render_reset(ctx->renderer, st);
+    PT_HIP(hipStreamSynchronize(st));
+}
+}  // namespace
+
 int pt_upload_scene(pt_context* ctx, const pt_triangle* tris, uint32_t n,
                     const pt_triangle_info* infos, uint32_t ninfos, const pt_dir_light* lights,
                     uint32_t nlights, const float* bmin, const float* bmax,
                     const pt_settings* settings) {
     if (!ctx) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
-        require(n == 0 || tris != nullptr, "tris is NULL");
-        require(ninfos == 0 || infos != nullptr, "infos is NULL");
-        require(nlights == 0 || lights != nullptr, "lights is NULL");
         require((bmin == nullptr) == (bmax == nullptr), "bmin and bmax must both be given or both NULL");
-        require(n < (1u << 30), "at most 2^30 - 1 triangles");
-        uint32_t max_id = 0;
-        for (uint32_t i = 0; i < n; ++i) max_id = std::max(max_id, tris[i].id);
-        require(n == 0 || (infos != nullptr && ninfos > max_id),
-                "infos must hold a record for every triangle id");
-        pt::DeviceScene& s = ctx->scene;
-        hipStream_t st = ctx->stream;
-        s.ready = false;
-        s.n = n;
-        s.ninfos = ninfos;
-        // Scene::upload (Scene.cu:28-55): deep copy, then the BVH build.
+        const uint32_t offs[2] = {0, n};
+        float box[6];
         if (bmin) {
-            for (int k = 0; k < 3; ++k) {
-                s.bmin[k] = bmin[k];
-                s.bmax[k] = bmax[k];
-            }
-        } else {
-            pt_scene_bounds(tris, n, s.bmin, s.bmax);
+            std::memcpy(box, bmin, 3 * sizeof(float));
+            std::memcpy(box + 3, bmax, 3 * sizeof(float));
         }
-        if (n) {
-            s.tris.reserve(sizeof(pt_triangle) * static_cast<size_t>(n));
-            PT_HIP(hipMemcpyAsync(s.tris.as<void>(), tris, sizeof(pt_triangle) * static_cast<size_t>(n),
-                                  hipMemcpyHostToDevice, st));
-        }
-        if (ninfos) {
-            s.infos.reserve(sizeof(pt_triangle_info) * static_cast<size_t>(ninfos));
-            PT_HIP(hipMemcpyAsync(s.infos.as<void>(), infos,
-                                  sizeof(pt_triangle_info) * static_cast<size_t>(ninfos),
-                                  hipMemcpyHostToDevice, st));
-        }
-        if (settings) ctx->renderer.settings = *settings;
-        ctx->renderer.lights = nlights;
-        pt::build_lbvh(s, st);
-        PT_HIP(hipStreamSynchronize(st));
-        s.ready = true;
-        // CUDAPathTracer::upload resets the frame counter (cuda_path_tracer.cu:469).
-        pt::render_reset(ctx->renderer, st);
-        PT_HIP(hipStreamSynchronize(st));
+        upload_scenes(ctx, tris, offs, 1, infos, ninfos, lights, nlights, bmin ? box : nullptr, settings);
+    });
+}
+
+int pt_upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* scene_offsets,
+                     uint32_t nscenes, const pt_triangle_info* infos, uint32_t ninfos,
+                     const float* boxes, const pt_settings* settings) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        upload_scenes(ctx, tris, scene_offsets, nscenes, infos, ninfos, nullptr, 0, boxes, settings);
     });
 }
 
@@ -180,10 +222,16 @@ int pt_build_bvh(pt_context* ctx, float* ms_out) {
 
 int pt_get_bvh(pt_context* ctx, uint32_t* morton, uint32_t* perm, uint32_t* children,
                uint32_t* parents_leaf, uint32_t* parents_internal, float* bboxes) {
+    return pt_get_bvh_scene(ctx, 0, morton, perm, children, parents_leaf, parents_internal, bboxes);
+}
+
+int pt_get_bvh_scene(pt_context* ctx, uint32_t scene, uint32_t* morton, uint32_t* perm,
+                     uint32_t* children, uint32_t* parents_leaf, uint32_t* parents_internal,
+                     float* bboxes) {
     if (!ctx) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
-        pt::dump_bvh(ctx->scene, ctx->stream, morton, perm, children, parents_leaf,
+        pt::dump_bvh(ctx->scene, scene, ctx->stream, morton, perm, children, parents_leaf,
                      parents_internal, bboxes);
     });
 }
@@ -191,11 +239,24 @@ int pt_get_bvh(pt_context* ctx, uint32_t* morton, uint32_t* perm, uint32_t* chil
 int pt_get_scene_info(pt_context* ctx, uint32_t* ntris, float* bmin, float* bmax) {
     if (!ctx) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
-        if (ntris) *ntris = ctx->scene.n;
+        const pt::DeviceScene& s = ctx->scene;
+        if (ntris) *ntris = s.n;
         for (int k = 0; k < 3; ++k) {
-            if (bmin) bmin[k] = ctx->scene.bmin[k];
-            if (bmax) bmax[k] = ctx->scene.bmax[k];
+            const bool have = s.box_host.size() >= 6;
+            if (bmin) bmin[k] = have ? s.box_host[k] : 0.0f;
+            if (bmax) bmax[k] = have ? s.box_host[3 + k] : 0.0f;
         }
+    });
+}
+
+int pt_trace_closest_scenes_device(pt_context* ctx, const float* d_rays, const uint32_t* d_scene,
+                                   uint32_t n, uint32_t* d_id, float* d_t, float* d_uv, void* stream) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        require(n == 0 || (d_rays != nullptr && d_scene != nullptr), "rays or scene ids NULL");
+        require(n < (1u << 31), "at most 2^31 - 1 rays per call");
+        pt::trace_closest_scenes(ctx->scene, d_rays, d_scene, n, d_id, d_t, d_uv, pick(ctx, stream));
     });
 }
 
@@ -270,14 +331,14 @@ int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed) {
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
         PT_HIP(hipEventRecord(ctx->ev0, st));
-        pt::render_passes(ctx->renderer, ctx->scene, 1, nullptr, &pass_seed, st);
+        pt::render_passes(ctx->renderer, ctx->scene, 1, nullptr, &pass_seed, nullptr, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
         ctx->renderer.timed = true;
     });
 }
 
 int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const float* dirs,
-                     const uint64_t* seeds) {
+                     const uint64_t* seeds, const uint32_t* scenes) {
     if (!ctx || !seeds) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
@@ -286,10 +347,15 @@ int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const floa
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
         PT_HIP(hipEventRecord(ctx->ev0, st));
-        pt::render_passes(ctx->renderer, ctx->scene, npasses, dirs, seeds, st);
+        pt::render_passes(ctx->renderer, ctx->scene, npasses, dirs, seeds, scenes, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
         ctx->renderer.timed = true;
     });
+}
+
+int pt_last_frame_rays(pt_context* ctx, uint64_t* rays) {
+    if (!ctx || !rays) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { *rays = pt::render_last_rays(ctx->renderer, ctx->stream); });
 }
 
 int pt_reset_frames(pt_context* ctx) {

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -54,6 +55,10 @@ class DevBuf {
     template <class T>
     T* as() const { return static_cast<T*>(ptr_); }
     size_t capacity() const { return cap_; }
+    friend void swap(DevBuf& a, DevBuf& b) noexcept {
+        std::swap(a.ptr_, b.ptr_);
+        std::swap(a.cap_, b.cap_);
+    }
 
    private:
     void* ptr_ = nullptr;
@@ -97,6 +102,7 @@ class HostBuf {
 struct DeviceBvh {
     uint32_t n = 0;
     DevBuf keys, vals, keys_alt, vals_alt;  // sorted Morton keys, perm (+ ping-pong)
+    DevBuf scene_of;                        // scene of each leaf slot
     DevBuf children, parents_leaf, parents_internal, counters;
     DevBuf nodes, tris, root_box;
     DevBuf tile_hist, digit_total;  // radix sort scratch
@@ -104,14 +110,20 @@ struct DeviceBvh {
     DevBuf ray_counter;                      // persistent traversal: next ray
 };
 
+// One or more scenes: scene s owns triangles [off[s], off[s+1]) and its own
+// LBVH over them (internal node off[s] is its root).
 struct DeviceScene {
     uint32_t n = 0;
     uint32_t ninfos = 0;
-    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
-    DevBuf tris;   // pt_triangle[n] as uploaded
-    DevBuf infos;  // pt_triangle_info[ninfos]
+    std::vector<uint32_t> off_host{0, 0};  // nscenes + 1 offsets
+    std::vector<float> box_host;           // per scene: min.xyz, max.xyz
+    DevBuf off;                            // device copy of off_host
+    DevBuf morton_box;                     // per scene: min.xyz, 1/(max-min).xyz
+    DevBuf tris;                           // pt_triangle[n] as uploaded
+    DevBuf infos;                          // pt_triangle_info[ninfos]
     DeviceBvh bvh;
     bool ready = false;
+    uint32_t nscenes() const { return static_cast<uint32_t>(off_host.size()) - 1; }
 };
 
 // Builder timings (ms) of the last build, from HIP events.
@@ -121,10 +133,10 @@ struct BuildTimes {
 
 // LBVH build: Morton keys -> 4-pass LDS radix sort -> radix tree -> refit.
 void build_lbvh(DeviceScene& scene, hipStream_t stream);
-// Parity dump helpers (device -> host, synchronous).
-void dump_bvh(DeviceScene& scene, hipStream_t stream, uint32_t* morton, uint32_t* perm,
-              uint32_t* children, uint32_t* parents_leaf, uint32_t* parents_internal,
-              float* bboxes);
+// Parity dump of one scene in local indices (device -> host, synchronous).
+void dump_bvh(DeviceScene& scene, uint32_t scene_index, hipStream_t stream, uint32_t* morton,
+              uint32_t* perm, uint32_t* children, uint32_t* parents_leaf,
+              uint32_t* parents_internal, float* bboxes);
 
 // Launch shape of the persistent closest-hit kernel (tunable).
 struct TraceTuning {
@@ -136,6 +148,9 @@ TraceTuning& trace_tuning();
 // Closest hit over a device ray batch (rays: 6 floats each).
 void trace_closest(const DeviceScene& scene, const float* d_rays, uint32_t n, uint32_t* d_id,
                    float* d_t, float* d_uv, hipStream_t stream);
+// Closest hit with a scene index per ray (batched scenes).
+void trace_closest_scenes(const DeviceScene& scene, const float* d_rays, const uint32_t* d_scene,
+                          uint32_t n, uint32_t* d_id, float* d_t, float* d_uv, hipStream_t stream);
 // Same traversal, also counting inner iterations, box tests and triangle
 // tests into d_counters[3] (uint64, accumulated).
 void trace_closest_counted(const DeviceScene& scene, const float* d_rays, uint32_t n,
@@ -148,9 +163,9 @@ struct PassParams {
     float eye[3];
     uint32_t pass;   // pass index since the last reset (stratum, RNG key)
     uint32_t seedw;  // 32-bit pass seed (RNG key)
-    uint32_t pad;
+    uint32_t scene;  // scene of a batch traced by this pass
 };
-constexpr uint32_t kMaxPasses = 64;  // passes per frame
+constexpr uint32_t kMaxPasses = 1024;  // passes per frame
 constexpr int kParamRing = 4;        // pinned PassParams slots in flight
 
 // Progressive renderer state (wavefront queues + FP64 accumulator).
@@ -172,6 +187,7 @@ struct Renderer {
     alignas(8) char graph_key[160] = {};
     uint32_t lights = 0;  // directional lights in the scene (not rendered)
     bool timed = false;   // a pass has been bracketed by the context events
+    uint32_t last_levels = 0;  // queue levels of the last frame
     Renderer() = default;
     Renderer(const Renderer&) = delete;
     Renderer& operator=(const Renderer&) = delete;
@@ -188,7 +204,9 @@ void render_reset(Renderer& r, hipStream_t stream);
 // npasses 1-spp passes in one frame; dirs = 12 floats per pass (NULL: the
 // renderer's corner rays for every pass); seeds[npasses].
 void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, const float* dirs,
-                   const uint64_t* seeds, hipStream_t stream);
+                   const uint64_t* seeds, const uint32_t* scenes, hipStream_t stream);
+// Rays traced by the last frame (sum of its queue lengths); synchronises.
+uint64_t render_last_rays(Renderer& r, hipStream_t stream);
 
 }  // namespace pt
 

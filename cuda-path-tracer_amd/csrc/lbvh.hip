@@ -1,10 +1,16 @@
 // LBVH construction on gfx950 (reference: CUDA::BVHBuilder::build,
-// BVHBuilder.cu:207-269, after Karras 2012).
+// BVHBuilder.cu:207-269, after Karras 2012), for one scene or for a batch of
+// scenes at once.  Scenes are contiguous segments [off[s], off[s+1]) of the
+// triangle array; every array below is indexed by global leaf slot, and the
+// internal nodes of scene s are numbered off[s] + local index (slot
+// off[s+1]-1 is unused), so each node's index lies inside its own leaf range.
+// Child and parent indices are global; the parity dump converts to local.
 //
-//   k_morton        30-bit Morton key + identity index per triangle
+//   k_morton        30-bit Morton key (the scene's box) + index + scene id
 //   k_sort_hist     \
-//   k_sort_rowscan   } x4: LSD radix sort, 8-bit digits, stable
-//   k_sort_scatter  /     (wave64 ballot match + LDS staging per 4096-key tile)
+//   k_sort_rowscan   } LSD radix sort, 8-bit digits, stable: 4 passes over
+//   k_sort_scatter  /  the key, then 1-2 over the scene id when batched
+//                      (wave64 ballot match + LDS staging per 4096-key tile)
 //   k_radix_tree    Karras split per internal node, parents, counter reset,
 //                   list of nodes whose leaf range crosses a refit block
 //   k_refit_local   sorted triangle records + bottom-up AABBs inside each
@@ -63,26 +69,44 @@ __device__ __forceinline__ uint32_t quantize1024(float x) {
     return static_cast<uint32_t>(static_cast<int32_t>(s));
 }
 
+// Scene of leaf slot i: the last s with off[s] <= i (off has nscenes + 1 entries).
+__device__ __forceinline__ uint32_t scene_of_slot(const uint32_t* __restrict__ off, uint32_t nscenes,
+                                                  uint32_t i) {
+    uint32_t lo = 0, hi = nscenes - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // morton_encoder (BVHBuilder.cu:12-32): centroid ((v0 + v1) + v2) / 3, mapped
-// into the enclosing box with invScale computed on the host.
+// into the scene's enclosing box with invScale computed on the host
+// (box[6*s..] = min.xyz, invScale.xyz).  Also zeroes the refit list counters.
 __global__ __launch_bounds__(256) void k_morton(const pt_triangle* __restrict__ tris, uint32_t n,
-                                                float mnx, float mny, float mnz, float ivx,
-                                                float ivy, float ivz, uint32_t* __restrict__ keys,
+                                                const uint32_t* __restrict__ off, uint32_t nscenes,
+                                                const float* __restrict__ box,
+                                                uint32_t* __restrict__ keys,
                                                 uint32_t* __restrict__ vals,
+                                                uint32_t* __restrict__ scene_of,
                                                 uint32_t* __restrict__ span_count,
                                                 uint32_t n_span_counts) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_span_counts) span_count[i] = 0u;
     if (i >= n) return;
+    const uint32_t sc = scene_of_slot(off, nscenes, i);
+    const float* bx = box + 6 * static_cast<size_t>(sc);
     const float* t = reinterpret_cast<const float*>(tris + i);
     float cx = ((t[0] + t[3]) + t[6]) / 3.0f;
     float cy = ((t[1] + t[4]) + t[7]) / 3.0f;
     float cz = ((t[2] + t[5]) + t[8]) / 3.0f;
-    uint32_t qx = quantize1024((cx - mnx) * ivx);
-    uint32_t qy = quantize1024((cy - mny) * ivy);
-    uint32_t qz = quantize1024((cz - mnz) * ivz);
+    uint32_t qx = quantize1024((cx - bx[0]) * bx[3]);
+    uint32_t qy = quantize1024((cy - bx[1]) * bx[4]);
+    uint32_t qz = quantize1024((cz - bx[2]) * bx[5]);
     keys[i] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
     vals[i] = i;
+    scene_of[i] = sc;
 }
 
 // ---- radix sort ---------------------------------------------------------------
@@ -109,8 +133,19 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_
     return pre + x - v;
 }
 
+// Digit of an element: bits [shift, shift+8) of its key, or -- in the
+// batched build's last passes -- of the scene id of its original slot.
+template <bool BY_SCENE>
+__device__ __forceinline__ uint32_t sort_digit(uint32_t k, uint32_t v,
+                                               const uint32_t* __restrict__ scene_of, int shift) {
+    return ((BY_SCENE ? scene_of[v] : k) >> shift) & 255u;
+}
+
 // Per-tile digit histogram: tile_hist[digit * num_tiles + tile].
+template <bool BY_SCENE>
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ vals,
+                                                            const uint32_t* __restrict__ scene_of,
                                                             uint32_t n, int shift,
                                                             uint32_t num_tiles,
                                                             uint32_t* __restrict__ tile_hist) {
@@ -122,7 +157,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __re
 #pragma unroll 4
     for (int r = 0; r < kSortKeysPerThread; ++r) {
         uint32_t i = base + r * kSortThreads + tid;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[sort_digit<BY_SCENE>(keys[i], BY_SCENE ? vals[i] : 0u, scene_of, shift)], 1u);
     }
     __syncthreads();
     tile_hist[static_cast<size_t>(tid) * num_tiles + blockIdx.x] = h[tid];
@@ -152,9 +187,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_rowscan(uint32_t* __restr
 // found with eight ballots, their order is the popcount of lower peer lanes,
 // and a per-wave LDS counter per digit carries the count between rounds.
 // Keys are then staged in LDS in digit order and written out in runs.
+template <bool BY_SCENE>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
-    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    const uint32_t* __restrict__ scene_of, uint32_t n, int shift,
     uint32_t num_tiles, const uint32_t* __restrict__ tile_hist,
     const uint32_t* __restrict__ digit_total) {
     __shared__ uint32_t s_wc[4][256];
@@ -177,13 +214,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint64_t lt = (1ull << lane) - 1ull;
 
     uint32_t key[kSortKeysPerThread], val[kSortKeysPerThread], rk[kSortKeysPerThread];
+    uint32_t dig[kSortKeysPerThread];
 #pragma unroll
     for (int r = 0; r < kSortKeysPerThread; ++r) {
         const uint32_t li = wave * (kSortKeysPerThread * 64) + r * 64 + lane;
         const bool valid = li < count;
         const uint32_t k = valid ? keys_in[base + li] : 0u;
         const uint32_t v = valid ? vals_in[base + li] : 0u;
-        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t d = valid ? sort_digit<BY_SCENE>(k, v, scene_of, shift) : 0u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -199,6 +237,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
         }
         key[r] = k;
         val[r] = v;
+        dig[r] = d;
         rk[r] = old + rank;
     }
     __syncthreads();
@@ -221,7 +260,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     for (int r = 0; r < kSortKeysPerThread; ++r) {
         const uint32_t li = wave * (kSortKeysPerThread * 64) + r * 64 + lane;
         if (li < count) {
-            const uint32_t d = (key[r] >> shift) & 255u;
+            const uint32_t d = dig[r];
             const uint32_t pos = s_dstart[d] + s_wc[wave][d] + rk[r];
             s_keys[pos] = key[r];
             s_vals[pos] = val[r];
@@ -230,10 +269,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __syncthreads();
     for (uint32_t i = tid; i < count; i += kSortThreads) {
         const uint32_t k = s_keys[i];
-        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t v = s_vals[i];
+        const uint32_t d = sort_digit<BY_SCENE>(k, v, scene_of, shift);
         const uint32_t g = s_gbase[d] + (i - s_dstart[d]);
         keys_out[g] = k;
-        vals_out[g] = s_vals[i];
+        vals_out[g] = v;
     }
 }
 
@@ -250,12 +290,15 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
 }
 
 // radix_tree_construction_kernel (BVHBuilder.cu:135-196), one thread per
-// internal node.  Also initialises the node's refit counter with its refit
-// level (from its leaf range [min(i,j), max(i,j)]) and appends level-1 nodes
-// to their 65536-block's list (span_list[block * kMidCap ...], count in
-// span_count[1 + block]) and level-2 nodes to the top list (span_top, count
-// in span_count[0]).
+// internal node, on its scene's segment of sorted keys with local indices
+// (the index augmentation of delta uses them, as in a single-scene build).
+// Also initialises the node's refit counter with its refit level (from its
+// global leaf range) and appends level-1 nodes to their 65536-block's list
+// (span_list[block * kMidCap ...], count in span_count[1 + block]) and level-2
+// nodes to the top list (span_top, count in span_count[0]).
 __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__ keys, uint32_t n,
+                                                    const uint32_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ scene_of,
                                                     uint32_t* __restrict__ children,
                                                     uint32_t* __restrict__ parents_leaf,
                                                     uint32_t* __restrict__ parents_internal,
@@ -264,56 +307,61 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ span_list,
                                                     uint32_t* __restrict__ span_top,
                                                     uint32_t* __restrict__ span_count) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t N = n;
-    if (i >= N - 1) return;
-    const uint32_t ki = keys[i];
-    const int dr = delta(keys, N, i, ki, i + 1) - delta(keys, N, i, ki, i - 1);
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t sc = scene_of[g];
+    const uint32_t base = off[sc];
+    const int64_t N = off[sc + 1] - base;
+    const int64_t i = g - base;
+    if (i >= N - 1) return;  // unused slot of the scene (or a 1-triangle scene)
+    const uint32_t* sk = keys + base;
+    const uint32_t ki = sk[i];
+    const int dr = delta(sk, N, i, ki, i + 1) - delta(sk, N, i, ki, i - 1);
     const int64_t d = (dr > 0) - (dr < 0);
-    const int dmin = delta(keys, N, i, ki, i - d);
+    const int dmin = delta(sk, N, i, ki, i - d);
     int64_t lmax = 2;
-    while (delta(keys, N, i, ki, i + lmax * d) > dmin) lmax *= 2;
+    while (delta(sk, N, i, ki, i + lmax * d) > dmin) lmax *= 2;
     int64_t l = 0;
     while (lmax > 1) {
         lmax /= 2;
-        if (delta(keys, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
+        if (delta(sk, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
     }
     const int64_t j = i + l * d;
-    const int dnode = delta(keys, N, i, ki, j);
+    const int dnode = delta(sk, N, i, ki, j);
     int64_t s = 0;
     while (l > 1) {
         l = (l + 1) / 2;
-        if (delta(keys, N, i, ki, i + (s + l) * d) > dnode) s += l;
+        if (delta(sk, N, i, ki, i + (s + l) * d) > dnode) s += l;
     }
     s = i + s * d + (d < 0 ? d : 0);
     uint32_t type = 0;
     if ((i < j ? i : j) != s) {
         type |= 1u;
-        parents_internal[s] = static_cast<uint32_t>(i);
+        parents_internal[base + s] = g;
     } else {
-        parents_leaf[s] = static_cast<uint32_t>(i);
+        parents_leaf[base + s] = g;
     }
     if ((i > j ? i : j) != s + 1) {
         type |= 2u;
-        parents_internal[s + 1] = static_cast<uint32_t>(i);
+        parents_internal[base + s + 1] = g;
     } else {
-        parents_leaf[s + 1] = static_cast<uint32_t>(i);
+        parents_leaf[base + s + 1] = g;
     }
-    const uint32_t w = (static_cast<uint32_t>(s) << 2) | type;
-    children[i] = w;
-    nodes[4 * i + 3] = make_float4(__uint_as_float(w), 0.0f, 0.0f, 0.0f);
-    const int64_t lo = i < j ? i : j, hi = i > j ? i : j;
+    const uint32_t w = ((base + static_cast<uint32_t>(s)) << 2) | type;
+    children[g] = w;
+    nodes[4 * static_cast<size_t>(g) + 3] = make_float4(__uint_as_float(w), 0.0f, 0.0f, 0.0f);
+    const int64_t lo = base + (i < j ? i : j), hi = base + (i > j ? i : j);
     uint32_t level = 0;
     if ((lo / kRefitBlock) != (hi / kRefitBlock)) level = ((lo / kMidBlock) != (hi / kMidBlock)) ? 2u : 1u;
-    counters[i] = level << kLevelShift;
+    counters[g] = level << kLevelShift;
     if (level == 1) {
         const uint32_t blk = static_cast<uint32_t>(lo / kMidBlock);
         const uint32_t slot = atomicAdd(span_count + 1 + blk, 1u);
-        if (slot < kMidCap) span_list[static_cast<size_t>(blk) * kMidCap + slot] = static_cast<uint32_t>(i);
+        if (slot < kMidCap) span_list[static_cast<size_t>(blk) * kMidCap + slot] = g;
     } else if (level == 2) {
-        span_top[atomicAdd(span_count, 1u)] = static_cast<uint32_t>(i);
+        span_top[atomicAdd(span_count, 1u)] = g;
     }
-    if (i == 0) parents_internal[0] = kInvalid;  // BVHBuilder.cu:224
+    if (i == 0) parents_internal[g] = kInvalid;  // BVHBuilder.cu:224
 }
 
 // ---- refit ------------------------------------------------------------------------
@@ -373,6 +421,7 @@ struct LdsBoxes {
 // parent's traversal record.
 __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     const pt_triangle* __restrict__ in_tris, const uint32_t* __restrict__ perm,
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ scene_of,
     const uint32_t* __restrict__ children, uint32_t* __restrict__ parents_leaf,
     const uint32_t* __restrict__ parents_internal, const uint32_t* __restrict__ counters,
     float4* __restrict__ nodes, float4* __restrict__ tri_recs, float* __restrict__ root_box,
@@ -403,9 +452,10 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     b.mxx = gmax(gmax(gmax(-kFltMax, v0.x), v1.x), v2.x);
     b.mxy = gmax(gmax(gmax(-kFltMax, v0.y), v1.y), v2.y);
     b.mxz = gmax(gmax(gmax(-kFltMax, v0.z), v1.z), v2.z);
-    if (n == 1) {
-        parents_leaf[0] = kInvalid;
-        store_root(root_box, b);
+    const uint32_t sc = scene_of[k];
+    if (off[sc + 1] - off[sc] == 1) {  // one-triangle scene: no internal node
+        parents_leaf[k] = kInvalid;
+        store_root(root_box + 6 * static_cast<size_t>(sc), b);
         return;
     }
     s_leaf.put(tid, b);
@@ -426,7 +476,7 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
         s_node.put(p - base, b);
         const uint32_t pp = parents_internal[p];
         if (pp == kInvalid) {
-            store_root(root_box, b);
+            store_root(root_box + 6 * static_cast<size_t>(scene_of[p]), b);
             return;
         }
         store_slot(nodes, pp, (children[pp] >> 2) == p, b);
@@ -443,6 +493,7 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
 // listed node is one "arrival"; the thread completing a node climbs on while
 // the parent is of the same level.
 __device__ __forceinline__ void refit_level(const uint32_t* __restrict__ children,
+                                            const uint32_t* __restrict__ scene_of,
                                             const uint32_t* __restrict__ parents_internal,
                                             uint32_t* __restrict__ counters,
                                             float4* __restrict__ nodes,
@@ -469,7 +520,7 @@ __device__ __forceinline__ void refit_level(const uint32_t* __restrict__ childre
                 const Box b = unite(lb, rb);
                 const uint32_t pp = parents_internal[p];
                 if (pp == kInvalid) {
-                    store_root(root_box, b);
+                    store_root(root_box + 6 * static_cast<size_t>(scene_of[p]), b);
                     break;
                 }
                 store_slot(nodes, pp, (children[pp] >> 2) == p, b);
@@ -482,6 +533,7 @@ __device__ __forceinline__ void refit_level(const uint32_t* __restrict__ childre
 
 // Level-1 nodes: one workgroup per 65536-leaf block.
 __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ scene_of,
                                                     const uint32_t* __restrict__ parents_internal,
                                                     uint32_t* __restrict__ counters,
                                                     float4* __restrict__ nodes,
@@ -489,43 +541,62 @@ __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ span_list,
                                                     const uint32_t* __restrict__ span_count) {
     const uint32_t blk = blockIdx.x;
-    refit_level(children, parents_internal, counters, nodes, root_box,
+    refit_level(children, scene_of, parents_internal, counters, nodes, root_box,
                 span_list + static_cast<size_t>(blk) * kMidCap, min(span_count[1 + blk], kMidCap), 1u);
 }
 
 // Level-2 nodes: one workgroup.
 __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ scene_of,
                                                     const uint32_t* __restrict__ parents_internal,
                                                     uint32_t* __restrict__ counters,
                                                     float4* __restrict__ nodes,
                                                     float* __restrict__ root_box,
                                                     const uint32_t* __restrict__ span_top,
                                                     const uint32_t* __restrict__ span_count) {
-    refit_level(children, parents_internal, counters, nodes, root_box, span_top, span_count[0], 2u);
+    refit_level(children, scene_of, parents_internal, counters, nodes, root_box, span_top, span_count[0], 2u);
 }
 
-// Parity dump: internal node boxes in the reference's BBox layout
-// (min.xyz, max.xyz), read back from the parents' records.
-__global__ __launch_bounds__(256) void k_dump_boxes(const float4* __restrict__ nodes,
+// Parity dump of scene [base, base + n): internal node boxes in the
+// reference's BBox layout (min.xyz, max.xyz), read back from the parents'
+// records, plus the child and parent words converted to local indices.
+__global__ __launch_bounds__(256) void k_dump_scene(const float4* __restrict__ nodes,
                                                     const uint32_t* __restrict__ children,
+                                                    const uint32_t* __restrict__ parents_leaf,
                                                     const uint32_t* __restrict__ parents_internal,
-                                                    const float* __restrict__ root_box,
-                                                    uint32_t n, float* __restrict__ out) {
+                                                    const float* __restrict__ root_box, uint32_t base,
+                                                    uint32_t n, float* __restrict__ out_boxes,
+                                                    uint32_t* __restrict__ out_children,
+                                                    uint32_t* __restrict__ out_pl,
+                                                    uint32_t* __restrict__ out_pi) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) {
+        const uint32_t p = parents_leaf[base + c];
+        out_pl[c] = p == kInvalid ? kInvalid : p - base;
+    }
     if (c + 1 >= n) return;
-    float* o = out + 6 * static_cast<size_t>(c);
-    if (c == 0) {
+    const uint32_t g = base + c;
+    out_children[c] = children[g] - (base << 2);
+    const uint32_t pg = parents_internal[g];
+    out_pi[c] = pg == kInvalid ? kInvalid : pg - base;
+    float* o = out_boxes + 6 * static_cast<size_t>(c);
+    if (pg == kInvalid) {
         for (int k = 0; k < 6; ++k) o[k] = root_box[k];
         return;
     }
-    const uint32_t p = parents_internal[c];
-    const float4* rec = nodes + 4 * static_cast<size_t>(p);
+    const float4* rec = nodes + 4 * static_cast<size_t>(pg);
     const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    if ((children[p] >> 2) == c) {
+    if ((children[pg] >> 2) == g) {
         o[0] = r0.x; o[1] = r0.z; o[2] = r1.x; o[3] = r0.y; o[4] = r0.w; o[5] = r1.y;
     } else {
         o[0] = r1.z; o[1] = r2.x; o[2] = r2.z; o[3] = r1.w; o[4] = r2.y; o[5] = r2.w;
     }
+}
+
+__global__ __launch_bounds__(256) void k_sub_u32(const uint32_t* __restrict__ in, uint32_t sub,
+                                                 uint32_t n, uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i] - sub;
 }
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
@@ -535,116 +606,137 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a
 void build_lbvh(DeviceScene& s, hipStream_t st) {
     DeviceBvh& b = s.bvh;
     const uint32_t n = s.n;
+    const uint32_t nscenes = static_cast<uint32_t>(s.off_host.size()) - 1;
     b.n = n;
     if (n == 0) return;
     if (n >= (1u << 30)) throw Error(PT_E_INVALID_ARG, "LBVH supports fewer than 2^30 triangles");
     const size_t n4 = static_cast<size_t>(n) * 4;
-    const size_t m = n > 1 ? n - 1 : 1;
     b.keys.reserve(n4);
     b.vals.reserve(n4);
     b.keys_alt.reserve(n4);
     b.vals_alt.reserve(n4);
-    b.children.reserve(m * 4);
+    b.scene_of.reserve(n4);
+    b.children.reserve(n4);
     b.parents_leaf.reserve(n4);
-    b.parents_internal.reserve(m * 4);
-    b.counters.reserve(m * 4);
-    b.nodes.reserve(m * 64);
+    b.parents_internal.reserve(n4);
+    b.counters.reserve(n4);
+    b.nodes.reserve(static_cast<size_t>(n) * 64);
     b.tris.reserve(static_cast<size_t>(n) * 48);
-    b.root_box.reserve(6 * sizeof(float));
+    b.root_box.reserve(static_cast<size_t>(nscenes) * 6 * sizeof(float));
     b.ray_counter.reserve(sizeof(uint32_t));
     const uint32_t num_tiles = div_up(n, kSortTile);
     b.tile_hist.reserve(static_cast<size_t>(num_tiles) * 256 * 4);
     b.digit_total.reserve(256 * 4);
     const uint32_t mid_blocks = div_up(n, kMidBlock);
     b.span_list.reserve(static_cast<size_t>(mid_blocks) * kMidCap * 4);
-    b.span_top.reserve(m * 4);
+    b.span_top.reserve(n4);
     b.span_count.reserve((1 + static_cast<size_t>(mid_blocks)) * 4);
 
-    // invScale = 1 / scale on the host (BVHBuilder.cu:15-17).
-    float inv[3];
-    for (int k = 0; k < 3; ++k) inv[k] = 1.0f / (s.bmax[k] - s.bmin[k]);
+    uint32_t* kin = b.keys.as<uint32_t>();
+    uint32_t* vin = b.vals.as<uint32_t>();
     hipLaunchKernelGGL(k_morton, dim3(div_up(std::max<uint64_t>(n, 1 + mid_blocks), 256)), dim3(256), 0, st,
-                       s.tris.as<pt_triangle>(), n, s.bmin[0], s.bmin[1], s.bmin[2], inv[0],
-                       inv[1], inv[2], b.keys.as<uint32_t>(), b.vals.as<uint32_t>(),
+                       s.tris.as<pt_triangle>(), n, s.off.as<uint32_t>(), nscenes,
+                       s.morton_box.as<float>(), kin, vin, b.scene_of.as<uint32_t>(),
                        b.span_count.as<uint32_t>(), 1 + mid_blocks);
     PT_HIP(hipGetLastError());
 
     if (n >= 2) {
-        uint32_t* kin = b.keys.as<uint32_t>();
-        uint32_t* vin = b.vals.as<uint32_t>();
         uint32_t* kout = b.keys_alt.as<uint32_t>();
         uint32_t* vout = b.vals_alt.as<uint32_t>();
-        for (int pass = 0; pass < 4; ++pass) {
-            const int shift = 8 * pass;
-            hipLaunchKernelGGL(k_sort_hist, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, n,
-                               shift, num_tiles, b.tile_hist.as<uint32_t>());
+        // 4 passes over the 30-bit keys, then the scene id's bytes (stable:
+        // each scene keeps its key order, ties in input order).
+        int scene_passes = 0;
+        for (uint32_t v = nscenes - 1; v != 0; v >>= 8) ++scene_passes;
+        const int passes = 4 + scene_passes;
+        for (int pass = 0; pass < passes; ++pass) {
+            const bool by_scene = pass >= 4;
+            const int shift = 8 * (by_scene ? pass - 4 : pass);
+            if (by_scene) {
+                hipLaunchKernelGGL(k_sort_hist<true>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                                   b.scene_of.as<uint32_t>(), n, shift, num_tiles, b.tile_hist.as<uint32_t>());
+            } else {
+                hipLaunchKernelGGL(k_sort_hist<false>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                                   b.scene_of.as<uint32_t>(), n, shift, num_tiles, b.tile_hist.as<uint32_t>());
+            }
             hipLaunchKernelGGL(k_sort_rowscan, dim3(256), dim3(kSortThreads), 0, st,
                                b.tile_hist.as<uint32_t>(), num_tiles, b.digit_total.as<uint32_t>());
-            hipLaunchKernelGGL(k_sort_scatter, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
-                               kout, vout, n, shift, num_tiles, b.tile_hist.as<uint32_t>(),
-                               b.digit_total.as<uint32_t>());
+            if (by_scene) {
+                hipLaunchKernelGGL(k_sort_scatter<true>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                                   kout, vout, b.scene_of.as<uint32_t>(), n, shift, num_tiles,
+                                   b.tile_hist.as<uint32_t>(), b.digit_total.as<uint32_t>());
+            } else {
+                hipLaunchKernelGGL(k_sort_scatter<false>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                                   kout, vout, b.scene_of.as<uint32_t>(), n, shift, num_tiles,
+                                   b.tile_hist.as<uint32_t>(), b.digit_total.as<uint32_t>());
+            }
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
+        if (passes % 2) {
+            // Odd pass count: the result sits in the alternate buffers.
+            swap(b.keys, b.keys_alt);
+            swap(b.vals, b.vals_alt);
+        }
         PT_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n - 1, 256)), dim3(256), 0, st,
-                           b.keys.as<uint32_t>(), n, b.children.as<uint32_t>(),
+        hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n, 256)), dim3(256), 0, st, b.keys.as<uint32_t>(), n,
+                           s.off.as<uint32_t>(), b.scene_of.as<uint32_t>(), b.children.as<uint32_t>(),
                            b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
-                           b.counters.as<uint32_t>(), b.nodes.as<float4>(),
-                           b.span_list.as<uint32_t>(), b.span_top.as<uint32_t>(),
-                           b.span_count.as<uint32_t>());
+                           b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.span_list.as<uint32_t>(),
+                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>());
         PT_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_refit_local, dim3(div_up(n, kRefitBlock)), dim3(kRefitBlock), 0, st,
-                       s.tris.as<pt_triangle>(), b.vals.as<uint32_t>(), b.children.as<uint32_t>(),
+                       s.tris.as<pt_triangle>(), b.vals.as<uint32_t>(), s.off.as<uint32_t>(),
+                       b.scene_of.as<uint32_t>(), b.children.as<uint32_t>(),
                        b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                        b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.tris.as<float4>(),
                        b.root_box.as<float>(), n);
     if (n > static_cast<uint32_t>(kRefitBlock)) {
         hipLaunchKernelGGL(k_refit_mid, dim3(mid_blocks), dim3(1024), 0, st, b.children.as<uint32_t>(),
-                           b.parents_internal.as<uint32_t>(), b.counters.as<uint32_t>(),
-                           b.nodes.as<float4>(), b.root_box.as<float>(),
+                           b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
+                           b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(),
                            b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>());
     }
     if (n > kMidBlock) {
         hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
-                           b.parents_internal.as<uint32_t>(), b.counters.as<uint32_t>(),
-                           b.nodes.as<float4>(), b.root_box.as<float>(),
+                           b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
+                           b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(),
                            b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>());
     }
     PT_HIP(hipGetLastError());
 }
 
-void dump_bvh(DeviceScene& s, hipStream_t st, uint32_t* morton, uint32_t* perm,
+void dump_bvh(DeviceScene& s, uint32_t scene, hipStream_t st, uint32_t* morton, uint32_t* perm,
               uint32_t* children, uint32_t* parents_leaf, uint32_t* parents_internal,
               float* bboxes) {
     const DeviceBvh& b = s.bvh;
-    const uint32_t n = b.n;
+    if (scene + 1 >= s.off_host.size()) throw Error(PT_E_INVALID_ARG, "scene index out of range");
+    const uint32_t base = s.off_host[scene];
+    const uint32_t n = s.off_host[scene + 1] - base;
     if (n == 0) return;
     const size_t m = n - 1;
-    if (morton) PT_HIP(hipMemcpyAsync(morton, b.keys.as<void>(), n * 4ull, hipMemcpyDeviceToHost, st));
-    if (perm) PT_HIP(hipMemcpyAsync(perm, b.vals.as<void>(), n * 4ull, hipMemcpyDeviceToHost, st));
-    if (parents_leaf)
-        PT_HIP(hipMemcpyAsync(parents_leaf, b.parents_leaf.as<void>(), n * 4ull,
-                              hipMemcpyDeviceToHost, st));
+    DevBuf tmp;
+    tmp.reserve(static_cast<size_t>(n) * 4 * 4 + m * 24 + 256);
+    uint32_t* t_perm = tmp.as<uint32_t>();
+    uint32_t* t_ch = t_perm + n;
+    uint32_t* t_pl = t_ch + n;
+    uint32_t* t_pi = t_pl + n;
+    float* t_box = reinterpret_cast<float*>(t_pi + n);
+    if (morton)
+        PT_HIP(hipMemcpyAsync(morton, b.keys.as<uint32_t>() + base, n * 4ull, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_sub_u32, dim3(div_up(n, 256)), dim3(256), 0, st, b.vals.as<uint32_t>() + base,
+                       base, n, t_perm);
+    hipLaunchKernelGGL(k_dump_scene, dim3(div_up(n, 256)), dim3(256), 0, st, b.nodes.as<float4>(),
+                       b.children.as<uint32_t>(), b.parents_leaf.as<uint32_t>(),
+                       b.parents_internal.as<uint32_t>(), b.root_box.as<float>() + 6 * static_cast<size_t>(scene),
+                       base, n, t_box, t_ch, t_pl, t_pi);
+    PT_HIP(hipGetLastError());
+    if (perm) PT_HIP(hipMemcpyAsync(perm, t_perm, n * 4ull, hipMemcpyDeviceToHost, st));
+    if (parents_leaf) PT_HIP(hipMemcpyAsync(parents_leaf, t_pl, n * 4ull, hipMemcpyDeviceToHost, st));
     if (m > 0) {
-        if (children)
-            PT_HIP(hipMemcpyAsync(children, b.children.as<void>(), m * 4, hipMemcpyDeviceToHost, st));
-        if (parents_internal)
-            PT_HIP(hipMemcpyAsync(parents_internal, b.parents_internal.as<void>(), m * 4,
-                                  hipMemcpyDeviceToHost, st));
-        if (bboxes) {
-            DevBuf tmp;
-            tmp.reserve(m * 24);
-            hipLaunchKernelGGL(k_dump_boxes, dim3(div_up(m, 256)), dim3(256), 0, st,
-                               b.nodes.as<float4>(), b.children.as<uint32_t>(),
-                               b.parents_internal.as<uint32_t>(), b.root_box.as<float>(), n,
-                               tmp.as<float>());
-            PT_HIP(hipGetLastError());
-            PT_HIP(hipMemcpyAsync(bboxes, tmp.as<void>(), m * 24, hipMemcpyDeviceToHost, st));
-            PT_HIP(hipStreamSynchronize(st));
-            return;
-        }
+        if (children) PT_HIP(hipMemcpyAsync(children, t_ch, m * 4, hipMemcpyDeviceToHost, st));
+        if (parents_internal) PT_HIP(hipMemcpyAsync(parents_internal, t_pi, m * 4, hipMemcpyDeviceToHost, st));
+        if (bboxes) PT_HIP(hipMemcpyAsync(bboxes, t_box, m * 24, hipMemcpyDeviceToHost, st));
     }
     PT_HIP(hipStreamSynchronize(st));
 }

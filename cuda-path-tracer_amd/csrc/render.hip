@@ -93,18 +93,27 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
 
 // Closest hit for every ray of queue `level` (count on the device): the
 // persistent traversal, rays handed out from ray_counters[level].
-__global__ __launch_bounds__(kPersistThreads) void k_extend(const float4* __restrict__ nodes,
+// Each path traces the scene of its pass (root internal node off[scene]).
+__global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
+                                                            const PassParams* __restrict__ pp,
+                                                            const float4* __restrict__ nodes,
                                                             const float4* __restrict__ tris,
-                                                            uint32_t ntri, RayQueue q, HitQueue hq,
+                                                            const uint32_t* __restrict__ off,
+                                                            RayQueue q, HitQueue hq,
                                                             const uint32_t* __restrict__ counters,
                                                             uint32_t* __restrict__ ray_counters,
                                                             uint32_t level, uint32_t refill) {
     __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
     const uint32_t count = counters[level];
     TraceCount cnt{0, 0, 0};
-    auto load = [&](uint32_t i, V3& o, V3& d) {
+    auto load = [&](uint32_t i, V3& o, V3& d, uint32_t& single) -> uint32_t {
         o = v3(q.ox[i], q.oy[i], q.oz[i]);
         d = v3(q.dx[i], q.dy[i], q.dz[i]);
+        const uint32_t sc = pp[q.path[i] / fc.npix].scene;
+        const uint32_t b = off[sc], e = off[sc + 1];
+        if (e - b >= 2) return b;
+        if (e - b == 1) single = b;
+        return kInvalid;
     };
     auto store = [&](uint32_t i, const TraceHit& h) {
         hq.t[i] = h.t;
@@ -112,14 +121,6 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const float4* __rest
         hq.u[i] = h.u;
         hq.v[i] = h.v;
     };
-    if (ntri < 2) {
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-            V3 o, d;
-            load(i, o, d);
-            store(i, trace_ray<false>(nodes, tris, ntri, o, d, cnt));
-        }
-        return;
-    }
     persistent_trace<false>(nodes, tris, count, ray_counters + level, refill, s_stack, load, store, cnt);
 }
 
@@ -318,7 +319,7 @@ struct GraphKey {
     const void* nodes;
     const void* tris;
     const void* infos;
-    uint32_t ntri;
+    const void* off;
     const void* queues;
     const void* params;
     const void* accum;
@@ -340,9 +341,9 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt, nlev);
     for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
-        hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st,
-                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.bvh.n,
-                           L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
+        hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
+                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
+                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
         hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl);
@@ -381,10 +382,13 @@ void render_reset(Renderer& r, hipStream_t st) {
 }
 
 void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, const float* dirs,
-                   const uint64_t* seeds, hipStream_t st) {
+                   const uint64_t* seeds, const uint32_t* scenes, hipStream_t st) {
     if (!scene.ready) throw Error(PT_E_NO_SCENE, "no scene uploaded");
     if (r.settings.bounces > 60) throw Error(PT_E_INVALID_ARG, "at most 60 bounces");
-    if (npasses == 0 || npasses > kMaxPasses) throw Error(PT_E_INVALID_ARG, "1 .. 64 passes per call");
+    if (npasses == 0 || npasses > kMaxPasses) throw Error(PT_E_INVALID_ARG, "1 .. 1024 passes per call");
+    if (scenes)
+        for (uint32_t k = 0; k < npasses; ++k)
+            if (scenes[k] >= scene.nscenes()) throw Error(PT_E_INVALID_ARG, "scene index out of range");
     const size_t npix = static_cast<size_t>(r.w) * r.h;
     if (npix * npasses >= (1ull << 31)) throw Error(PT_E_INVALID_ARG, "too many paths in one frame");
     r.queues.reserve(slab_bytes(npix * npasses) * kQueueArrays);
@@ -404,7 +408,7 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         std::memcpy(hp[k].eye, r.eye, sizeof hp[k].eye);
         hp[k].pass = r.spp + k;
         hp[k].seedw = static_cast<uint32_t>(seeds[k]) ^ static_cast<uint32_t>(seeds[k] >> 32);
-        hp[k].pad = 0;
+        hp[k].scene = scenes ? scenes[k] : 0u;
     }
     PT_HIP(hipMemcpyAsync(r.params.as<void>(), hp, sizeof(PassParams) * npasses, hipMemcpyHostToDevice, st));
     PT_HIP(hipEventRecord(r.ring_ev[slot], st));
@@ -430,7 +434,7 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         key.nodes = scene.bvh.nodes.as<void>();
         key.tris = scene.bvh.tris.as<void>();
         key.infos = scene.infos.as<void>();
-        key.ntri = scene.bvh.n;
+        key.off = scene.off.as<void>();
         key.queues = r.queues.as<void>();
         key.params = r.params.as<void>();
         key.accum = r.accum.as<void>();
@@ -462,6 +466,17 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         PT_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(r.graph_exec), st));
     }
     r.spp += npasses;
+    r.last_levels = fc.bounces + 1;
+}
+
+uint64_t render_last_rays(Renderer& r, hipStream_t st) {
+    if (r.last_levels == 0) return 0;
+    uint32_t h[64];
+    PT_HIP(hipMemcpyAsync(h, r.counters.as<void>(), r.last_levels * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PT_HIP(hipStreamSynchronize(st));
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < r.last_levels; ++k) total += h[k];
+    return total;
 }
 
 RenderTuning& render_tuning() {

@@ -56,10 +56,11 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
     TraceCount cnt{0, 0, 0};
     persistent_trace<COUNT>(
         nodes, tris, nrays, ray_counter, refill, s_stack,
-        [&](uint32_t i, V3& o, V3& d) {
+        [&](uint32_t i, V3& o, V3& d, uint32_t&) -> uint32_t {
             const float* rp = rays + 6 * static_cast<size_t>(i);
             o = v3(rp[0], rp[1], rp[2]);
             d = v3(rp[3], rp[4], rp[5]);
+            return 0u;
         },
         [&](uint32_t i, const TraceHit& h) {
             if (out_id) out_id[i] = h.id;
@@ -80,6 +81,36 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
             atomicAdd(counters + 2, c);
         }
     }
+}
+
+// Persistent closest hit over rays tagged with a scene of a batch: the ray
+// starts at its scene's root (internal node off[s]).
+__global__ __launch_bounds__(kPersistThreads) void k_trace_scenes(
+    const float4* __restrict__ nodes, const float4* __restrict__ tris, const uint32_t* __restrict__ off,
+    uint32_t nscenes, const float* __restrict__ rays, const uint32_t* __restrict__ scene, uint32_t nrays,
+    uint32_t* __restrict__ out_id, float* __restrict__ out_t, float2* __restrict__ out_uv,
+    uint32_t* __restrict__ ray_counter, uint32_t refill) {
+    __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
+    TraceCount cnt{0, 0, 0};
+    persistent_trace<false>(
+        nodes, tris, nrays, ray_counter, refill, s_stack,
+        [&](uint32_t i, V3& o, V3& d, uint32_t& single) -> uint32_t {
+            const float* rp = rays + 6 * static_cast<size_t>(i);
+            o = v3(rp[0], rp[1], rp[2]);
+            d = v3(rp[3], rp[4], rp[5]);
+            const uint32_t sc = scene[i];
+            if (sc >= nscenes) return kInvalid;  // unknown scene: a miss
+            const uint32_t b = off[sc], e = off[sc + 1];
+            if (e - b >= 2) return b;
+            if (e - b == 1) single = b;
+            return kInvalid;
+        },
+        [&](uint32_t i, const TraceHit& h) {
+            if (out_id) out_id[i] = h.id;
+            if (out_t) out_t[i] = h.t;
+            if (out_uv) out_uv[i] = make_float2(h.u, h.v);
+        },
+        cnt);
 }
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
@@ -122,6 +153,21 @@ TraceTuning& trace_tuning() {
         return v;
     }();
     return t;
+}
+
+void trace_closest_scenes(const DeviceScene& s, const float* d_rays, const uint32_t* d_scene,
+                          uint32_t n, uint32_t* d_id, float* d_t, float* d_uv, hipStream_t st) {
+    if (n == 0) return;
+    const TraceTuning& tune = trace_tuning();
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s.bvh.ray_counter.as<uint32_t>());
+    const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
+    const uint32_t blocks =
+        static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
+    hipLaunchKernelGGL(k_trace_scenes, dim3(blocks), dim3(kPersistThreads), 0, st,
+                       s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), s.off.as<uint32_t>(),
+                       s.nscenes(), d_rays, d_scene, n, d_id, d_t, reinterpret_cast<float2*>(d_uv),
+                       s.bvh.ray_counter.as<uint32_t>(), std::min(64u, std::max(1u, tune.refill)));
+    PT_HIP(hipGetLastError());
 }
 
 void trace_closest(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,

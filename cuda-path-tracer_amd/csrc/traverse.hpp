@@ -240,7 +240,10 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
 // least four per lane of the grid, else 64 so that a short queue still
 // spreads over every wave.  Idle lanes are refilled when fewer than `refill`
 // (1..64) lanes of the wave are busy.  load(i, o, d) fetches ray i;
-// store(i, hit) writes its result.
+// store(i, hit) writes its result.  load returns the ray's root internal
+// node, or kInvalid for a scene without one (fewer than two triangles); then
+// `single` names its one triangle (or kInvalid) and the ray is answered at
+// once without entering the loop.
 constexpr uint32_t kRayPool = 256;
 constexpr uint32_t kRayPoolSmall = 64;
 
@@ -283,18 +286,31 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
                 const uint32_t took = min(static_cast<uint32_t>(__popcll(idle)), avail);
                 if (take) {
                     ray = pool_next + rank;
-                    load(ray, r.o, r.d);
-                    r.dinv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-                    r.finite = __builtin_fabsf(r.dinv.x) <= kFltMax &&
-                               __builtin_fabsf(r.dinv.y) <= kFltMax &&
-                               __builtin_fabsf(r.dinv.z) <= kFltMax;
-                    r.minT = kMinTInit;
-                    r.best = kInvalid;
-                    r.bu = 0.0f;
-                    r.bv = 0.0f;
-                    r.node = 0;
-                    r.sp = 0;
-                    active = true;
+                    uint32_t single = kInvalid;
+                    const uint32_t root = load(ray, r.o, r.d, single);
+                    if (root == kInvalid) {
+                        TraceHit h{kFltMax, kInvalid, 0.0f, 0.0f};
+                        if (single != kInvalid) {
+                            float u = 0.0f, v = 0.0f;
+                            uint32_t id;
+                            if (COUNT) cnt.tris++;
+                            const float t = tri_hit(tris, single, r.o, r.d, u, v, id);
+                            if (t <= kMinTInit) h = TraceHit{t, id, u, v};
+                        }
+                        store(ray, h);
+                    } else {
+                        r.dinv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+                        r.finite = __builtin_fabsf(r.dinv.x) <= kFltMax &&
+                                   __builtin_fabsf(r.dinv.y) <= kFltMax &&
+                                   __builtin_fabsf(r.dinv.z) <= kFltMax;
+                        r.minT = kMinTInit;
+                        r.best = kInvalid;
+                        r.bu = 0.0f;
+                        r.bv = 0.0f;
+                        r.node = root;
+                        r.sp = 0;
+                        active = true;
+                    }
                 }
                 pool_next += took;
                 idle = ~__ballot(active);

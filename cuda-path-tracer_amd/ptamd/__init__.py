@@ -66,6 +66,9 @@ _SIGS = {
     "pt_set_stream": [_vp, _vp],
     "pt_synchronize": [_vp],
     "pt_upload_scene": [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp],
+    "pt_upload_scenes": [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp],
+    "pt_get_bvh_scene": [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp],
+    "pt_trace_closest_scenes_device": [_vp, _vp, _vp, _u32, _vp, _vp, _vp, _vp],
     "pt_set_settings": [_vp, _vp],
     "pt_build_bvh": [_vp, _vp],
     "pt_get_bvh": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -75,7 +78,8 @@ _SIGS = {
     "pt_trace_closest_counted": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp],
     "pt_set_primary_rays": [_vp, _vp, _vp],
     "pt_render_pass": [_vp, _i, _i, _u64],
-    "pt_render_passes": [_vp, _i, _i, _u32, _vp, _vp],
+    "pt_render_passes": [_vp, _i, _i, _u32, _vp, _vp, _vp],
+    "pt_last_frame_rays": [_vp, _vp],
     "pt_reset_frames": [_vp],
     "pt_read_accumulator": [_vp, _vp, _vp],
     "pt_write_accumulator": [_vp, _i, _i, _vp, _u32],
@@ -165,6 +169,25 @@ def batch_counts(nscenes, seed, lo=1024, hi=262144):
     return c
 
 
+def gen_batch(nscenes, seed, lo=1024, hi=262144, counts=None):
+    """Batched soups: nscenes config-0 soups (no emitter) with log-uniform sizes;
+    returns (tris, infos, offsets).  Triangle ids are global."""
+    if counts is None:
+        counts = batch_counts(nscenes, seed, lo, hi)
+    counts = np.asarray(counts, np.uint32)
+    offs = np.zeros(len(counts) + 1, np.uint32)
+    offs[1:] = np.cumsum(counts)
+    tris = np.zeros(int(offs[-1]), TRI_DTYPE)
+    infos = np.zeros(int(offs[-1]), INFO_DTYPE)
+    for s, c in enumerate(counts):
+        if c == 0:
+            continue
+        t = tris[offs[s]:offs[s + 1]]
+        i = infos[offs[s]:offs[s + 1]]
+        _check(_lib.pt_gen_soup(seed * 1000003 + s, int(c), 0, int(offs[s]), t.ctypes.data, i.ctypes.data))
+    return tris, infos, offs
+
+
 def camera_corners(eye, target, up=(0.0, 1.0, 0.0), fovy_deg=60.0, aspect=1.0, jx=0.0, jy=0.0):
     d = np.zeros(12, np.float32)
     eye, target, up = _f32(eye, 3), _f32(target, 3), _f32(up, 3)
@@ -232,6 +255,46 @@ class Context:
                                      _ptr(la), nl, _ptr(bmin_a), _ptr(bmax_a), st))
         self.n = len(tris)
 
+    def upload_scenes(self, tris, offsets, infos, boxes=None, settings=None):
+        """Batched scenes: scene s = tris[offsets[s]:offsets[s+1]]."""
+        tris = np.ascontiguousarray(tris, TRI_DTYPE)
+        infos = np.ascontiguousarray(infos, INFO_DTYPE)
+        offs = np.ascontiguousarray(offsets, np.uint32)
+        bx = None if boxes is None else np.ascontiguousarray(boxes, np.float32)
+        st = None if settings is None else ctypes.byref(settings)
+        self._c(_lib.pt_upload_scenes(self._h, _ptr(tris), _ptr(offs), len(offs) - 1, _ptr(infos),
+                                      len(infos), _ptr(bx), st))
+        self.n = int(offs[-1])
+        self.offsets = offs.copy()
+
+    def get_bvh_scene(self, scene):
+        n = int(self.offsets[scene + 1] - self.offsets[scene])
+        m = max(n - 1, 0)
+        out = dict(morton=np.zeros(n, np.uint32), perm=np.zeros(n, np.uint32),
+                   children=np.zeros(m, np.uint32), parents_leaf=np.zeros(n, np.uint32),
+                   parents_internal=np.zeros(m, np.uint32), bboxes=np.zeros((m, 6), np.float32))
+        self._c(_lib.pt_get_bvh_scene(self._h, scene, _ptr(out["morton"]), _ptr(out["perm"]),
+                                      _ptr(out["children"]), _ptr(out["parents_leaf"]),
+                                      _ptr(out["parents_internal"]), _ptr(out["bboxes"])))
+        return out
+
+    def trace_closest_scenes_device(self, d_rays, d_scene, n, d_id, d_t, d_uv=0, stream=0):
+        self._c(_lib.pt_trace_closest_scenes_device(self._h, d_rays, d_scene, n, d_id or None,
+                                                    d_t or None, d_uv or None, stream or None))
+
+    def trace_closest_scenes(self, rays, scene_ids):
+        from . import hip
+        rays = _f32(rays).reshape(-1, 6)
+        n = len(rays)
+        d_r = hip.DeviceArray.from_host(rays)
+        d_s = hip.DeviceArray.from_host(np.ascontiguousarray(scene_ids, np.uint32))
+        d_i = hip.DeviceArray(n, np.uint32)
+        d_t = hip.DeviceArray(n, np.float32)
+        d_uv = hip.DeviceArray((n, 2), np.float32)
+        self.trace_closest_scenes_device(d_r.ptr, d_s.ptr, n, d_i.ptr, d_t.ptr, d_uv.ptr)
+        self.synchronize()
+        return d_i.to_host(), d_t.to_host(), d_uv.to_host()
+
     def set_settings(self, settings):
         self._c(_lib.pt_set_settings(self._h, ctypes.byref(settings)))
 
@@ -284,11 +347,18 @@ class Context:
     def render_pass(self, w, h, seed):
         self._c(_lib.pt_render_pass(self._h, w, h, seed))
 
-    def render_passes(self, w, h, seeds, dirs=None):
-        """len(seeds) passes in one frame; dirs: (npasses, 12) corner rays or None."""
+    def render_passes(self, w, h, seeds, dirs=None, scenes=None):
+        """len(seeds) passes in one frame; dirs: (npasses, 12) corner rays or
+        None; scenes: scene of each pass (batched upload) or None."""
         seeds = np.ascontiguousarray(seeds, np.uint64)
         d = None if dirs is None else _f32(dirs, 12 * len(seeds))
-        self._c(_lib.pt_render_passes(self._h, w, h, len(seeds), _ptr(d), _ptr(seeds)))
+        sc = None if scenes is None else np.ascontiguousarray(scenes, np.uint32)
+        self._c(_lib.pt_render_passes(self._h, w, h, len(seeds), _ptr(d), _ptr(seeds), _ptr(sc)))
+
+    def last_frame_rays(self):
+        v = np.zeros(1, np.uint64)
+        self._c(_lib.pt_last_frame_rays(self._h, _ptr(v)))
+        return int(v[0])
 
     def reset_frames(self):
         self._c(_lib.pt_reset_frames(self._h))

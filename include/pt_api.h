@@ -108,6 +108,16 @@ int pt_upload_scene(pt_context* ctx, const pt_triangle* tris, uint32_t n,
                     const pt_dir_light* lights, uint32_t nlights,
                     const float* bmin, const float* bmax,
                     const pt_settings* settings);
+/* Batched scenes (no reference counterpart; the reference builds one scene
+ * per BVHBuilder::build call): scene s owns triangles
+ * [scene_offsets[s], scene_offsets[s+1]) and gets its own LBVH, identical to
+ * the one pt_upload_scene would build from those triangles alone; all scenes
+ * are built by one launch sequence (one segmented sort).  boxes: 6 floats per
+ * scene (min.xyz, max.xyz) or NULL for each scene's union of triangle boxes.
+ * Triangle ids index one shared infos array. */
+int pt_upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* scene_offsets,
+                     uint32_t nscenes, const pt_triangle_info* infos, uint32_t ninfos,
+                     const float* boxes, const pt_settings* settings);
 int pt_set_settings(pt_context* ctx, const pt_settings* settings);
 
 /* Rebuild the LBVH of the uploaded scene (BVHBuilder::build, BVHBuilder.cu:207).
@@ -125,6 +135,11 @@ int pt_build_bvh(pt_context* ctx, float* ms_out);
  * Any pointer may be NULL. */
 int pt_get_bvh(pt_context* ctx, uint32_t* morton, uint32_t* perm, uint32_t* children,
                uint32_t* parents_leaf, uint32_t* parents_internal, float* bboxes);
+/* pt_get_bvh for scene `scene` of a batch, in that scene's local indices. */
+int pt_get_bvh_scene(pt_context* ctx, uint32_t scene, uint32_t* morton, uint32_t* perm,
+                     uint32_t* children, uint32_t* parents_leaf, uint32_t* parents_internal,
+                     float* bboxes);
+/* Total triangles and the enclosing box of scene 0. */
 int pt_get_scene_info(pt_context* ctx, uint32_t* ntris, float* bmin, float* bmax);
 
 /* ---- closest hit ---------------------------------------------------------- */
@@ -136,6 +151,10 @@ int pt_trace_closest(pt_context* ctx, const float* rays, uint32_t n,
                      uint32_t* id, float* t, float* uv);
 int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n,
                             uint32_t* d_id, float* d_t, float* d_uv, void* stream);
+/* Closest hit in batched scenes: ray i is traced in scene d_scene[i]. */
+int pt_trace_closest_scenes_device(pt_context* ctx, const float* d_rays, const uint32_t* d_scene,
+                                   uint32_t n, uint32_t* d_id, float* d_t, float* d_uv,
+                                   void* stream);
 /* Measurement variant (no reference counterpart): the same traversal, also
  * counting inner-loop iterations, box tests and triangle tests into
  * counters[3] (host, uint64).  Host-synchronous. */
@@ -156,9 +175,13 @@ int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed);
  * launches one pass per frame): the accumulator ends up exactly as after
  * npasses calls of pt_render_pass with the same corner rays and seeds.
  * dirs = 12 floats per pass (NULL: the corner rays of pt_set_primary_rays for
- * every pass), seeds[npasses]; 1 <= npasses <= 64. */
+ * every pass), seeds[npasses]; 1 <= npasses <= 1024.  scenes (may be NULL =
+ * all scene 0) picks, per pass, the scene of a pt_upload_scenes batch that
+ * the pass traces; all passes add into the one accumulator. */
 int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const float* dirs,
-                     const uint64_t* seeds);
+                     const uint64_t* seeds, const uint32_t* scenes);
+/* Rays traced by the last frame (camera + bounce rays); synchronises. */
+int pt_last_frame_rays(pt_context* ctx, uint64_t* rays);
 /* CUDAPathTracer::resetFrames (cuda_path_tracer.h:28): zero the accumulator. */
 int pt_reset_frames(pt_context* ctx);
 /* Accumulator access: rgb_sum[3*w*h] FP64 sums (row 0 = bottom), spp passes. */

@@ -274,3 +274,60 @@ def test_errors_are_status_codes(ctx):
     with pytest.raises(ptamd.PtError) as e:
         ptamd.Context(99)
     assert e.value.code == ptamd.PT_E_NO_DEVICE
+
+
+def test_batched_scenes_build_and_trace(ctx):
+    """Batched build: every scene's arrays equal its own single-scene oracle
+    build (empty, 1-, 2-triangle and 4097-triangle scenes included), and traces
+    with per-ray scene ids equal per-scene oracle traces."""
+    counts = [3000, 0, 1, 2, 4097, 17, 65536, 1200, 1, 300, 70000, 5]
+    tris, infos, offs = ptamd.gen_batch(len(counts), 41, counts=counts)
+    say(f"[batched build] scenes={len(counts)} n={len(tris)}")
+    ctx.upload_scenes(tris, offs, infos)
+    rng = np.random.default_rng(3)
+    rays_all, sc_all, expect = [], [], []
+    for s in range(len(counts)):
+        t = tris[offs[s]:offs[s + 1]]
+        if len(t) == 0:
+            continue
+        bmin, bmax = ptamd.scene_bounds(t)
+        o = orc.build(t, bmin, bmax)
+        g = ctx.get_bvh_scene(s)
+        for k in ("morton", "perm", "children", "parents_leaf", "parents_internal"):
+            assert np.array_equal(g[k], o[k]), f"scene {s}: {k}"
+        assert np.array_equal(g["bboxes"].view(np.uint32), o["bboxes"].view(np.uint32)), f"scene {s}: boxes"
+        r = np.concatenate([ptamd.gen_shell_rays(3000, s, bmin, bmax, 1.3), interior_rays(3000, s, bmin, bmax)])
+        rays_all.append(r)
+        sc_all.append(np.full(len(r), s, np.uint32))
+        expect.append(orc.trace(t, o, r))
+    # rays of the empty scene and of an out-of-range scene miss
+    rays_all.append(rays_all[0][:50])
+    sc_all.append(np.concatenate([np.full(25, 1, np.uint32), np.full(25, 999, np.uint32)]))
+    expect.append((np.full(50, ptamd.INVALID, np.uint32), np.full(50, ptamd.FLT_MAX, np.float32),
+                   np.zeros((50, 2), np.float32)))
+    rays = np.concatenate(rays_all)
+    sc = np.concatenate(sc_all)
+    perm = rng.permutation(len(rays))  # interleave scenes across waves
+    ids, t, uv = ctx.trace_closest_scenes(rays[perm], sc[perm])
+    eids = np.concatenate([e[0] for e in expect])[perm]
+    et = np.concatenate([e[1] for e in expect])[perm]
+    euv = np.concatenate([e[2] for e in expect])[perm]
+    assert np.array_equal(ids, eids)
+    assert np.array_equal(t.view(np.uint32), et.view(np.uint32))
+    assert np.array_equal(uv.view(np.uint32), euv.view(np.uint32))
+    # a single-scene upload afterwards still builds and traces as before
+    check_build(ctx, *ptamd.gen_soup(5000, 2), tag="after-batch")
+
+
+def test_batched_512_scene_build_parity(ctx):
+    """Config-3 shape at reduced sizes: 512 soups, log-uniform [64, 4096]."""
+    tris, infos, offs = ptamd.gen_batch(512, 7, lo=64, hi=4096)
+    say(f"[batched build] 512 scenes n={len(tris)}")
+    ctx.upload_scenes(tris, offs, infos)
+    for s in range(0, 512, 7):
+        t = tris[offs[s]:offs[s + 1]]
+        o = orc.build(t, *ptamd.scene_bounds(t))
+        g = ctx.get_bvh_scene(s)
+        for k in ("morton", "perm", "children", "parents_leaf", "parents_internal"):
+            assert np.array_equal(g[k], o[k]), f"scene {s}: {k}"
+        assert np.array_equal(g["bboxes"].view(np.uint32), o["bboxes"].view(np.uint32)), f"scene {s}"

@@ -113,3 +113,35 @@ def test_render_invariant_under_tuning(ctx, scene):
     finally:
         ptamd.tune_set("trace.blocks", saved[0])
         ptamd.tune_set("trace.refill", saved[1])
+
+
+def test_batched_scene_frame(ctx):
+    """One frame whose passes trace different scenes of a batch (including a
+    1-triangle and an empty scene) equals the oracle pass by pass."""
+    counts = [20000, 1, 0, 5000, 65536, 2]
+    tris, infos, offs = ptamd.gen_batch(len(counts), 12, counts=counts)
+    # give every soup an emitter: the last scene's two triangles become lights
+    infos["emission"][offs[-2]:offs[-1]] = 10.0
+    st = ptamd.Settings(6, 3, 1.0, 1)
+    ctx.upload_scenes(tris, offs, infos, settings=st)
+    npass = len(counts)
+    seeds = np.arange(50, 50 + npass, dtype=np.uint64)
+    dirs = np.stack([pass_dirs(p) for p in range(npass)])
+    scenes = np.arange(npass, dtype=np.uint32)[::-1].copy()
+    say(f"[batched frame] {npass} scenes")
+    ctx.reset_frames()
+    ctx.render_passes(W, H, seeds, dirs, scenes)
+    acc, spp = ctx.read_accumulator(W, H)
+    rays = ctx.last_frame_rays()
+    assert spp == npass and rays >= W * H * npass
+    o = np.zeros((H, W, 3))
+    segs = 0
+    for p in range(npass):
+        s = int(scenes[p])
+        t = tris[offs[s]:offs[s + 1]]
+        bmin, bmax = ptamd.scene_bounds(t) if len(t) else (np.zeros(3), np.ones(3))
+        b = orc.build(t, bmin, bmax)
+        _, sg = orc.render_pass(t, b, infos, st, W, H, dirs[p], EYE, p, int(seeds[p]), o)
+        segs += sg
+    assert np.array_equal(acc, o)
+    assert rays == segs
