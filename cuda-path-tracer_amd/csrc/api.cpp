@@ -443,4 +443,60 @@ int pt_tune_get(const char* name, uint32_t* value) {
     return PT_OK;
 }
 
+int pt_cache_param_count(void) { return pt::RadianceCache::param_count(); }
+
+int pt_cache_init(pt_context* ctx, uint64_t seed, const float* bmin, const float* bmax) {
+    if (!ctx || !bmin || !bmax) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        ctx->cache.set_box(bmin, bmax);
+        ctx->cache.init(seed, ctx->stream);
+    });
+}
+
+int pt_cache_set_params(pt_context* ctx, const float* params) {
+    if (!ctx || !params) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { ctx->cache.set_params(params, ctx->stream); });
+}
+
+int pt_cache_get_params(pt_context* ctx, float* params) {
+    if (!ctx || !params) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { ctx->cache.get_params(params, ctx->stream); });
+}
+
+int pt_cache_set_adam(pt_context* ctx, float lr, float beta1, float beta2, float eps) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        ctx->cache.lr = lr;
+        ctx->cache.beta1 = beta1;
+        ctx->cache.beta2 = beta2;
+        ctx->cache.eps = eps;
+    });
+}
+
+int pt_cache_infer_device(pt_context* ctx, const float* d_raw, uint32_t n, float* d_out, void* stream) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        require(n == 0 || (d_raw && d_out), "NULL buffer");
+        ctx->cache.infer(d_raw, n, d_out, pick(ctx, stream));
+    });
+}
+
+int pt_cache_train_device(pt_context* ctx, const float* d_raw, const float* d_target, uint32_t n,
+                          float* d_loss, void* stream) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        require(n == 0 || (d_raw && d_target), "NULL buffer");
+        ctx->cache.train(d_raw, d_target, n, d_loss, nullptr, pick(ctx, stream));
+    });
+}
+
+int pt_cache_gradient_device(pt_context* ctx, const float* d_raw, const float* d_target, uint32_t n,
+                             float* d_grad, void* stream) {
+    if (!ctx || !d_grad) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        require(n > 0 && d_raw && d_target, "empty batch");
+        ctx->cache.train(d_raw, d_target, n, nullptr, d_grad, pick(ctx, stream));
+    });
+}
+
 }  // extern "C"

@@ -208,6 +208,29 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
 // Rays traced by the last frame (sum of its queue lengths); synchronises.
 uint64_t render_last_rays(Renderer& r, hipStream_t stream);
 
+// Learned radiance cache: the MLP's masters, Adam moments, fp16 fragments
+// and training scratch (csrc/mlp.hip).
+struct RadianceCache {
+    DevBuf master, m, v;   // fp32 parameters and Adam moments
+    DevBuf fwd, bwd;       // fp16 weight fragments (forward / backward)
+    DevBuf acts, deltas, part, loss;  // training scratch
+    float box_min[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};
+    float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
+    uint32_t step = 0;
+    bool ready = false;
+    static int param_count();
+    void set_box(const float* bmin, const float* bmax);
+    void init(uint64_t seed, hipStream_t st);
+    void set_params(const float* host, hipStream_t st);
+    void get_params(float* host, hipStream_t st);
+    void pack(hipStream_t st);
+    // raw: 12 floats per sample (position, normal, view direction, albedo).
+    void infer(const float* d_raw, uint32_t n, float* d_out, hipStream_t st);
+    // One Adam step on the batch (d_grad_out == NULL), or only the gradient.
+    void train(const float* d_raw, const float* d_target, uint32_t n, float* d_loss,
+               float* d_grad_out, hipStream_t st);
+};
+
 }  // namespace pt
 
 struct pt_context {
@@ -217,6 +240,7 @@ struct pt_context {
     std::string last_error;
     pt::DeviceScene scene;
     pt::Renderer renderer;
+    pt::RadianceCache cache;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pt::DevBuf io_rays, io_id, io_t, io_uv;  // staging for host-pointer traces
 };

@@ -93,6 +93,13 @@ _SIGS = {
     "pt_gen_batch_counts": [_u64, _u32, _u32, _u32, _vp],
     "pt_camera_corners": [_vp, _vp, _vp, _f, _f, _f, _f, _vp],
     "pt_pass_jitter": [_u64, _u32, _i, _i, _vp, _vp],
+    "pt_cache_init": [_vp, _u64, _vp, _vp],
+    "pt_cache_set_params": [_vp, _vp],
+    "pt_cache_get_params": [_vp, _vp],
+    "pt_cache_set_adam": [_vp, _f, _f, _f, _f],
+    "pt_cache_infer_device": [_vp, _vp, _u32, _vp, _vp],
+    "pt_cache_train_device": [_vp, _vp, _vp, _u32, _vp, _vp],
+    "pt_cache_gradient_device": [_vp, _vp, _vp, _u32, _vp, _vp],
     "pt_tune_set": [ctypes.c_char_p, _u32],
     "pt_tune_get": [ctypes.c_char_p, _vp],
 }
@@ -101,7 +108,11 @@ for _name, _args in _SIGS.items():
     _fn.argtypes = _args
     _fn.restype = _i
 
-EXPORTED = tuple(_SIGS)
+_lib.pt_cache_param_count.argtypes = []
+_lib.pt_cache_param_count.restype = _i
+
+EXPORTED = tuple(_SIGS) + ("pt_cache_param_count",)
+CACHE_PARAMS = int(_lib.pt_cache_param_count())
 
 
 def _ptr(a):
@@ -389,3 +400,65 @@ class Context:
 
     def set_stream(self, stream):
         self._c(_lib.pt_set_stream(self._h, stream or None))
+
+    # ---- learned radiance cache ------------------------------------------------
+    def cache_init(self, seed, bmin, bmax):
+        bmin, bmax = _f32(bmin, 3), _f32(bmax, 3)
+        self._c(_lib.pt_cache_init(self._h, seed, _ptr(bmin), _ptr(bmax)))
+
+    def cache_set_params(self, params):
+        params = _f32(params, CACHE_PARAMS)
+        self._c(_lib.pt_cache_set_params(self._h, _ptr(params)))
+
+    def cache_get_params(self):
+        out = np.zeros(CACHE_PARAMS, np.float32)
+        self._c(_lib.pt_cache_get_params(self._h, _ptr(out)))
+        return out
+
+    def cache_set_adam(self, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8):
+        self._c(_lib.pt_cache_set_adam(self._h, lr, beta1, beta2, eps))
+
+    def cache_infer_device(self, d_raw, n, d_out, stream=0):
+        self._c(_lib.pt_cache_infer_device(self._h, d_raw or None, n, d_out or None, stream or None))
+
+    def cache_train_device(self, d_raw, d_target, n, d_loss=0, stream=0):
+        self._c(_lib.pt_cache_train_device(self._h, d_raw or None, d_target or None, n,
+                                           d_loss or None, stream or None))
+
+    def cache_gradient_device(self, d_raw, d_target, n, d_grad, stream=0):
+        self._c(_lib.pt_cache_gradient_device(self._h, d_raw or None, d_target or None, n,
+                                              d_grad or None, stream or None))
+
+    def cache_infer(self, raw):
+        """raw: (n, 12) samples -> (n, 3) predicted radiance."""
+        from . import hip
+        raw = _f32(raw).reshape(-1, 12)
+        n = len(raw)
+        d_raw = hip.DeviceArray.from_host(raw)
+        d_out = hip.DeviceArray((n, 3), np.float32)
+        self.cache_infer_device(d_raw.ptr, n, d_out.ptr)
+        self.synchronize()
+        return d_out.to_host()
+
+    def cache_train(self, raw, target):
+        """One Adam step on the batch; returns the per-sample losses."""
+        from . import hip
+        raw = _f32(raw).reshape(-1, 12)
+        n = len(raw)
+        d_raw = hip.DeviceArray.from_host(raw)
+        d_tgt = hip.DeviceArray.from_host(_f32(target).reshape(n, 3))
+        d_loss = hip.DeviceArray(n, np.float32)
+        self.cache_train_device(d_raw.ptr, d_tgt.ptr, n, d_loss.ptr)
+        self.synchronize()
+        return d_loss.to_host()
+
+    def cache_gradient(self, raw, target):
+        from . import hip
+        raw = _f32(raw).reshape(-1, 12)
+        n = len(raw)
+        d_raw = hip.DeviceArray.from_host(raw)
+        d_tgt = hip.DeviceArray.from_host(_f32(target).reshape(n, 3))
+        d_g = hip.DeviceArray(CACHE_PARAMS, np.float32)
+        self.cache_gradient_device(d_raw.ptr, d_tgt.ptr, n, d_g.ptr)
+        self.synchronize()
+        return d_g.to_host()

@@ -193,6 +193,30 @@ int pt_read_image(pt_context* ctx, float* rgba);
 /* Wall time in ms of the last pt_render_pass (HIP events). */
 int pt_last_render_ms(pt_context* ctx, float* ms);
 
+/* ---- learned radiance cache (no reference counterpart) -----------------------
+ * MLP: 96 inputs (12-frequency sin/cos of the position mapped into the scene
+ * box [bmin, bmax] (72), normal, view direction, albedo, 15 zeros) ->
+ * 4 x (64, ReLU) -> 3 outputs with exp.  Parameters (fp32 masters, flat):
+ * W0[96][64] b0[64] W1[64][64] b1 W2 b2 W3 b3 W4[64][3] b4[3], W[in][out];
+ * pt_cache_param_count() of them.  Raw samples are 12 floats: position.xyz,
+ * normal.xyz, view.xyz, albedo.rgb.  Training minimises the relative L2 loss
+ * (1/n) sum (y - t)^2 / (y^2 + 0.01) (y not differentiated in the
+ * denominator) with Adam (default lr 1e-3, betas 0.9/0.99, eps 1e-8). */
+int pt_cache_param_count(void);
+int pt_cache_init(pt_context* ctx, uint64_t seed, const float* bmin, const float* bmax);
+int pt_cache_set_params(pt_context* ctx, const float* params);
+int pt_cache_get_params(pt_context* ctx, float* params);
+int pt_cache_set_adam(pt_context* ctx, float lr, float beta1, float beta2, float eps);
+/* d_out[3*i] = prediction for sample i. */
+int pt_cache_infer_device(pt_context* ctx, const float* d_raw, uint32_t n, float* d_out,
+                          void* stream);
+/* One Adam step on a batch; d_loss (may be NULL) receives per-sample losses. */
+int pt_cache_train_device(pt_context* ctx, const float* d_raw, const float* d_target,
+                          uint32_t n, float* d_loss, void* stream);
+/* Gradient of the batch loss (no update), d_grad[pt_cache_param_count()]. */
+int pt_cache_gradient_device(pt_context* ctx, const float* d_raw, const float* d_target,
+                             uint32_t n, float* d_grad, void* stream);
+
 /* ---- launch tuning (no reference counterpart) -------------------------------
  * Process-wide launch-shape knobs read by the kernels' host launchers; results
  * do not depend on them (tests check this).  Names:

@@ -76,6 +76,18 @@ void or_render_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_
                     const float* eye, uint32_t pass, uint64_t seed, double* accum,
                     uint64_t* seg_counts, int threads);
 
+/* Radiance-cache MLP in FP64 (mlp_oracle.cpp): parameter count, encoding of
+ * one raw sample (12 floats) into 96 inputs, forward (exp outputs), loss and
+ * gradient of the relative L2 loss, and one Adam step. */
+int or_mlp_param_count(void);
+void or_mlp_encode(const float* raw, const float* bmin, const float* bmax, double* x);
+void or_mlp_forward(const float* params, const float* raw, uint32_t n, const float* bmin,
+                    const float* bmax, double* out);
+double or_mlp_loss_grad(const float* params, const float* raw, const float* target, uint32_t n,
+                        const float* bmin, const float* bmax, double* grad);
+void or_adam_step(float* w, float* m, float* v, const double* g, uint32_t count, uint32_t t,
+                  float lr, float b1, float b2, float eps);
+
 /* Sampling helpers shared by the render, exposed for tests. */
 void or_pcg4d(const uint32_t* in, uint32_t* out);
 void or_sincos2pi(float u, float* s, float* c);

@@ -28,6 +28,11 @@ _SIGS = {
     "or_trace": ([_vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _vp, _i], None),
     "or_brute": ([_vp, _u32, _vp, _u32, _vp, _vp, _vp, _i], None),
     "or_render_pass": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp, _vp, _i], None),
+    "or_mlp_param_count": ([], _i),
+    "or_mlp_encode": ([_vp, _vp, _vp, _vp], None),
+    "or_mlp_forward": ([_vp, _vp, _u32, _vp, _vp, _vp], None),
+    "or_mlp_loss_grad": ([_vp, _vp, _vp, _u32, _vp, _vp, _vp], ctypes.c_double),
+    "or_adam_step": ([_vp, _vp, _vp, _vp, _u32, _u32, _f, _f, _f, _f], None),
     "or_pcg4d": ([_vp, _vp], None),
     "or_sincos2pi": ([_f, _vp, _vp], None),
     "or_to_hemisphere": ([_vp, _vp, _vp], None),
@@ -164,3 +169,36 @@ def to_hemisphere(v, n):
     v, n = _f32(v), _f32(n)
     _lib.or_to_hemisphere(_p(v), _p(n), _p(out))
     return out
+
+
+# ---- radiance-cache MLP (FP64) --------------------------------------------------
+def mlp_param_count():
+    return int(_lib.or_mlp_param_count())
+
+
+def mlp_encode(raw, bmin, bmax):
+    raw, bmin, bmax = _f32(raw), _f32(bmin), _f32(bmax)
+    x = np.zeros(96, np.float64)
+    _lib.or_mlp_encode(_p(raw), _p(bmin), _p(bmax), _p(x))
+    return x
+
+
+def mlp_forward(params, raw, bmin, bmax):
+    params, raw, bmin, bmax = _f32(params), _f32(raw).reshape(-1, 12), _f32(bmin), _f32(bmax)
+    out = np.zeros((len(raw), 3), np.float64)
+    _lib.or_mlp_forward(_p(params), _p(raw), len(raw), _p(bmin), _p(bmax), _p(out))
+    return out
+
+
+def mlp_loss_grad(params, raw, target, bmin, bmax):
+    params, raw = _f32(params), _f32(raw).reshape(-1, 12)
+    target, bmin, bmax = _f32(target).reshape(-1, 3), _f32(bmin), _f32(bmax)
+    g = np.zeros(mlp_param_count(), np.float64)
+    loss = _lib.or_mlp_loss_grad(_p(params), _p(raw), _p(target), len(raw), _p(bmin), _p(bmax), _p(g))
+    return float(loss), g
+
+
+def adam_step(w, m, v, g, t, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
+    """In place on float32 w, m, v."""
+    g = np.ascontiguousarray(g, np.float64)
+    _lib.or_adam_step(_p(w), _p(m), _p(v), _p(g), len(w), t, lr, b1, b2, eps)
