@@ -499,4 +499,25 @@ int pt_cache_gradient_device(pt_context* ctx, const float* d_raw, const float* d
     });
 }
 
+int pt_render_cache_frame(pt_context* ctx, int w, int h, uint64_t pass_seed,
+                          const pt_cache_settings* settings) {
+    if (!ctx || !settings) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        if (ctx->renderer.lights != 0)
+            throw pt::Error(PT_E_UNSUPPORTED, "directional lights are not supported by the renderer");
+        hipStream_t st = ctx->stream;
+        pt::render_resize(ctx->renderer, w, h, st);
+        PT_HIP(hipEventRecord(ctx->ev0, st));
+        pt::render_cache_frame(ctx->renderer, ctx->scene, ctx->cache, pass_seed, *settings, st);
+        PT_HIP(hipEventRecord(ctx->ev1, st));
+        ctx->renderer.timed = true;
+    });
+}
+
+int pt_cache_last_batch(pt_context* ctx, float* raw, float* target, uint32_t n) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { pt::render_last_batch(ctx->renderer, raw, target, n, ctx->stream); });
+}
+
 }  // extern "C"

@@ -188,6 +188,10 @@ struct Renderer {
     uint32_t lights = 0;  // directional lights in the scene (not rendered)
     bool timed = false;   // a pass has been bracketed by the context events
     uint32_t last_levels = 0;  // queue levels of the last frame
+    DevBuf cache_io;           // cache-frame queries and training batch
+    uint32_t last_train = 0;   // training paths of the last cache frame
+    void* cache_graph_exec = nullptr;  // hipGraphExec_t of the last cache frame
+    alignas(8) char cache_graph_key[320] = {};
     Renderer() = default;
     Renderer(const Renderer&) = delete;
     Renderer& operator=(const Renderer&) = delete;
@@ -208,27 +212,40 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
 // Rays traced by the last frame (sum of its queue lengths); synchronises.
 uint64_t render_last_rays(Renderer& r, hipStream_t stream);
 
+struct RadianceCache;
+// One 1-spp cache frame (pixel paths end in the cache at x2; training paths
+// extend past x2 and train the cache when PT_CACHE_TRAIN is set).
+void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& cache, uint64_t seed,
+                        const pt_cache_settings& cs, hipStream_t stream);
+// The first n samples (raw, target) of the last cache frame's training batch.
+void render_last_batch(Renderer& r, float* raw, float* target, uint32_t n, hipStream_t stream);
+
 // Learned radiance cache: the MLP's masters, Adam moments, fp16 fragments
 // and training scratch (csrc/mlp.hip).
 struct RadianceCache {
     DevBuf master, m, v;   // fp32 parameters and Adam moments
     DevBuf fwd, bwd;       // fp16 weight fragments (forward / backward)
     DevBuf acts, deltas, part, loss;  // training scratch
+    DevBuf dstep;          // uint32 Adam step count (device side)
     float box_min[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
-    uint32_t step = 0;
     bool ready = false;
     static int param_count();
     void set_box(const float* bmin, const float* bmax);
     void init(uint64_t seed, hipStream_t st);
     void set_params(const float* host, hipStream_t st);
     void get_params(float* host, hipStream_t st);
-    void pack(hipStream_t st);
+    void pack(hipStream_t st, uint32_t advance_step);
     // raw: 12 floats per sample (position, normal, view direction, albedo).
     void infer(const float* d_raw, uint32_t n, float* d_out, hipStream_t st);
     // One Adam step on the batch (d_grad_out == NULL), or only the gradient.
     void train(const float* d_raw, const float* d_target, uint32_t n, float* d_loss,
                float* d_grad_out, hipStream_t st);
+    // The same in two halves: scratch for batches up to n (allocates), then
+    // launches only (capturable into a graph).
+    void prepare_train(uint32_t n);
+    void launch_train(const float* d_raw, const float* d_target, uint32_t n, float* d_loss,
+                      float* d_grad_out, hipStream_t st);
 };
 
 }  // namespace pt

@@ -62,10 +62,13 @@ __device__ __forceinline__ floatx4 mfma(half4 a, half4 b, floatx4 c) {
 // Masters -> fp16 fragments.  Forward fragment (l, r, f), lane L, element e:
 // W_l[16f + 4(L/16) + e][16r + L%16]; backward fragment (l, r, f):
 // W_l[16r + L%16][16f + 4(L/16) + e]; zero outside the layer's shape.
+// Also advances the device-side Adam step counter by `advance` (one thread).
 __global__ __launch_bounds__(kThreads) void k_mlp_pack(const float* __restrict__ master,
                                                        _Float16* __restrict__ fwd,
-                                                       _Float16* __restrict__ bwd) {
+                                                       _Float16* __restrict__ bwd,
+                                                       uint32_t* __restrict__ step, uint32_t advance) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && advance) step[0] += advance;
     const int frag = t / 64, lane = t % 64;
     if (frag >= kFwdFrags + kBwdFrags) return;
     int l = 0, r = 0, f = 0;
@@ -265,7 +268,8 @@ __device__ __forceinline__ half4 mask4(floatx4 d, half4 h) {
 // leading dimension ld (>= n, multiple of 16): act_l = input of layer l
 // (act0: 96 rows, act1..4: 64) and delta_l = dL/dz_l scaled by n
 // (delta0..3: 64 rows, delta4: 16 rows with rows 3..15 zero).  loss[s] is the
-// sample's sum over channels.
+// sample's sum over channels.  A sample whose first target channel is NaN is
+// masked: zero loss, zero deltas (it still counts in the mean's n).
 __global__ __launch_bounds__(kThreads) void k_mlp_fwd_bwd(
     const uint2* __restrict__ fwd, const uint2* __restrict__ bwd, const float* __restrict__ master,
     const float* __restrict__ raw, const float* __restrict__ target, uint32_t n, size_t ld,
@@ -294,7 +298,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd_bwd(
             for (int r = 0; r < 4; ++r) store_fm(act[l], ld, 16 * r, s, n, H[l][r], kHid);
         // Output delta: d/dz of (e^z - t)^2 / (sg(e^z)^2 + eps), per sample.
         floatx4 d4 = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (lane < 16 && s < n) {
+        if (lane < 16 && s < n && target[3 * static_cast<size_t>(s)] == target[3 * static_cast<size_t>(s)]) {
             float lsum = 0.0f;
             float zz[3] = {z.x, z.y, z.z};
             float dd[3];
@@ -310,6 +314,8 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd_bwd(
             d4.y = dd[1];
             d4.z = dd[2];
             loss[s] = lsum;
+        } else if (lane < 16 && s < n) {
+            loss[s] = 0.0f;
         }
         half4 D[4];
         D[0] = to_half4(d4);
@@ -414,10 +420,12 @@ __global__ __launch_bounds__(kThreads) void k_mlp_wgrad(const _Float16* __restri
 
 // Adam (Kingma & Ba 2015) on the fp32 masters; the gradient is the sum of the
 // slice partials in slice order divided by n (the loss is a mean over n).
+// The bias corrections use step t = step[0] + 1, read on the device so a
+// captured frame replays with the right t (k_mlp_pack advances it).
 __global__ __launch_bounds__(kThreads) void k_mlp_adam(float* __restrict__ master, float* __restrict__ m,
                                                        float* __restrict__ v, const float* __restrict__ part,
                                                        uint32_t nslices, float inv_n, float lr, float b1,
-                                                       float b2, float eps, float c1, float c2,
+                                                       float b2, float eps, const uint32_t* __restrict__ step,
                                                        float* __restrict__ grad_out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kParams) return;
@@ -428,6 +436,9 @@ __global__ __launch_bounds__(kThreads) void k_mlp_adam(float* __restrict__ maste
         grad_out[p] = g;
         return;
     }
+    const float t = static_cast<float>(step[0] + 1u);
+    const float c1 = 1.0f - powf(b1, t);
+    const float c2 = 1.0f - powf(b2, t);
     const float mk = b1 * m[p] + (1.0f - b1) * g;
     const float vk = b2 * v[p] + (1.0f - b2) * g * g;
     m[p] = mk;
@@ -436,6 +447,8 @@ __global__ __launch_bounds__(kThreads) void k_mlp_adam(float* __restrict__ maste
 }
 
 uint32_t blocks_for(uint64_t work, uint32_t per) { return static_cast<uint32_t>((work + per - 1) / per); }
+
+constexpr uint32_t kSliceLen = 1024;  // samples per weight-gradient partial
 
 }  // namespace
 
@@ -474,13 +487,14 @@ void RadianceCache::set_params(const float* host, hipStream_t st) {
     v.reserve(kParams * sizeof(float));
     fwd.reserve(static_cast<size_t>(kFwdFrags) * kFragHalves * 2);
     bwd.reserve(static_cast<size_t>(kBwdFrags) * kFragHalves * 2);
+    dstep.reserve(sizeof(uint32_t));
     PT_HIP(hipMemcpyAsync(master.as<void>(), host, kParams * sizeof(float), hipMemcpyHostToDevice, st));
     PT_HIP(hipMemsetAsync(m.as<void>(), 0, kParams * sizeof(float), st));
     PT_HIP(hipMemsetAsync(v.as<void>(), 0, kParams * sizeof(float), st));
+    PT_HIP(hipMemsetAsync(dstep.as<void>(), 0, sizeof(uint32_t), st));
     PT_HIP(hipStreamSynchronize(st));
-    step = 0;
     ready = true;
-    pack(st);
+    pack(st, 0);
 }
 
 void RadianceCache::get_params(float* host, hipStream_t st) {
@@ -489,9 +503,10 @@ void RadianceCache::get_params(float* host, hipStream_t st) {
     PT_HIP(hipStreamSynchronize(st));
 }
 
-void RadianceCache::pack(hipStream_t st) {
+void RadianceCache::pack(hipStream_t st, uint32_t advance) {
     hipLaunchKernelGGL(k_mlp_pack, dim3(blocks_for((kFwdFrags + kBwdFrags) * 64, kThreads)), dim3(kThreads), 0,
-                       st, master.as<float>(), fwd.as<_Float16>(), bwd.as<_Float16>());
+                       st, master.as<float>(), fwd.as<_Float16>(), bwd.as<_Float16>(), dstep.as<uint32_t>(),
+                       advance);
     PT_HIP(hipGetLastError());
 }
 
@@ -509,17 +524,18 @@ void RadianceCache::infer(const float* d_raw, uint32_t n, float* d_out, hipStrea
     PT_HIP(hipGetLastError());
 }
 
-void RadianceCache::train(const float* d_raw, const float* d_target, uint32_t n, float* d_loss,
-                          float* d_grad_out, hipStream_t st) {
-    if (!ready) throw Error(PT_E_INVALID_ARG, "radiance cache not initialised");
-    if (n == 0) return;
+void RadianceCache::prepare_train(uint32_t n) {
     const size_t ld = (static_cast<size_t>(n) + 15) & ~size_t(15);
     acts.reserve(ld * (kIn + 4 * kHid) * 2);
     deltas.reserve(ld * (5 * kHid) * 2);
-    const uint32_t slice_len = 1024;
-    const uint32_t nslices = blocks_for(n, slice_len);
-    part.reserve(static_cast<size_t>(nslices) * kParams * sizeof(float));
+    part.reserve(static_cast<size_t>(blocks_for(n, kSliceLen)) * kParams * sizeof(float));
     loss.reserve(ld * sizeof(float));
+}
+
+void RadianceCache::launch_train(const float* d_raw, const float* d_target, uint32_t n, float* d_loss,
+                                 float* d_grad_out, hipStream_t st) {
+    const size_t ld = (static_cast<size_t>(n) + 15) & ~size_t(15);
+    const uint32_t nslices = blocks_for(n, kSliceLen);
     SceneBox b;
     for (int k = 0; k < 3; ++k) {
         b.mn[k] = box_min[k];
@@ -530,19 +546,21 @@ void RadianceCache::train(const float* d_raw, const float* d_target, uint32_t n,
                        acts.as<_Float16>(), deltas.as<_Float16>(), d_loss ? d_loss : loss.as<float>());
     constexpr int kTasks = fwd_base(kLayers) + 4 * 4 + 1;
     hipLaunchKernelGGL(k_mlp_wgrad, dim3(blocks_for(static_cast<uint64_t>(kTasks) * nslices, kThreads / 64)),
-                       dim3(kThreads), 0, st, acts.as<_Float16>(), deltas.as<_Float16>(), n, ld, slice_len,
+                       dim3(kThreads), 0, st, acts.as<_Float16>(), deltas.as<_Float16>(), n, ld, kSliceLen,
                        nslices, part.as<float>());
-    float c1 = 1.0f, c2 = 1.0f;
-    if (!d_grad_out) {
-        ++step;
-        c1 = 1.0f - std::pow(beta1, static_cast<float>(step));
-        c2 = 1.0f - std::pow(beta2, static_cast<float>(step));
-    }
     hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kParams, kThreads)), dim3(kThreads), 0, st, master.as<float>(),
                        m.as<float>(), v.as<float>(), part.as<float>(), nslices, 1.0f / static_cast<float>(n), lr,
-                       beta1, beta2, eps, c1, c2, d_grad_out);
-    if (!d_grad_out) pack(st);
+                       beta1, beta2, eps, dstep.as<uint32_t>(), d_grad_out);
+    if (!d_grad_out) pack(st, 1);
     PT_HIP(hipGetLastError());
+}
+
+void RadianceCache::train(const float* d_raw, const float* d_target, uint32_t n, float* d_loss,
+                          float* d_grad_out, hipStream_t st) {
+    if (!ready) throw Error(PT_E_INVALID_ARG, "radiance cache not initialised");
+    if (n == 0) return;
+    prepare_train(n);
+    launch_train(d_raw, d_target, n, d_loss, d_grad_out, st);
 }
 
 }  // namespace pt

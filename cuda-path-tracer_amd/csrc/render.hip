@@ -10,12 +10,20 @@
 //   k_accumulate   per pixel, FP64 sums += each pass's radiance, in pass order
 // Queue lengths live on the device, so the host never waits inside a frame,
 // and a frame's kernel sequence is captured once as a hipGraph and replayed.
+//
+// Cache frames (learned radiance cache, no reference counterpart): every
+// pixel's path stops at its second vertex x2 (after the first bounce) and adds
+// throughput * cache(x2); a set of training paths, started on 1/16 of the
+// pixels, records x2's features and extends past it to give the cache's
+// target there (the radiance the suffix brings back, per unit throughput).
 // Per-pass values (corner rays, eye, pass index, seed) are read from device
 // memory, so replays only need a fresh copy of that small block.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <string>
+#include <type_traits>
 
 #include "context.hpp"
 #include "sampling.hpp"
@@ -49,6 +57,23 @@ struct PathState {
     float *ar, *ag, *ab;  // throughput ("absorbed", cuda_path_tracer.cu:259)
     float *cr, *cg, *cb;  // radiance ("accumulatedColor")
 };
+
+// Cache-frame buffers: queries of the pixel paths (slot = pixel) and the
+// training batch (slot = training path).  Raw samples are 12 floats:
+// position, shading normal, incoming direction, albedo.
+struct CacheIO {
+    float* qraw;        // [npix][12]
+    float* qout;        // [npix][3] cache predictions
+    uint32_t* qvalid;   // path reached x2
+    float* traw;        // [ntrain][12]
+    float* ttgt;        // [ntrain][3] targets (NaN: masked)
+    uint32_t* tvalid;
+    uint32_t ntrain;        // training paths
+    uint32_t train_levels;  // queue levels of a training path (2 + extra bounces)
+};
+
+// Training paths draw from their own random stream.
+constexpr uint32_t kTrainSalt = 0x5bd1e995u;
 
 constexpr int kThreads = 256;
 
@@ -109,7 +134,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
     auto load = [&](uint32_t i, V3& o, V3& d, uint32_t& single) -> uint32_t {
         o = v3(q.ox[i], q.oy[i], q.oz[i]);
         d = v3(q.dx[i], q.dy[i], q.dz[i]);
-        const uint32_t sc = pp[q.path[i] / fc.npix].scene;
+        const uint32_t sc = pp[min(q.path[i] / fc.npix, fc.npasses - 1u)].scene;
         const uint32_t b = off[sc], e = off[sc + 1];
         if (e - b >= 2) return b;
         if (e - b == 1) single = b;
@@ -126,12 +151,14 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
+// CACHE: a cache frame (one pass; paths >= npix are training paths).
+template <bool CACHE>
 __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                                                     const PassParams* __restrict__ pp,
                                                     const pt_triangle_info* __restrict__ infos,
                                                     RayQueue qin, HitQueue hq, RayQueue qout,
                                                     PathState ps, uint32_t* __restrict__ counters,
-                                                    uint32_t level) {
+                                                    uint32_t level, CacheIO io) {
     const uint32_t count = counters[level];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -144,8 +171,11 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
         uint32_t path = 0;
         if (i < count && hq.id[i] != kInvalid) {
             path = qin.path[i];
-            const PassParams& P = pp[path / fc.npix];
-            const uint32_t pix = path % fc.npix;
+            const bool train = CACHE && path >= fc.npix;
+            const PassParams& P = pp[CACHE ? 0u : path / fc.npix];
+            // RNG key: pixel (training paths: their index, own stream).
+            const uint32_t pix = train ? path - fc.npix : path % fc.npix;
+            const uint32_t seedw = train ? (P.seedw ^ kTrainSalt) : P.seedw;
             const V3 o = v3(qin.ox[i], qin.oy[i], qin.oz[i]);
             const V3 d = v3(qin.dx[i], qin.dy[i], qin.dz[i]);
             const float t = hq.t[i];
@@ -182,10 +212,29 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                 absorbed = mul(absorbed, albedo);
                 acc = add(acc, scale(mul(absorbed, L), fc.intensity));
             }
+            if (CACHE && level == 1) {
+                // x2: a pixel path's cache query, or a training sample whose
+                // suffix radiance is gathered from here with throughput 1.
+                float* f = train ? io.traw + 12 * static_cast<size_t>(pix) : io.qraw + 12 * static_cast<size_t>(path);
+                f[0] = pos.x; f[1] = pos.y; f[2] = pos.z;
+                f[3] = nrm.x; f[4] = nrm.y; f[5] = nrm.z;
+                f[6] = d.x;   f[7] = d.y;   f[8] = d.z;
+                f[9] = albedo.x; f[10] = albedo.y; f[11] = albedo.z;
+                if (train) {
+                    io.tvalid[pix] = 1u;
+                    absorbed = v3(1.0f, 1.0f, 1.0f);
+                    acc = v3(0.0f, 0.0f, 0.0f);
+                } else {
+                    io.qvalid[path] = 1u;
+                }
+            }
             // Next bounce index = level (bounce loop, cuda_path_tracer.cu:267).
+            // Cache frames: pixel paths end at x2, training paths after
+            // train_levels - 1 bounces.
             const uint32_t b = level;
-            if (b < fc.bounces) {
-                const U4 rv = pcg4d(U4{pix, P.pass, b, P.seedw});
+            const uint32_t limit = CACHE ? (train ? io.train_levels - 1u : 1u) : fc.bounces;
+            if (b < limit) {
+                const U4 rv = pcg4d(U4{pix, P.pass, b, seedw});
                 float u1 = u24(rv.x), u2 = u24(rv.y);
                 cont = true;
                 if (b >= fc.rr_start) {
@@ -257,6 +306,88 @@ __global__ __launch_bounds__(kThreads) void k_accumulate(const FrameConst fc, Pa
         accum[3 * static_cast<size_t>(p) + 0] = r;
         accum[3 * static_cast<size_t>(p) + 1] = g;
         accum[3 * static_cast<size_t>(p) + 2] = b;
+    }
+}
+
+// Cache frame: pixel paths (0 .. npix) as k_generate with one pass, then
+// training paths on one cell of every 4x4 pixel block, the cell (ox, oy)
+// stepping through all 16 with the pass index; training path k takes block
+// floor(k * nblocks / ntrain), so the paths spread evenly over the image.
+__global__ __launch_bounds__(kThreads) void k_generate_cache(const FrameConst fc,
+                                                             const PassParams* __restrict__ pp,
+                                                             RayQueue q, PathState ps,
+                                                             uint32_t* __restrict__ counters,
+                                                             uint32_t* __restrict__ ray_counters,
+                                                             uint32_t nlevels, CacheIO io) {
+    const uint32_t npaths = fc.npix + io.ntrain;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid == 0) {
+        counters[0] = npaths;
+        for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
+        for (uint32_t k = 0; k <= nlevels; ++k) ray_counters[k] = 0u;
+    }
+    const PassParams& P = pp[0];
+    const uint32_t w = static_cast<uint32_t>(fc.w), h = static_cast<uint32_t>(fc.h);
+    const uint32_t ox = min(P.pass % 4u, w - 1u), oy = min((P.pass / 4u) % 4u, h - 1u);
+    const uint32_t cw = (w - ox + 3u) / 4u, ch = (h - oy + 3u) / 4u;
+    const uint64_t ncell = static_cast<uint64_t>(cw) * ch;
+    for (uint32_t p = gid; p < npaths; p += gridDim.x * blockDim.x) {
+        uint32_t x, y;
+        if (p < fc.npix) {
+            x = p % w;
+            y = p / w;
+            io.qvalid[p] = 0u;
+        } else {
+            const uint32_t k = p - fc.npix;
+            const uint32_t c = static_cast<uint32_t>((static_cast<uint64_t>(k) * ncell) / io.ntrain);
+            x = ox + 4u * (c % cw);
+            y = oy + 4u * (c / cw);
+            io.tvalid[k] = 0u;
+        }
+        const float tx = static_cast<float>(x) / static_cast<float>(fc.w);
+        const float ty = static_cast<float>(y) / static_cast<float>(fc.h);
+        const V3 d00 = v3(P.dirs[0], P.dirs[1], P.dirs[2]);
+        const V3 d10 = v3(P.dirs[3], P.dirs[4], P.dirs[5]);
+        const V3 d11 = v3(P.dirs[6], P.dirs[7], P.dirs[8]);
+        const V3 d01 = v3(P.dirs[9], P.dirs[10], P.dirs[11]);
+        const V3 d = mix(mix(d00, d10, tx), mix(d01, d11, tx), ty);
+        q.ox[p] = P.eye[0];
+        q.oy[p] = P.eye[1];
+        q.oz[p] = P.eye[2];
+        q.dx[p] = d.x;
+        q.dy[p] = d.y;
+        q.dz[p] = d.z;
+        q.path[p] = p;
+        ps.cr[p] = 0.0f;
+        ps.cg[p] = 0.0f;
+        ps.cb[p] = 0.0f;
+    }
+}
+
+// Pixel paths that reached x2: radiance += throughput * cache(x2).
+__global__ __launch_bounds__(kThreads) void k_cache_resolve(const FrameConst fc, CacheIO io, PathState ps) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < fc.npix; p += gridDim.x * blockDim.x) {
+        if (!io.qvalid[p]) continue;
+        const float* o = io.qout + 3 * static_cast<size_t>(p);
+        ps.cr[p] = ps.cr[p] + ps.ar[p] * o[0];
+        ps.cg[p] = ps.cg[p] + ps.ag[p] * o[1];
+        ps.cb[p] = ps.cb[p] + ps.ab[p] * o[2];
+    }
+}
+
+// Training targets: the suffix radiance of training paths that reached x2;
+// NaN (masked) for the others.
+__global__ __launch_bounds__(kThreads) void k_train_targets(const FrameConst fc, CacheIO io, PathState ps) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < io.ntrain; k += gridDim.x * blockDim.x) {
+        const size_t p = static_cast<size_t>(fc.npix) + k;
+        float* t = io.ttgt + 3 * static_cast<size_t>(k);
+        if (io.tvalid[k]) {
+            t[0] = ps.cr[p];
+            t[1] = ps.cg[p];
+            t[2] = ps.cb[p];
+        } else {
+            t[0] = t[1] = t[2] = __builtin_nanf("");
+        }
     }
 }
 
@@ -344,9 +475,9 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
         hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
                            scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
-        hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
-                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl);
+                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, CacheIO{});
     }
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
                        r.accum.as<double>());
@@ -356,6 +487,7 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
 
 Renderer::~Renderer() {
     if (graph_exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec));
+    if (cache_graph_exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(cache_graph_exec));
     for (auto& e : ring_ev)
         if (e) (void)hipEventDestroy(e);
 }
@@ -381,24 +513,16 @@ void render_reset(Renderer& r, hipStream_t st) {
     PT_HIP(hipGetLastError());
 }
 
-void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, const float* dirs,
-                   const uint64_t* seeds, const uint32_t* scenes, hipStream_t st) {
-    if (!scene.ready) throw Error(PT_E_NO_SCENE, "no scene uploaded");
-    if (r.settings.bounces > 60) throw Error(PT_E_INVALID_ARG, "at most 60 bounces");
-    if (npasses == 0 || npasses > kMaxPasses) throw Error(PT_E_INVALID_ARG, "1 .. 1024 passes per call");
-    if (scenes)
-        for (uint32_t k = 0; k < npasses; ++k)
-            if (scenes[k] >= scene.nscenes()) throw Error(PT_E_INVALID_ARG, "scene index out of range");
-    const size_t npix = static_cast<size_t>(r.w) * r.h;
-    if (npix * npasses >= (1ull << 31)) throw Error(PT_E_INVALID_ARG, "too many paths in one frame");
-    r.queues.reserve(slab_bytes(npix * npasses) * kQueueArrays);
+namespace {
+
+// Per-pass block: through a ring of pinned slots, so a slot is rewritten only
+// after its previous copy has left.
+void upload_pass_params(Renderer& r, uint32_t npasses, const float* dirs, const uint64_t* seeds,
+                        const uint32_t* scenes, hipStream_t st) {
     r.params.reserve(sizeof(PassParams) * kMaxPasses);
     r.params_host.reserve(sizeof(PassParams) * kMaxPasses * kParamRing);
     if (!r.ring_ev[0])
         for (auto& e : r.ring_ev) PT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-
-    // Per-pass block: through a ring of pinned slots, so a slot is rewritten
-    // only after its previous copy has left.
     const int slot = r.ring_next;
     r.ring_next = (r.ring_next + 1) % kParamRing;
     if (r.ring_used[slot]) PT_HIP(hipEventSynchronize(r.ring_ev[slot]));
@@ -413,6 +537,51 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
     PT_HIP(hipMemcpyAsync(r.params.as<void>(), hp, sizeof(PassParams) * npasses, hipMemcpyHostToDevice, st));
     PT_HIP(hipEventRecord(r.ring_ev[slot], st));
     r.ring_used[slot] = true;
+}
+
+// Capture `launch` into the graph slot (exec, key) unless the key matches
+// the last capture, then launch the graph.
+template <typename Key, typename Launch>
+void replay(void*& exec, char* key_store, size_t key_cap, const Key& key, hipStream_t st, Launch launch) {
+    static_assert(std::is_trivially_copyable<Key>::value, "graph key");
+    if (sizeof(Key) > key_cap) throw Error(PT_E_INTERNAL, "graph key storage");
+    if (!exec || std::memcmp(key_store, &key, sizeof key) != 0) {
+        if (exec) {
+            PT_HIP(hipStreamSynchronize(st));
+            PT_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(exec)));
+            exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        PT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        launch();
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (le != hipSuccess) throw Error(PT_E_HIP, std::string("frame capture: ") + hipGetErrorString(le));
+        if (ce != hipSuccess) throw Error(PT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) throw Error(PT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        exec = ex;
+        std::memcpy(key_store, &key, sizeof key);
+    }
+    PT_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(exec), st));
+}
+
+}  // namespace
+
+void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, const float* dirs,
+                   const uint64_t* seeds, const uint32_t* scenes, hipStream_t st) {
+    if (!scene.ready) throw Error(PT_E_NO_SCENE, "no scene uploaded");
+    if (r.settings.bounces > 60) throw Error(PT_E_INVALID_ARG, "at most 60 bounces");
+    if (npasses == 0 || npasses > kMaxPasses) throw Error(PT_E_INVALID_ARG, "1 .. 1024 passes per call");
+    if (scenes)
+        for (uint32_t k = 0; k < npasses; ++k)
+            if (scenes[k] >= scene.nscenes()) throw Error(PT_E_INVALID_ARG, "scene index out of range");
+    const size_t npix = static_cast<size_t>(r.w) * r.h;
+    if (npix * npasses >= (1ull << 31)) throw Error(PT_E_INVALID_ARG, "too many paths in one frame");
+    r.queues.reserve(slab_bytes(npix * npasses) * kQueueArrays);
+    upload_pass_params(r, npasses, dirs, seeds, scenes, st);
 
     FrameConst fc;
     fc.w = r.w;
@@ -442,31 +611,164 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         key.blocks = trace_tuning().blocks;
         key.refill = trace_tuning().refill;
         key.stream = st;
-        static_assert(sizeof(GraphKey) <= sizeof(r.graph_key), "graph key storage");
-        if (!r.graph_exec || std::memcmp(r.graph_key, &key, sizeof key) != 0) {
-            if (r.graph_exec) {
-                PT_HIP(hipStreamSynchronize(st));
-                PT_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(r.graph_exec)));
-                r.graph_exec = nullptr;
-            }
-            hipGraph_t g = nullptr;
-            PT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            launch_frame(r, scene, fc, st);
-            const hipError_t le = hipGetLastError();
-            const hipError_t ce = hipStreamEndCapture(st, &g);
-            if (le != hipSuccess) throw Error(PT_E_HIP, std::string("frame capture: ") + hipGetErrorString(le));
-            if (ce != hipSuccess) throw Error(PT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-            hipGraphExec_t ex = nullptr;
-            const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) throw Error(PT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
-            r.graph_exec = ex;
-            std::memcpy(r.graph_key, &key, sizeof key);
-        }
-        PT_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(r.graph_exec), st));
+        replay(r.graph_exec, r.graph_key, sizeof r.graph_key, key, st,
+               [&] { launch_frame(r, scene, fc, st); });
     }
     r.spp += npasses;
     r.last_levels = fc.bounces + 1;
+}
+
+namespace {
+
+struct CacheFrame {
+    FrameConst fc;
+    CacheIO io;
+    uint32_t steps;   // Adam steps over the training batch
+    bool train;
+};
+
+void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& cache, const CacheFrame& cf,
+                        hipStream_t st) {
+    const FrameConst& fc = cf.fc;
+    const CacheIO& io = cf.io;
+    const size_t npaths = static_cast<size_t>(fc.npix) + io.ntrain;
+    Layout L = carve(r.queues.as<char>(), npaths);
+    uint32_t* cnt = r.counters.as<uint32_t>();
+    uint32_t* rcnt = cnt + 64;
+    const PassParams* pp = r.params.as<PassParams>();
+    const TraceTuning& tune = trace_tuning();
+    const uint32_t nlev = io.ntrain ? std::max(2u, io.train_levels) : 2u;
+    const uint32_t grid = grid_for(npaths);
+    const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
+    const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
+    hipLaunchKernelGGL(k_generate_cache, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt,
+                       nlev, io);
+    for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
+        hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
+                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
+                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
+        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kThreads), 0, st, fc, pp,
+                           scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
+                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io);
+    }
+    cache.infer(io.qraw, fc.npix, io.qout, st);
+    hipLaunchKernelGGL(k_cache_resolve, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, io, L.ps);
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
+                       r.accum.as<double>());
+    if (io.ntrain) {
+        hipLaunchKernelGGL(k_train_targets, dim3(grid_for(io.ntrain)), dim3(kThreads), 0, st, fc, io, L.ps);
+        if (cf.train) {
+            const uint32_t chunk = (io.ntrain + cf.steps - 1) / cf.steps;
+            for (uint32_t b = 0; b < io.ntrain; b += chunk) {
+                const uint32_t n = std::min(chunk, io.ntrain - b);
+                cache.launch_train(io.traw + 12 * static_cast<size_t>(b), io.ttgt + 3 * static_cast<size_t>(b),
+                                   n, nullptr, nullptr, st);
+            }
+        }
+    }
+}
+
+struct CacheGraphKey {
+    FrameConst fc;
+    CacheIO io;
+    uint32_t steps, train, blocks, refill;
+    const void* ptrs[12];
+    float box[6], adam[4];
+    hipStream_t stream;
+};
+
+}  // namespace
+
+void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& cache, uint64_t seed,
+                        const pt_cache_settings& cs, hipStream_t st) {
+    if (!scene.ready) throw Error(PT_E_NO_SCENE, "no scene uploaded");
+    if (!cache.ready) throw Error(PT_E_INVALID_ARG, "radiance cache not initialised");
+    if (cs.extra_bounces > 58) throw Error(PT_E_INVALID_ARG, "at most 58 extra bounces");
+    if (cs.train_steps < 1 || cs.train_steps > 64) throw Error(PT_E_INVALID_ARG, "1 .. 64 training steps");
+    if (cs.flags & ~static_cast<uint32_t>(PT_CACHE_TRAIN)) throw Error(PT_E_INVALID_ARG, "unknown cache flag");
+    const size_t npix = static_cast<size_t>(r.w) * r.h;
+    if (cs.train_samples > npix) throw Error(PT_E_INVALID_ARG, "more training paths than pixels");
+    if (npix + cs.train_samples >= (1ull << 31)) throw Error(PT_E_INVALID_ARG, "too many paths in one frame");
+    const uint32_t ntrain = cs.train_samples;
+
+    CacheFrame cf;
+    cf.fc.w = r.w;
+    cf.fc.h = r.h;
+    cf.fc.npix = static_cast<uint32_t>(npix);
+    cf.fc.npasses = 1;
+    cf.fc.bounces = 1;
+    cf.fc.rr_start = r.settings.rr_start;
+    cf.fc.intensity = r.settings.indirect_intensity;
+    cf.fc.stratified = r.settings.stratified;
+    cf.steps = cs.train_steps;
+    cf.train = (cs.flags & PT_CACHE_TRAIN) != 0;
+
+    // Allocation first: nothing below may allocate inside a capture.
+    r.queues.reserve(slab_bytes(npix + ntrain) * kQueueArrays);
+    auto bytes = [](size_t n) { return (n * 4 + 255) & ~size_t(255); };
+    r.cache_io.reserve(bytes(npix * 12) + bytes(npix * 3) + bytes(npix) + bytes(size_t(ntrain) * 12) +
+                       bytes(size_t(ntrain) * 3) + bytes(ntrain));
+    char* base = r.cache_io.as<char>();
+    auto take = [&](size_t n) {
+        char* p = base;
+        base += bytes(n);
+        return p;
+    };
+    cf.io.qraw = reinterpret_cast<float*>(take(npix * 12));
+    cf.io.qout = reinterpret_cast<float*>(take(npix * 3));
+    cf.io.qvalid = reinterpret_cast<uint32_t*>(take(npix));
+    cf.io.traw = reinterpret_cast<float*>(take(size_t(ntrain) * 12));
+    cf.io.ttgt = reinterpret_cast<float*>(take(size_t(ntrain) * 3));
+    cf.io.tvalid = reinterpret_cast<uint32_t*>(take(ntrain));
+    cf.io.ntrain = ntrain;
+    cf.io.train_levels = 2u + cs.extra_bounces;
+    if (ntrain && cf.train) cache.prepare_train((ntrain + cf.steps - 1) / cf.steps);
+    upload_pass_params(r, 1, nullptr, &seed, nullptr, st);
+
+    if (!render_tuning().graphs) {
+        launch_cache_frame(r, scene, cache, cf, st);
+        PT_HIP(hipGetLastError());
+    } else {
+        CacheGraphKey key;
+        std::memset(&key, 0, sizeof key);
+        key.fc = cf.fc;
+        key.io = cf.io;
+        key.steps = cf.steps;
+        key.train = cf.train;
+        key.blocks = trace_tuning().blocks;
+        key.refill = trace_tuning().refill;
+        const void* ptrs[12] = {scene.bvh.nodes.as<void>(), scene.bvh.tris.as<void>(), scene.infos.as<void>(),
+                                scene.off.as<void>(),       r.queues.as<void>(),     r.params.as<void>(),
+                                r.accum.as<void>(),         r.counters.as<void>(),   cache.master.as<void>(),
+                                cache.acts.as<void>(),      cache.part.as<void>(),   cache.loss.as<void>()};
+        std::memcpy(key.ptrs, ptrs, sizeof ptrs);
+        for (int k = 0; k < 3; ++k) {
+            key.box[k] = cache.box_min[k];
+            key.box[3 + k] = cache.box_inv[k];
+        }
+        key.adam[0] = cache.lr;
+        key.adam[1] = cache.beta1;
+        key.adam[2] = cache.beta2;
+        key.adam[3] = cache.eps;
+        key.stream = st;
+        replay(r.cache_graph_exec, r.cache_graph_key, sizeof r.cache_graph_key, key, st,
+               [&] { launch_cache_frame(r, scene, cache, cf, st); });
+    }
+    r.spp += 1;
+    r.last_levels = ntrain ? std::max(2u, cf.io.train_levels) : 2u;
+    r.last_train = ntrain;
+}
+
+void render_last_batch(Renderer& r, float* raw, float* target, uint32_t n, hipStream_t st) {
+    if (n > r.last_train) throw Error(PT_E_INVALID_ARG, "more samples than the last cache frame's batch");
+    if (n == 0) return;
+    const size_t npix = static_cast<size_t>(r.w) * r.h;
+    auto bytes = [](size_t k) { return (k * 4 + 255) & ~size_t(255); };
+    const char* base = r.cache_io.as<char>() + bytes(npix * 12) + bytes(npix * 3) + bytes(npix);
+    if (raw) PT_HIP(hipMemcpyAsync(raw, base, size_t(n) * 48, hipMemcpyDeviceToHost, st));
+    base += bytes(size_t(r.last_train) * 12);
+    if (target) PT_HIP(hipMemcpyAsync(target, base, size_t(n) * 12, hipMemcpyDeviceToHost, st));
+    PT_HIP(hipStreamSynchronize(st));
 }
 
 uint64_t render_last_rays(Renderer& r, hipStream_t st) {

@@ -47,6 +47,19 @@ class Settings(ctypes.Structure):
                 ("indirect_intensity", ctypes.c_float), ("stratified", ctypes.c_uint32)]
 
 
+PT_CACHE_TRAIN = 1
+
+
+class CacheSettings(ctypes.Structure):
+    """pt_cache_settings: training paths per frame, Adam steps per frame,
+    bounces of a training path past its cache vertex, flags."""
+    _fields_ = [("train_samples", ctypes.c_uint32), ("train_steps", ctypes.c_uint32),
+                ("extra_bounces", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+    def __init__(self, train_samples=65536, train_steps=4, extra_bounces=8, train=True):
+        super().__init__(train_samples, train_steps, extra_bounces, PT_CACHE_TRAIN if train else 0)
+
+
 class PtError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"pt error {code}: {msg}")
@@ -100,6 +113,8 @@ _SIGS = {
     "pt_cache_infer_device": [_vp, _vp, _u32, _vp, _vp],
     "pt_cache_train_device": [_vp, _vp, _vp, _u32, _vp, _vp],
     "pt_cache_gradient_device": [_vp, _vp, _vp, _u32, _vp, _vp],
+    "pt_render_cache_frame": [_vp, _i, _i, _u64, _vp],
+    "pt_cache_last_batch": [_vp, _vp, _vp, _u32],
     "pt_tune_set": [ctypes.c_char_p, _u32],
     "pt_tune_get": [ctypes.c_char_p, _vp],
 }
@@ -462,3 +477,13 @@ class Context:
         self.cache_gradient_device(d_raw.ptr, d_tgt.ptr, n, d_g.ptr)
         self.synchronize()
         return d_g.to_host()
+
+    def render_cache_frame(self, w, h, seed, settings=None):
+        settings = settings or CacheSettings()
+        self._c(_lib.pt_render_cache_frame(self._h, w, h, seed, ctypes.byref(settings)))
+
+    def cache_last_batch(self, n):
+        raw = np.zeros((n, 12), np.float32)
+        tgt = np.zeros((n, 3), np.float32)
+        self._c(_lib.pt_cache_last_batch(self._h, _ptr(raw), _ptr(tgt), n))
+        return raw, tgt

@@ -217,6 +217,28 @@ int pt_cache_train_device(pt_context* ctx, const float* d_raw, const float* d_ta
 int pt_cache_gradient_device(pt_context* ctx, const float* d_raw, const float* d_target,
                              uint32_t n, float* d_grad, void* stream);
 
+/* Cache frames: a 1-spp pass over w x h pixels (corner rays, eye and RR
+ * settings as for pt_render_pass) whose paths end at their second vertex x2
+ * (after the first bounce) and add throughput * cache(x2).  train_samples
+ * training paths start on one pixel of every 4x4 block (the cell steps
+ * through all 16 with the pass index), record x2 and extend extra_bounces
+ * bounces past it; their suffix radiance is the cache's target at x2.  With
+ * PT_CACHE_TRAIN the cache then takes train_steps Adam steps over the batch
+ * (split evenly), so the next frame queries the updated network.  The whole
+ * frame replays as one captured graph. */
+#define PT_CACHE_TRAIN 1u
+typedef struct pt_cache_settings {
+    uint32_t train_samples; /* training paths per frame, <= w*h (65536) */
+    uint32_t train_steps;   /* Adam steps per frame, 1..64 (4) */
+    uint32_t extra_bounces; /* bounces of a training path past x2, <= 58 (8) */
+    uint32_t flags;         /* PT_CACHE_TRAIN */
+} pt_cache_settings;
+int pt_render_cache_frame(pt_context* ctx, int w, int h, uint64_t pass_seed,
+                          const pt_cache_settings* settings);
+/* The first n (raw, target) samples of the last cache frame's training batch:
+ * raw 12 floats, target 3 floats (NaN: the path never reached x2). */
+int pt_cache_last_batch(pt_context* ctx, float* raw, float* target, uint32_t n);
+
 /* ---- launch tuning (no reference counterpart) -------------------------------
  * Process-wide launch-shape knobs read by the kernels' host launchers; results
  * do not depend on them (tests check this).  Names:

@@ -8,6 +8,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -633,6 +634,154 @@ void or_render_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_
         a[2] += static_cast<double>(accumc.z);
     }
     if (seg_counts) *seg_counts = segs;
+}
+
+// Learned-cache frame (the build's own addition; no reference counterpart).
+// Pixel paths follow or_render_pass to their second vertex x2 and add
+// throughput * cache(x2), the cache evaluated in FP64 by or_mlp_forward with
+// the given parameters.  Training path k starts on pixel
+// (ox + 4 (c % cw), oy + 4 (c / cw)), c = floor(k * cw * ch / ntrain),
+// (ox, oy) = (pass % 4, (pass / 4) % 4) clamped to the image; it draws from
+// the stream seedw ^ 0x5bd1e995 keyed by k, records x2 (position, normal,
+// incoming direction, albedo) in train_raw[12k..] and gathers the suffix
+// radiance past x2 (throughput reset to 1 there) for extra_bounces bounces
+// into train_target[3k..]; NaN when it never reaches x2.
+void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                          const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                          const pt_settings* st, int w, int h, const float* dirs, const float* eye,
+                          uint32_t pass, uint64_t seed, const float* params, const float* cbmin,
+                          const float* cbmax, uint32_t ntrain, uint32_t extra_bounces, double* accum,
+                          float* train_raw, float* train_target, int threads) {
+    Bvh bvh{tris, perm, children, boxes, n};
+    const V3 d00 = vld(dirs), d10 = vld(dirs + 3), d11 = vld(dirs + 6), d01 = vld(dirs + 9);
+    const V3 e = vld(eye);
+    const uint32_t seedw = static_cast<uint32_t>(seed) ^ static_cast<uint32_t>(seed >> 32);
+    const uint32_t rr_start = st->rr_start;
+    const float intensity = st->indirect_intensity;
+    const bool strat = st->stratified != 0;
+    if (threads <= 0) threads = 1;
+    const int64_t npix = static_cast<int64_t>(w) * h;
+    const uint32_t ox = std::min(pass % 4u, static_cast<uint32_t>(w) - 1u);
+    const uint32_t oy = std::min((pass / 4u) % 4u, static_cast<uint32_t>(h) - 1u);
+    const uint32_t cw = (static_cast<uint32_t>(w) - ox + 3u) / 4u;
+    const uint32_t ch = (static_cast<uint32_t>(h) - oy + 3u) / 4u;
+
+    // One path of the cache frame; returns whether it reached x2.  levels:
+    // hits to take (2 for a pixel path).  train: reset at x2 and continue.
+    auto walk = [&](int x, int y, uint32_t key, uint32_t sw, uint32_t levels, bool train, float* raw,
+                    V3& absorbed_out, V3& acc_out) -> bool {
+        const float tx = static_cast<float>(x) / static_cast<float>(w);
+        const float ty = static_cast<float>(y) / static_cast<float>(h);
+        V3 d = mix(mix(d00, d10, tx), mix(d01, d11, tx), ty);
+        V3 o = e;
+        V3 acc{0.0f, 0.0f, 0.0f};
+        bool reached = false;
+        Hit hit = trace_one(bvh, o, d, nullptr);
+        if (hit.id == kInvalid) return false;
+        V3 pos = add(scale(d, hit.t), o);
+        ShadeInfo si = extract(infos[hit.id], hit.u, hit.v);
+        V3 albedo = si.albedo;
+        if (emissive(si.emission)) {
+            acc = si.emission;
+            albedo = V3{1.0f, 1.0f, 1.0f};
+        }
+        V3 absorbed = albedo;
+        V3 nrm = si.normal;
+        for (uint32_t i = 0; i + 1 < levels; ++i) {
+            uint32_t rv[4] = {key, pass, i, sw};
+            pcg4d(rv);
+            float u1 = u24(rv[0]), u2 = u24(rv[1]);
+            if (i >= rr_start) {
+                float p = gmax(gmax(albedo.x, albedo.y), albedo.z);
+                p = gmin(p, 1.0f);
+                if (!(u24(rv[2]) < p)) break;
+                absorbed = V3{absorbed.x / p, absorbed.y / p, absorbed.z / p};
+            }
+            o = add(pos, scale(nrm, 1e-5f));
+            if (strat && i == 0) {
+                uint32_t p1 = pass % 64u;
+                u1 = (static_cast<float>(p1 % 8u) + u1) / 8.0f;
+                u2 = (static_cast<float>(p1 / 8u) + u2) / 8.0f;
+            }
+            float r = std::sqrt(u1);
+            float sn, cs;
+            sincos2pi(u2, sn, cs);
+            V3 local{r * cs, r * sn, std::sqrt(1.0f - u1)};
+            d = to_hemisphere(local, nrm);
+            hit = trace_one(bvh, o, d, nullptr);
+            if (hit.id == kInvalid) break;
+            pos = add(scale(d, hit.t), o);
+            si = extract(infos[hit.id], hit.u, hit.v);
+            nrm = si.normal;
+            albedo = si.albedo;
+            V3 L{0.0f, 0.0f, 0.0f};
+            if (emissive(si.emission)) {
+                float c = dot(neg(d), nrm);
+                c = gmax(c, 0.0f);
+                L = scale(si.emission, c);
+                albedo = V3{1.0f, 1.0f, 1.0f};
+            }
+            absorbed = mul(absorbed, albedo);
+            acc = add(acc, scale(mul(absorbed, L), intensity));
+            if (i == 0) {
+                const float f[12] = {pos.x, pos.y, pos.z, nrm.x, nrm.y, nrm.z,
+                                     d.x,   d.y,   d.z,   albedo.x, albedo.y, albedo.z};
+                std::memcpy(raw, f, sizeof f);
+                reached = true;
+                if (train) {
+                    absorbed = V3{1.0f, 1.0f, 1.0f};
+                    acc = V3{0.0f, 0.0f, 0.0f};
+                }
+            }
+        }
+        absorbed_out = absorbed;
+        acc_out = acc;
+        return reached;
+    };
+
+    std::vector<float> qraw(static_cast<size_t>(npix) * 12, 0.0f);
+    std::vector<float> qabs(static_cast<size_t>(npix) * 3), qacc(static_cast<size_t>(npix) * 3);
+    std::vector<uint8_t> qvalid(static_cast<size_t>(npix));
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int64_t pix = 0; pix < npix; ++pix) {
+        V3 a{0, 0, 0}, c{0, 0, 0};
+        qvalid[pix] = walk(static_cast<int>(pix % w), static_cast<int>(pix / w), static_cast<uint32_t>(pix), seedw,
+                           2u, false, &qraw[12 * pix], a, c);
+        qabs[3 * pix] = a.x;
+        qabs[3 * pix + 1] = a.y;
+        qabs[3 * pix + 2] = a.z;
+        qacc[3 * pix] = c.x;
+        qacc[3 * pix + 1] = c.y;
+        qacc[3 * pix + 2] = c.z;
+    }
+    std::vector<double> pred(static_cast<size_t>(npix) * 3);
+    or_mlp_forward(params, qraw.data(), static_cast<uint32_t>(npix), cbmin, cbmax, pred.data());
+    for (int64_t pix = 0; pix < npix; ++pix) {
+        for (int k = 0; k < 3; ++k) {
+            float v = qacc[3 * pix + k];
+            if (qvalid[pix]) v = v + qabs[3 * pix + k] * static_cast<float>(pred[3 * pix + k]);
+            accum[3 * pix + k] += static_cast<double>(v);
+        }
+    }
+    const uint32_t tsw = seedw ^ 0x5bd1e995u;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int64_t k = 0; k < static_cast<int64_t>(ntrain); ++k) {
+        const uint64_t c = (static_cast<uint64_t>(k) * (static_cast<uint64_t>(cw) * ch)) / ntrain;
+        const int x = static_cast<int>(ox + 4u * static_cast<uint32_t>(c % cw));
+        const int y = static_cast<int>(oy + 4u * static_cast<uint32_t>(c / cw));
+        V3 a{0, 0, 0}, acc{0, 0, 0};
+        float* raw = train_raw + 12 * k;
+        for (int j = 0; j < 12; ++j) raw[j] = 0.0f;
+        const bool ok = walk(x, y, static_cast<uint32_t>(k), tsw, 2u + extra_bounces, true, raw, a, acc);
+        float* t = train_target + 3 * k;
+        if (ok) {
+            t[0] = acc.x;
+            t[1] = acc.y;
+            t[2] = acc.z;
+        } else {
+            t[0] = t[1] = t[2] = std::numeric_limits<float>::quiet_NaN();
+        }
+    }
 }
 
 }  // extern "C"

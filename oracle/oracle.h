@@ -88,6 +88,19 @@ double or_mlp_loss_grad(const float* params, const float* raw, const float* targ
 void or_adam_step(float* w, float* m, float* v, const double* g, uint32_t count, uint32_t t,
                   float lr, float b1, float b2, float eps);
 
+/* Learned-cache frame (build addition, no reference counterpart): pixel
+ * paths end at their second vertex x2 and add throughput * cache(x2) (FP64
+ * MLP with `params`, box [cbmin, cbmax]) into accum; ntrain training paths
+ * give train_raw[12 * ntrain] and train_target[3 * ntrain] (NaN: never
+ * reached x2).  Follows or_render_pass for the shared path steps. */
+void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                          const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                          const pt_settings* settings, int w, int h, const float* dirs,
+                          const float* eye, uint32_t pass, uint64_t seed, const float* params,
+                          const float* cbmin, const float* cbmax, uint32_t ntrain,
+                          uint32_t extra_bounces, double* accum, float* train_raw,
+                          float* train_target, int threads);
+
 /* Sampling helpers shared by the render, exposed for tests. */
 void or_pcg4d(const uint32_t* in, uint32_t* out);
 void or_sincos2pi(float u, float* s, float* c);

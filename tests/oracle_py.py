@@ -33,6 +33,8 @@ _SIGS = {
     "or_mlp_forward": ([_vp, _vp, _u32, _vp, _vp, _vp], None),
     "or_mlp_loss_grad": ([_vp, _vp, _vp, _u32, _vp, _vp, _vp], ctypes.c_double),
     "or_adam_step": ([_vp, _vp, _vp, _vp, _u32, _u32, _f, _f, _f, _f], None),
+    "or_render_cache_pass": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp, _vp, _vp,
+                              _u32, _u32, _vp, _vp, _vp, _i], None),
     "or_pcg4d": ([_vp, _vp], None),
     "or_sincos2pi": ([_f, _vp, _vp], None),
     "or_to_hemisphere": ([_vp, _vp, _vp], None),
@@ -202,3 +204,19 @@ def adam_step(w, m, v, g, t, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
     """In place on float32 w, m, v."""
     g = np.ascontiguousarray(g, np.float64)
     _lib.or_adam_step(_p(w), _p(m), _p(v), _p(g), len(w), t, lr, b1, b2, eps)
+
+
+def render_cache_pass(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, seed, params, cbmin, cbmax,
+                      ntrain, extra_bounces, accum=None, threads=THREADS):
+    """One cache frame: returns (accum, train_raw (ntrain, 12), train_target (ntrain, 3))."""
+    if accum is None:
+        accum = np.zeros((h, w, 3), np.float64)
+    raw = np.zeros((ntrain, 12), np.float32)
+    tgt = np.zeros((ntrain, 3), np.float32)
+    dirs, eye, params = _f32(dirs), _f32(eye), _f32(params)
+    cbmin, cbmax = _f32(cbmin), _f32(cbmax)
+    _lib.or_render_cache_pass(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
+                              _p(infos), ctypes.byref(settings), w, h, _p(dirs), _p(eye), pass_index, seed,
+                              _p(params), _p(cbmin), _p(cbmax), ntrain, extra_bounces, _p(accum), _p(raw),
+                              _p(tgt), threads)
+    return accum, raw, tgt

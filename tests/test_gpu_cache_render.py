@@ -1,0 +1,161 @@
+"""Cache frames on the GPU against the oracle's cache frame.
+
+  training batch  bit-identical to the oracle (positions, normals, incoming
+                  directions, albedos and suffix-radiance targets; NaN where
+                  the path never reached x2): the path arithmetic is the plain
+                  render's, which is bit-exact
+  image           mean relative per-channel error <= 1e-3 against the oracle
+                  with the same fp16-rounded weights (the cache's fp16
+                  inference is the only difference)
+  training        the in-frame Adam steps equal cache_train over the same
+                  batch chunks, bit for bit
+  graphs          replayed cache frames (accumulator and weights) equal eager
+                  frames bit for bit
+  convergence     on a closed room, cache frames reach a lower RMSE against a
+                  high-spp plain render than plain 1-spp frames at equal frame
+                  counts
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+import ptamd
+from test_gpu_cache import fp16_weights
+from test_oracle_mlp import random_params
+
+pytestmark = pytest.mark.gpu
+
+W, H = 64, 48
+EYE, TGT = (0.0, 0.0, -1.9), (0.0, 0.0, 0.0)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dirs_for(w, h):
+    return ptamd.camera_corners(EYE, TGT, (0, 1, 0), 60.0, w / h)
+
+
+@pytest.fixture(scope="module")
+def room():
+    tris, infos = ptamd.gen_sphere(48, 3, room=True)
+    return tris, infos
+
+
+def setup(ctx, room, st, w=W, h=H):
+    tris, infos = room
+    ctx.upload_scene(tris, infos, settings=st)
+    ctx.set_primary_rays(dirs_for(w, h), EYE)
+    ctx.reset_frames()
+    bmin, bmax = ptamd.scene_bounds(tris)
+    return bmin, bmax
+
+
+def rel_err(g, o):
+    den = np.maximum(np.abs(o).mean(axis=(0, 1)), 1e-12)
+    return np.abs(g - o).mean(axis=(0, 1)) / den
+
+
+@pytest.mark.parametrize("rr_start,strat", [(3, 1), (0, 0)])
+def test_cache_frame_matches_oracle(ctx, room, rr_start, strat):
+    st = ptamd.Settings(8, rr_start, 1.0, strat)
+    bmin, bmax = setup(ctx, room, st)
+    ctx.cache_init(5, bmin, bmax)
+    p = random_params(21, bias=0.2)
+    ctx.cache_set_params(p)
+    ntrain, extra = 700, 6
+    cs = ptamd.CacheSettings(ntrain, 1, extra, train=False)
+    tris, infos = room
+    bvh = orc.build(tris, bmin, bmax)
+    acc_o = np.zeros((H, W, 3))
+    for f in range(3):
+        seed = 1000 + f
+        say(f"[cache frame] rr_start={rr_start} frame {f}")
+        ctx.render_cache_frame(W, H, seed, cs)
+        raw, tgt = ctx.cache_last_batch(ntrain)
+        acc_o, oraw, otgt = orc.render_cache_pass(tris, bvh, infos, st, W, H, dirs_for(W, H), EYE, f, seed,
+                                                  fp16_weights(p), bmin, bmax, ntrain, extra, acc_o)
+        valid = ~np.isnan(otgt[:, 0])
+        assert valid.sum() > ntrain // 2
+        np.testing.assert_array_equal(np.isnan(tgt[:, 0]), ~valid)
+        np.testing.assert_array_equal(raw[valid], oraw[valid])
+        np.testing.assert_array_equal(tgt[valid], otgt[valid])
+    acc_g, spp = ctx.read_accumulator(W, H)
+    assert spp == 3
+    err = rel_err(acc_g, acc_o)
+    say(f"  image mean relative per-channel error {err}")
+    assert acc_o.sum() > 0 and max(err) <= 1e-3
+
+
+def test_in_frame_training_equals_cache_train(ctx, room):
+    st = ptamd.Settings(8, 3, 1.0, 1)
+    bmin, bmax = setup(ctx, room, st)
+    ctx.cache_init(9, bmin, bmax)
+    p0 = ctx.cache_get_params()
+    ntrain, steps = 1024, 4
+    ctx.render_cache_frame(W, H, 77, ptamd.CacheSettings(ntrain, steps, 8, train=True))
+    p1 = ctx.cache_get_params()
+    raw, tgt = ctx.cache_last_batch(ntrain)
+    assert not np.array_equal(p0, p1)
+    ctx.cache_set_params(p0)
+    for k in range(steps):
+        sl = slice(k * ntrain // steps, (k + 1) * ntrain // steps)
+        ctx.cache_train(raw[sl], tgt[sl])
+    np.testing.assert_array_equal(ctx.cache_get_params(), p1)
+
+
+def test_cache_frames_graph_equals_eager(ctx, room):
+    st = ptamd.Settings(8, 3, 1.0, 1)
+    out = {}
+    for graphs in (1, 0):
+        ptamd.tune_set("render.graphs", graphs)
+        try:
+            bmin, bmax = setup(ctx, room, st)
+            ctx.cache_init(13, bmin, bmax)
+            for f in range(5):
+                ctx.render_cache_frame(W, H, 500 + f, ptamd.CacheSettings(512, 2, 8, train=True))
+            out[graphs] = (ctx.read_accumulator(W, H)[0], ctx.cache_get_params())
+        finally:
+            ptamd.tune_set("render.graphs", 1)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+def test_cache_frame_errors(ctx, room):
+    st = ptamd.Settings(8, 3, 1.0, 1)
+    setup(ctx, room, st)
+    with pytest.raises(ptamd.PtError):
+        ctx.render_cache_frame(W, H, 1, ptamd.CacheSettings(W * H + 1, 1, 8))
+    with pytest.raises(ptamd.PtError):
+        ctx.render_cache_frame(W, H, 1, ptamd.CacheSettings(64, 0, 8))
+    with pytest.raises(ptamd.PtError):
+        ctx.render_cache_frame(W, H, 1, ptamd.CacheSettings(64, 1, 59))
+
+
+def test_cache_lowers_rmse_at_equal_frames(ctx, room):
+    w, h = 96, 72
+    extra = 7
+    st = ptamd.Settings(1 + extra, 3, 1.0, 1)
+    bmin, bmax = setup(ctx, room, st, w, h)
+    say("[cache rmse] plain reference 2048 spp")
+    seeds = np.arange(1, 1 + 512, dtype=np.uint64)
+    for k in range(4):
+        ctx.render_passes(w, h, seeds + 512 * k)
+    ref = ctx.read_accumulator(w, h)[0] / (4 * 512)
+    frames = 64
+    ctx.reset_frames()
+    ctx.render_passes(w, h, np.arange(10 ** 6, 10 ** 6 + frames, dtype=np.uint64))
+    plain = ctx.read_accumulator(w, h)[0] / frames
+    ctx.reset_frames()
+    ctx.cache_init(3, bmin, bmax)
+    for f in range(frames):
+        ctx.render_cache_frame(w, h, 2 * 10 ** 6 + f, ptamd.CacheSettings(2048, 4, extra, train=True))
+    cached = ctx.read_accumulator(w, h)[0] / frames
+    r_plain = float(np.sqrt(np.mean((plain - ref) ** 2)))
+    r_cache = float(np.sqrt(np.mean((cached - ref) ** 2)))
+    say(f"  RMSE after {frames} frames: plain {r_plain:.5f}, cache {r_cache:.5f}")
+    assert np.isfinite(r_cache) and r_cache < r_plain
