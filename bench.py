@@ -7,9 +7,13 @@ fBm (871,200 fp32 triangles).  Rays, BVH and outputs are resident in HBM before
 the timed region.  Beside it: LBVH build ms (2^20-triangle soup and the mesh)
 and the CPU oracle on a bounded sample of the same rays.
 
+Further legs: progressive render spp/s, the 512-scene batched build and
+frame, and the radiance cache's time-to-RMSE against the plain tracer.
+
 Prints one JSON line.  Options: --gpus N (only 1 is supported), --steps K,
---warmup W, --rays R, --no-cpu (skip the CPU baseline), --profile (short run
-for rocprofv3: 2 warmup + 3 steps, no CPU baseline).
+--warmup W, --rays R, --no-cpu (skip the CPU baseline), --no-batch,
+--no-cache, --cache-full, --profile (short run for rocprofv3: 2 warmup + 3
+steps, no CPU baseline, no batched or cache legs).
 """
 import argparse
 import json
@@ -27,6 +31,176 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cache_leg(ctx, stream, args, seed_mesh):
+    """Time-to-RMSE of cache frames against the plain tracer on the room
+    scene (871,200-triangle mesh + room + ceiling emitter), 1 spp per frame,
+    both against a plain FP64 reference of ref_spp passes.  The plain paths
+    have 1 + 8 bounces, the length of a cache training path, so both converge
+    to the same image.  RMSE is over all pixels and channels; the reference's
+    own variance (from its two halves) is subtracted from every MSE."""
+    import numpy as np
+
+    import ptamd
+    from ptamd import hip
+
+    w, h = (1920, 1080) if args.cache_full else (960, 540)
+    ntrain = 65536 if args.cache_full else 16384
+    ref_spp = args.cache_ref_spp or (4096 if args.cache_full else 8192)
+    extra, steps, frames, plain_max = 8, 4, 300, 1024
+    rt, ri = ptamd.gen_sphere(660, seed_mesh, room=True)
+    st = ptamd.Settings(1 + extra, 3, 1.0, 1)
+    ctx.upload_scene(rt, ri, settings=st)
+    bmin, bmax = ptamd.scene_bounds(rt)
+    eye, tgt = (0.0, 0.0, -1.9), (0.0, 0.0, 0.0)
+
+    def dirs(p, seed):
+        return ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, *ptamd.pass_jitter(seed, p, w, h))
+
+    # Reference: two halves of ref_spp / 2 passes each (16 passes per frame).
+    halves = []
+    for half in range(2):
+        ctx.reset_frames()
+        for b in range(0, ref_spp // 2, 16):
+            ps = range(b, b + 16)
+            ctx.render_passes(w, h, np.array([10 ** 8 * (half + 1) + p for p in ps], np.uint64),
+                              np.stack([dirs(p, 31 + half) for p in ps]))
+            if b % 1024 == 0:
+                log(f"[bench] cache leg: reference half {half}, {b} / {ref_spp // 2} passes")
+        halves.append(ctx.read_accumulator(w, h)[0] / (ref_spp // 2))
+    ref = 0.5 * (halves[0] + halves[1])
+    ref_var = float(np.mean((halves[0] - halves[1]) ** 2)) / 4.0
+    del halves
+
+    def mse(acc, spp):
+        return float(np.mean((acc / spp - ref) ** 2)) - ref_var
+
+    checkpoints = [1 << k for k in range(11)]
+    e0, e1 = hip.Event(), hip.Event()
+
+    def run(nframes, frame, reset=None):
+        curve, t, done = [], 0.0, 0
+        ctx.reset_frames()
+        frame(0)  # warm-up frame: allocation and graph capture
+        stream.synchronize()
+        ctx.reset_frames()
+        if reset:
+            reset()
+        for cp in [c for c in checkpoints if c < nframes] + [nframes]:
+            e0.record(stream)
+            for f in range(done, cp):
+                frame(f)
+            e1.record(stream)
+            e1.synchronize()
+            t += e0.elapsed_ms(e1)
+            done = cp
+            acc, spp = ctx.read_accumulator(w, h)
+            curve.append((cp, round(t, 3), mse(acc, spp)))
+        return curve
+
+    plain_seeds = np.arange(2 * 10 ** 8, 2 * 10 ** 8 + plain_max, dtype=np.uint64)
+    plain_dirs = [dirs(p, 41) for p in range(plain_max)]
+
+    def plain_frame(f):
+        ctx.render_passes(w, h, plain_seeds[f:f + 1], plain_dirs[f][None])
+
+    plain = run(plain_max, plain_frame)
+    log(f"[bench] cache leg: plain {plain_max} spp in {plain[-1][1]:.0f} ms, RMSE {max(plain[-1][2], 0) ** 0.5:.5f}")
+
+    cs = ptamd.CacheSettings(ntrain, steps, extra, train=True)
+
+    def cache_frame(f):
+        ctx.set_primary_rays(dirs(f, 43), eye)
+        ctx.render_cache_frame(w, h, 3 * 10 ** 8 + f, cs)
+
+    # A fresh cache: its training starts with the timed frames.
+    ctx.cache_init(7, bmin, bmax)
+    cached = run(frames, cache_frame, reset=lambda: ctx.cache_init(7, bmin, bmax))
+    log(f"[bench] cache leg: {frames} cache frames in {cached[-1][1]:.0f} ms, "
+        f"RMSE {max(cached[-1][2], 0) ** 0.5:.5f}")
+
+    def rmse(m):
+        return round(max(m, 0.0) ** 0.5, 6)
+
+    def time_to(curve, target_mse):
+        prev = None
+        for spp, t, m in curve:
+            if m <= target_mse:
+                if prev is None or prev[2] <= m:
+                    return t
+                # log-log interpolation between the two checkpoints
+                import math
+                a = (math.log(prev[2]) - math.log(target_mse)) / (math.log(prev[2]) - math.log(max(m, 1e-30)))
+                return prev[1] * (t / prev[1]) ** a
+            prev = (spp, t, m)
+        return None
+
+    def at_time(curve, t_target):
+        best = None
+        for spp, t, m in curve:
+            if t <= t_target:
+                best = m
+        return best
+
+    target = plain[-1][2]
+    t_plain = plain[-1][1]
+    t_cache = time_to(cached, target)
+    plain_at_cache_end = at_time(plain, cached[-1][1])
+    out = {
+        "resolution": [w, h], "frames": frames, "train_samples_per_frame": ntrain, "adam_steps_per_frame": steps,
+        "extra_bounces": extra, "plain_bounces": 1 + extra, "rr_start": 3, "reference_spp": ref_spp,
+        "reference_rmse_noise": rmse(ref_var),
+        "plain_curve": [(c, t, rmse(m)) for c, t, m in plain],
+        "cache_curve": [(c, t, rmse(m)) for c, t, m in cached],
+        "target_rmse_plain_1024spp": rmse(target), "plain_ms_to_target": round(t_plain, 1),
+        "cache_ms_to_target": None if t_cache is None else round(t_cache, 1),
+        "time_ratio_cache_vs_plain": None if t_cache is None else round(t_cache / t_plain, 4),
+        "equal_time_rmse": {"ms": cached[-1][1], "cache": rmse(cached[-1][2]),
+                            "plain": None if plain_at_cache_end is None else rmse(plain_at_cache_end)},
+        "ms_per_frame": {"plain": round(t_plain / plain_max, 3), "cache": round(cached[-1][1] / frames, 3)},
+    }
+
+    # MLP kernels alone on this workload's sizes: inference over every pixel's
+    # query and one Adam step over a training chunk, against the dense fp16
+    # MFMA peak (2P flops per query, 6P per training sample, P = 18,624 MACs
+    # per sample plus biases).
+    P = 18624
+    rng = np.random.default_rng(0)
+    npix = w * h
+    raw = np.zeros((npix, 12), np.float32)
+    raw[:, :3] = rng.uniform(bmin, bmax, (npix, 3))
+    raw[:, 3:9] = rng.normal(size=(npix, 6))
+    raw[:, 9:] = 0.5
+    d_raw = hip.DeviceArray.from_host(raw)
+    d_out = hip.DeviceArray((npix, 3), np.float32)
+    chunk = ntrain // steps
+    d_tgt = hip.DeviceArray.from_host(rng.uniform(0, 1, (chunk, 3)).astype(np.float32))
+    for _ in range(3):
+        ctx.cache_infer_device(d_raw.ptr, npix, d_out.ptr, stream.handle)
+        ctx.cache_train_device(d_raw.ptr, d_tgt.ptr, chunk, 0, stream.handle)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.cache_infer_device(d_raw.ptr, npix, d_out.ptr, stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    inf_ms = e0.elapsed_ms(e1) / reps
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.cache_train_device(d_raw.ptr, d_tgt.ptr, chunk, 0, stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    tr_ms = e0.elapsed_ms(e1) / reps
+    out["mlp"] = {"infer_ms": round(inf_ms, 4), "infer_queries": npix,
+                  "infer_tflops": round(2 * P * npix / (inf_ms * 1e-3) / 1e12, 2),
+                  "train_step_ms": round(tr_ms, 4), "train_samples": chunk,
+                  "train_tflops": round(6 * P * chunk / (tr_ms * 1e-3) / 1e12, 2),
+                  "mfma_fp16_dense_peak_tflops": 2500.0}
+    log(f"[bench] cache leg: target RMSE {out['target_rmse_plain_1024spp']}: plain {t_plain:.0f} ms, "
+        f"cache {out['cache_ms_to_target']} ms; MLP infer {inf_ms:.3f} ms "
+        f"({out['mlp']['infer_tflops']} TFLOP/s), train step {tr_ms:.3f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -38,6 +212,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched leg")
+    ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache leg")
+    ap.add_argument("--cache-full", action="store_true",
+                    help="radiance-cache leg at 1920x1080 with 65,536 training samples per frame "
+                         "(default: 960x540, training scaled to 16,384)")
+    ap.add_argument("--cache-ref-spp", type=int, default=0,
+                    help="reference spp of the cache leg (default 8192, 4096 with --cache-full)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported (this build runs on one GPU; got {args.gpus})",
@@ -45,6 +225,7 @@ def main():
         sys.exit(2)
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.no_tune, args.no_batch = 3, 2, True, True, True
+        args.no_cache = True
 
     import numpy as np
 
@@ -207,6 +388,8 @@ def main():
             f"{single_build_ms:.2f} ms one per launch; frame {frame_ms:.2f} ms "
             f"({batched['aggregate_Mrays_per_s']} Mrays/s) vs {single_ms:.2f} ms one scene per frame")
 
+    cache = None if args.no_cache else cache_leg(ctx, stream, args, seed_mesh)
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -260,6 +443,7 @@ def main():
         "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
         "render": render,
         "batched_scenes": batched,
+        "radiance_cache": cache,
         "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),
                       "tri_tests_per_ray": round(tri_tests / n, 3),
                       "algorithmic_bytes_per_ray": round(bytes_per_launch / n, 1), "hit_rate": round(hit_rate, 4)},

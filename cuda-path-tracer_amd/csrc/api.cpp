@@ -121,6 +121,7 @@ void upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* off
     require(n == 0 || tris != nullptr, "tris is NULL");
     require(ninfos == 0 || infos != nullptr, "infos is NULL");
     require(nlights == 0 || lights != nullptr, "lights is NULL");
+    require(nlights <= 64, "at most 64 directional lights");
     require(n < (1u << 30), "at most 2^30 - 1 triangles");
     uint32_t max_id = 0;
     for (uint32_t i = 0; i < n; ++i) max_id = std::max(max_id, tris[i].id);
@@ -163,8 +164,13 @@ void upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* off
         PT_HIP(hipMemcpyAsync(s.infos.as<void>(), infos, sizeof(pt_triangle_info) * static_cast<size_t>(ninfos),
                               hipMemcpyHostToDevice, st));
     }
+    s.nlights = nlights;
+    if (nlights) {
+        s.lights.reserve(sizeof(pt_dir_light) * nlights);
+        PT_HIP(hipMemcpyAsync(s.lights.as<void>(), lights, sizeof(pt_dir_light) * nlights, hipMemcpyHostToDevice,
+                              st));
+    }
     if (settings) ctx->renderer.settings = *settings;
-    ctx->renderer.lights = nlights;
     pt::build_lbvh(s, st);
     PT_HIP(hipStreamSynchronize(st));
     s.ready = true;
@@ -326,8 +332,6 @@ int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed) {
     if (!ctx) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
-        if (ctx->renderer.lights != 0)
-            throw pt::Error(PT_E_UNSUPPORTED, "directional lights are not supported by the renderer");
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
         PT_HIP(hipEventRecord(ctx->ev0, st));
@@ -342,8 +346,6 @@ int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const floa
     if (!ctx || !seeds) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
-        if (ctx->renderer.lights != 0)
-            throw pt::Error(PT_E_UNSUPPORTED, "directional lights are not supported by the renderer");
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
         PT_HIP(hipEventRecord(ctx->ev0, st));
@@ -504,8 +506,6 @@ int pt_render_cache_frame(pt_context* ctx, int w, int h, uint64_t pass_seed,
     if (!ctx || !settings) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
-        if (ctx->renderer.lights != 0)
-            throw pt::Error(PT_E_UNSUPPORTED, "directional lights are not supported by the renderer");
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
         PT_HIP(hipEventRecord(ctx->ev0, st));

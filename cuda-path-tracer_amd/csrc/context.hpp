@@ -121,6 +121,8 @@ struct DeviceScene {
     DevBuf morton_box;                     // per scene: min.xyz, 1/(max-min).xyz
     DevBuf tris;                           // pt_triangle[n] as uploaded
     DevBuf infos;                          // pt_triangle_info[ninfos]
+    DevBuf lights;                         // pt_dir_light[nlights] (all scenes)
+    uint32_t nlights = 0;
     DeviceBvh bvh;
     bool ready = false;
     uint32_t nscenes() const { return static_cast<uint32_t>(off_host.size()) - 1; }
@@ -185,7 +187,6 @@ struct Renderer {
     int ring_next = 0;
     void* graph_exec = nullptr;   // hipGraphExec_t of the last captured frame
     alignas(8) char graph_key[160] = {};
-    uint32_t lights = 0;  // directional lights in the scene (not rendered)
     bool timed = false;   // a pass has been bracketed by the context events
     uint32_t last_levels = 0;  // queue levels of the last frame
     DevBuf cache_io;           // cache-frame queries and training batch

@@ -41,6 +41,7 @@ struct FrameConst {
     uint32_t rr_start;
     float intensity;
     uint32_t stratified;
+    uint32_t nlights;  // directional lights (shadow rays at every vertex)
 };
 
 // Structure-of-arrays queue of rays (origin, direction, path index).
@@ -151,6 +152,26 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
 
+// Light reaching `pos` from a directional light (computeL, LightAlgorithms.h:
+// 12-20): a shadow ray from pos - dir * 1e-3 towards the light; any hit
+// (closest-hit t < FLT_MAX, trace.h:45-50) shadows it.
+__device__ __forceinline__ V3 light_L(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                      uint32_t b, uint32_t e, V3 pos, const pt_dir_light& lt) {
+    const V3 ld = ld3(lt.direction);
+    const TraceHit h = trace_ray_in(nodes, tris, b, e, sub(pos, scale(ld, 1e-3f)), neg(ld));
+    if (h.t < kFltMax) return v3(0.0f, 0.0f, 0.0f);
+    return scale(ld3(lt.color), lt.intensity);
+}
+
+// blinnPhongBRDF (cuda_path_tracer.cu:36-50).
+__device__ __forceinline__ V3 blinn_phong(V3 Id, V3 kd, V3 Is, V3 ks, V3 n, V3 l, V3 hw, float shininess) {
+    const float df = gmax(dot(n, l), 0.0f);
+    const V3 diffuse = scale(mul(Id, kd), df);
+    V3 specular = v3(0.0f, 0.0f, 0.0f);
+    if (shininess > 0.0f && df > 0.0f) specular = scale(mul(Is, ks), pow_pos(gmax(dot(n, hw), 0.0f), shininess));
+    return add(diffuse, specular);
+}
+
 // CACHE: a cache frame (one pass; paths >= npix are training paths).
 template <bool CACHE>
 __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
@@ -158,7 +179,11 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                                                     const pt_triangle_info* __restrict__ infos,
                                                     RayQueue qin, HitQueue hq, RayQueue qout,
                                                     PathState ps, uint32_t* __restrict__ counters,
-                                                    uint32_t level, CacheIO io) {
+                                                    uint32_t level, CacheIO io,
+                                                    const pt_dir_light* __restrict__ lights,
+                                                    const float4* __restrict__ nodes,
+                                                    const float4* __restrict__ tris,
+                                                    const uint32_t* __restrict__ off) {
     const uint32_t count = counters[level];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -191,12 +216,23 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
             const bool emissive = ((e.x + e.y) + e.z) > 1e-7f;
             V3 absorbed, acc;
             if (level == 0) {
-                // Primary hit: emission or (no directional lights) zero
-                // (cuda_path_tracer.cu:194-199, 258-259).
+                // Primary hit: emission, else Blinn-Phong from the directional
+                // lights with shadow rays, the light term L summed over the
+                // lights seen so far (cuda_path_tracer.cu:194-214, 258-259).
                 acc = v3(0.0f, 0.0f, 0.0f);
                 if (emissive) {
                     acc = e;
                     albedo = v3(1.0f, 1.0f, 1.0f);
+                } else if (fc.nlights) {
+                    const uint32_t sb = off[P.scene], se = off[P.scene + 1];
+                    V3 Lc = v3(0.0f, 0.0f, 0.0f);
+                    for (uint32_t k = 0; k < fc.nlights; ++k) {
+                        const V3 ld = ld3(lights[k].direction);
+                        Lc = add(Lc, light_L(nodes, tris, sb, se, pos, lights[k]));
+                        const V3 hw = scale(sub(neg(d), ld), 0.5f);
+                        acc = add(acc, blinn_phong(Lc, albedo, Lc, v3(0.5f, 0.5f, 0.5f), nrm, neg(ld), hw,
+                                                   in.specularity));
+                    }
                 }
                 absorbed = albedo;
             } else {
@@ -208,6 +244,13 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                     const float c = gmax(dot(neg(d), nrm), 0.0f);
                     L = scale(e, c);
                     albedo = v3(1.0f, 1.0f, 1.0f);
+                } else if (fc.nlights) {
+                    // Shadow rays to the lights with the receiver cosine (:314-321).
+                    const uint32_t sb = off[P.scene], se = off[P.scene + 1];
+                    for (uint32_t k = 0; k < fc.nlights; ++k) {
+                        const V3 ld = ld3(lights[k].direction);
+                        L = add(L, scale(light_L(nodes, tris, sb, se, pos, lights[k]), gmax(dot(neg(ld), nrm), 0.0f)));
+                    }
                 }
                 absorbed = mul(absorbed, albedo);
                 acc = add(acc, scale(mul(absorbed, L), fc.intensity));
@@ -455,6 +498,7 @@ struct GraphKey {
     const void* params;
     const void* accum;
     const void* counters;
+    const void* lights;
     uint32_t blocks, refill;
     hipStream_t stream;
 };
@@ -477,7 +521,8 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
                            scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
         hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
-                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, CacheIO{});
+                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, CacheIO{}, scene.lights.as<pt_dir_light>(),
+                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
     }
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
                        r.accum.as<double>());
@@ -592,6 +637,7 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
     fc.rr_start = r.settings.rr_start;
     fc.intensity = r.settings.indirect_intensity;
     fc.stratified = r.settings.stratified;
+    fc.nlights = scene.nlights;
 
     if (!render_tuning().graphs) {
         launch_frame(r, scene, fc, st);
@@ -608,6 +654,7 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         key.params = r.params.as<void>();
         key.accum = r.accum.as<void>();
         key.counters = r.counters.as<void>();
+        key.lights = scene.lights.as<void>();
         key.blocks = trace_tuning().blocks;
         key.refill = trace_tuning().refill;
         key.stream = st;
@@ -649,7 +696,8 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
                            scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
         hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
-                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io);
+                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
+                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
     }
     cache.infer(io.qraw, fc.npix, io.qout, st);
     hipLaunchKernelGGL(k_cache_resolve, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, io, L.ps);
@@ -672,7 +720,7 @@ struct CacheGraphKey {
     FrameConst fc;
     CacheIO io;
     uint32_t steps, train, blocks, refill;
-    const void* ptrs[12];
+    const void* ptrs[13];
     float box[6], adam[4];
     hipStream_t stream;
 };
@@ -700,6 +748,7 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     cf.fc.rr_start = r.settings.rr_start;
     cf.fc.intensity = r.settings.indirect_intensity;
     cf.fc.stratified = r.settings.stratified;
+    cf.fc.nlights = scene.nlights;
     cf.steps = cs.train_steps;
     cf.train = (cs.flags & PT_CACHE_TRAIN) != 0;
 
@@ -737,10 +786,11 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
         key.train = cf.train;
         key.blocks = trace_tuning().blocks;
         key.refill = trace_tuning().refill;
-        const void* ptrs[12] = {scene.bvh.nodes.as<void>(), scene.bvh.tris.as<void>(), scene.infos.as<void>(),
+        const void* ptrs[13] = {scene.bvh.nodes.as<void>(), scene.bvh.tris.as<void>(), scene.infos.as<void>(),
                                 scene.off.as<void>(),       r.queues.as<void>(),     r.params.as<void>(),
                                 r.accum.as<void>(),         r.counters.as<void>(),   cache.master.as<void>(),
-                                cache.acts.as<void>(),      cache.part.as<void>(),   cache.loss.as<void>()};
+                                cache.acts.as<void>(),      cache.part.as<void>(),   cache.loss.as<void>(),
+                                scene.lights.as<void>()};
         std::memcpy(key.ptrs, ptrs, sizeof ptrs);
         for (int k = 0; k < 3; ++k) {
             key.box[k] = cache.box_min[k];

@@ -43,13 +43,13 @@ __device__ __forceinline__ float tri_hit(const float4* __restrict__ tris, uint32
 template <bool COUNT, bool FINITE>
 __device__ __forceinline__ TraceHit traverse_impl(const float4* __restrict__ nodes,
                                                   const float4* __restrict__ tris, V3 o, V3 d,
-                                                  V3 dinv, TraceCount& cnt) {
+                                                  V3 dinv, TraceCount& cnt, uint32_t root = 0) {
     float minT = kMinTInit;
     uint32_t best = kInvalid;
     float bu = 0.0f, bv = 0.0f;
     uint32_t stack[kStackMax];
     int sp = 0;
-    uint32_t node = 0;
+    uint32_t node = root;
     for (;;) {
         for (;;) {
             const float4* rec = nodes + 4 * static_cast<size_t>(node);
@@ -137,6 +137,28 @@ __device__ __forceinline__ TraceHit trace_ray(const float4* __restrict__ nodes,
                         __builtin_fabsf(dinv.z) <= kFltMax;
     if (finite) return traverse_impl<COUNT, true>(nodes, tris, o, d, dinv, cnt);
     return traverse_impl<COUNT, false>(nodes, tris, o, d, dinv, cnt);
+}
+
+// One ray in the scene whose triangles (and internal nodes) occupy slots
+// [b, e) of a batched BVH; a single scene has b = 0.
+__device__ __forceinline__ TraceHit trace_ray_in(const float4* __restrict__ nodes,
+                                                 const float4* __restrict__ tris, uint32_t b, uint32_t e,
+                                                 V3 o, V3 d) {
+    TraceHit miss{kFltMax, kInvalid, 0.0f, 0.0f};
+    TraceCount cnt{0, 0, 0};
+    if (e - b == 0) return miss;
+    if (e - b == 1) {
+        float u = 0.0f, v = 0.0f;
+        uint32_t id;
+        const float h = tri_hit(tris, b, o, d, u, v, id);
+        if (h <= kMinTInit) return TraceHit{h, id, u, v};
+        return miss;
+    }
+    const V3 dinv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool finite = __builtin_fabsf(dinv.x) <= kFltMax && __builtin_fabsf(dinv.y) <= kFltMax &&
+                        __builtin_fabsf(dinv.z) <= kFltMax;
+    if (finite) return traverse_impl<false, true>(nodes, tris, o, d, dinv, cnt, b);
+    return traverse_impl<false, false>(nodes, tris, o, d, dinv, cnt, b);
 }
 
 // ---- persistent traversal -------------------------------------------------

@@ -65,8 +65,10 @@ typedef struct pt_triangle_info {
     float specularity;
 } pt_triangle_info;
 
-/* Directional light (reference: CUDA::DirectionalLight, Lights.h:6-11).
- * Accepted by the ABI; rendering with lights returns PT_E_UNSUPPORTED. */
+/* Directional light (reference: CUDA::DirectionalLight, Lights.h:6-11);
+ * direction points from the light into the scene; at most 64 per scene.
+ * Primary hits get Blinn-Phong from every light, bounce hits its cosine
+ * term, each behind a shadow ray (LightAlgorithms.h:12-20). */
 typedef struct pt_dir_light {
     float color[3];
     float direction[3];

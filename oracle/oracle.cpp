@@ -288,6 +288,103 @@ inline ShadeInfo extract(const pt_triangle_info& in, float u, float v) {
 
 inline bool emissive(V3 e) { return ((e.x + e.y) + e.z) > 1e-7f; }  // CUDA_EPSILON
 
+inline float bits_to_float(uint32_t b) {
+    float f;
+    std::memcpy(&f, &b, sizeof f);
+    return f;
+}
+
+// x^y for x >= 0, y > 0 (the Blinn-Phong specular power; the build's own
+// function, not libm's pow): 2^(y log2 x), log2 of the mantissa m in
+// [sqrt(1/2), sqrt(2)) from atanh((m-1)/(m+1)) (odd series to s^11), 2^f for
+// |f| <= 1/2 from exp's Taylor series to degree 9; below 2^-125 the result is
+// 0.  Same float operations as the kernels.
+float pow_pos(float x, float y) {
+    if (!(x > 0.0f)) return 0.0f;
+    if (x > FLT_MAX) return x;
+    int e = 0;
+    if (x < 1.17549435e-38f) {
+        x = x * 16777216.0f;
+        e = -24;
+    }
+    uint32_t bits;
+    std::memcpy(&bits, &x, sizeof bits);
+    e += static_cast<int>((bits >> 23) & 0xffu) - 127;
+    float m = bits_to_float((bits & 0x7fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float p = 1.0f / 11.0f;
+    p = p * s2 + 1.0f / 9.0f;
+    p = p * s2 + 1.0f / 7.0f;
+    p = p * s2 + 1.0f / 5.0f;
+    p = p * s2 + 1.0f / 3.0f;
+    p = p * s2 + 1.0f;
+    const float log2m = (2.88539008f * s) * p;
+    const float t = y * static_cast<float>(e) + y * log2m;
+    if (t < -125.0f) return 0.0f;
+    if (t > 127.0f) return bits_to_float(0x7f800000u);
+    const float n = std::rint(t);
+    const float f = (t - n) * 0.693147181f;
+    float q = 1.0f / 362880.0f;
+    q = q * f + 1.0f / 40320.0f;
+    q = q * f + 1.0f / 5040.0f;
+    q = q * f + 1.0f / 720.0f;
+    q = q * f + 1.0f / 120.0f;
+    q = q * f + 1.0f / 24.0f;
+    q = q * f + 1.0f / 6.0f;
+    q = q * f + 0.5f;
+    q = q * f + 1.0f;
+    q = q * f + 1.0f;
+    return q * bits_to_float(static_cast<uint32_t>(static_cast<int>(n) + 127) << 23);
+}
+
+// computeL (LightAlgorithms.h:12-20): shadow ray from pos - dir * 1e-3
+// (CUDA_EPSILON3) towards the light, a closest-hit trace (trace.h:45-50);
+// in shadow iff t < FLT_MAX.
+V3 light_L(const Bvh& bvh, V3 pos, const pt_dir_light& lt) {
+    const V3 ld = vld(lt.direction);
+    const Hit h = trace_one(bvh, sub(pos, scale(ld, 1e-3f)), neg(ld), nullptr);
+    if (h.t < FLT_MAX) return V3{0.0f, 0.0f, 0.0f};
+    return scale(vld(lt.color), lt.intensity);
+}
+
+// blinnPhongBRDF (cuda_path_tracer.cu:36-50).
+V3 blinn_phong(V3 Id, V3 kd, V3 Is, V3 ks, V3 n, V3 l, V3 hw, float shininess) {
+    const float df = gmax(dot(n, l), 0.0f);
+    const V3 diffuse = scale(mul(Id, kd), df);
+    V3 specular{0.0f, 0.0f, 0.0f};
+    if (shininess > 0.0f && df > 0.0f) specular = scale(mul(Is, ks), pow_pos(gmax(dot(n, hw), 0.0f), shininess));
+    return add(diffuse, specular);
+}
+
+// Primary-hit shading of a non-emissive surface (cuda_path_tracer.cu:201-214):
+// L sums the lights seen so far, each light adds Blinn-Phong with that L.
+V3 primary_lights(const Bvh& bvh, V3 pos, V3 d, V3 nrm, V3 albedo, float spec, const pt_dir_light* lights,
+                  uint32_t nlights) {
+    V3 Lc{0.0f, 0.0f, 0.0f}, color{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < nlights; ++k) {
+        const V3 ld = vld(lights[k].direction);
+        Lc = add(Lc, light_L(bvh, pos, lights[k]));
+        const V3 hw = scale(sub(neg(d), ld), 0.5f);
+        color = add(color, blinn_phong(Lc, albedo, Lc, V3{0.5f, 0.5f, 0.5f}, nrm, neg(ld), hw, spec));
+    }
+    return color;
+}
+
+// Bounce-hit light term of a non-emissive surface (cuda_path_tracer.cu:314-321).
+V3 bounce_lights(const Bvh& bvh, V3 pos, V3 nrm, const pt_dir_light* lights, uint32_t nlights) {
+    V3 L{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < nlights; ++k) {
+        const V3 ld = vld(lights[k].direction);
+        L = add(L, scale(light_L(bvh, pos, lights[k]), gmax(dot(neg(ld), nrm), 0.0f)));
+    }
+    return L;
+}
+
 }  // namespace
 
 extern "C" {
@@ -550,6 +647,15 @@ void or_render_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_
                     const pt_settings* st, int w, int h, const float* dirs, const float* eye,
                     uint32_t pass, uint64_t seed, double* accum, uint64_t* seg_counts,
                     int threads) {
+    or_render_pass_lit(tris, perm, children, boxes, n, infos, nullptr, 0, st, w, h, dirs, eye, pass, seed,
+                       accum, seg_counts, threads);
+}
+
+void or_render_pass_lit(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                        const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                        const pt_dir_light* lights, uint32_t nlights, const pt_settings* st, int w,
+                        int h, const float* dirs, const float* eye, uint32_t pass, uint64_t seed,
+                        double* accum, uint64_t* seg_counts, int threads) {
     Bvh bvh{tris, perm, children, boxes, n};
     const V3 d00 = vld(dirs), d10 = vld(dirs + 3), d11 = vld(dirs + 6), d01 = vld(dirs + 9);
     const V3 e = vld(eye);
@@ -578,12 +684,14 @@ void or_render_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_
             V3 pos = add(scale(d, hit.t), o);
             ShadeInfo si = extract(infos[hit.id], hit.u, hit.v);
             V3 albedo = si.albedo;
+            V3 nrm = si.normal;
             if (emissive(si.emission)) {
                 accumc = si.emission;
                 albedo = V3{1.0f, 1.0f, 1.0f};
+            } else if (nlights) {
+                accumc = primary_lights(bvh, pos, d, nrm, albedo, infos[hit.id].specularity, lights, nlights);
             }
             V3 absorbed = albedo;
-            V3 nrm = si.normal;
             // Indirect bounces (cuda_path_tracer.cu:267-330).
             for (uint32_t i = 0; i < bounces; ++i) {
                 uint32_t rv[4] = {static_cast<uint32_t>(pix), pass, i, seedw};
@@ -623,6 +731,8 @@ void or_render_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_
                     c = gmax(c, 0.0f);
                     L = scale(si.emission, c);
                     albedo = V3{1.0f, 1.0f, 1.0f};
+                } else if (nlights) {
+                    L = bounce_lights(bvh, pos, nrm, lights, nlights);
                 }
                 absorbed = mul(absorbed, albedo);
                 accumc = add(accumc, scale(mul(absorbed, L), intensity));
@@ -648,6 +758,7 @@ void or_render_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_
 // into train_target[3k..]; NaN when it never reaches x2.
 void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
                           const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                          const pt_dir_light* lights, uint32_t nlights,
                           const pt_settings* st, int w, int h, const float* dirs, const float* eye,
                           uint32_t pass, uint64_t seed, const float* params, const float* cbmin,
                           const float* cbmax, uint32_t ntrain, uint32_t extra_bounces, double* accum,
@@ -681,12 +792,14 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
         V3 pos = add(scale(d, hit.t), o);
         ShadeInfo si = extract(infos[hit.id], hit.u, hit.v);
         V3 albedo = si.albedo;
+        V3 nrm = si.normal;
         if (emissive(si.emission)) {
             acc = si.emission;
             albedo = V3{1.0f, 1.0f, 1.0f};
+        } else if (nlights) {
+            acc = primary_lights(bvh, pos, d, nrm, albedo, infos[hit.id].specularity, lights, nlights);
         }
         V3 absorbed = albedo;
-        V3 nrm = si.normal;
         for (uint32_t i = 0; i + 1 < levels; ++i) {
             uint32_t rv[4] = {key, pass, i, sw};
             pcg4d(rv);
@@ -720,6 +833,8 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
                 c = gmax(c, 0.0f);
                 L = scale(si.emission, c);
                 albedo = V3{1.0f, 1.0f, 1.0f};
+            } else if (nlights) {
+                L = bounce_lights(bvh, pos, nrm, lights, nlights);
             }
             absorbed = mul(absorbed, albedo);
             acc = add(acc, scale(mul(absorbed, L), intensity));
@@ -783,5 +898,7 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
         }
     }
 }
+
+float or_pow_pos(float x, float y) { return pow_pos(x, y); }
 
 }  // extern "C"

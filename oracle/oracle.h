@@ -88,6 +88,17 @@ double or_mlp_loss_grad(const float* params, const float* raw, const float* targ
 void or_adam_step(float* w, float* m, float* v, const double* g, uint32_t count, uint32_t t,
                   float lr, float b1, float b2, float eps);
 
+/* The same pass with directional lights (a10: shadowRay trace.h:45-50,
+ * computeL LightAlgorithms.h:12-20, primary Blinn-Phong
+ * cuda_path_tracer.cu:163-214, bounce light term :314-321). */
+void or_render_pass_lit(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                        const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                        const pt_dir_light* lights, uint32_t nlights, const pt_settings* settings,
+                        int w, int h, const float* dirs, const float* eye, uint32_t pass,
+                        uint64_t seed, double* accum, uint64_t* seg_counts, int threads);
+/* x^y for x >= 0, y > 0 as the specular power is computed (own function). */
+float or_pow_pos(float x, float y);
+
 /* Learned-cache frame (build addition, no reference counterpart): pixel
  * paths end at their second vertex x2 and add throughput * cache(x2) (FP64
  * MLP with `params`, box [cbmin, cbmax]) into accum; ntrain training paths
@@ -95,6 +106,7 @@ void or_adam_step(float* w, float* m, float* v, const double* g, uint32_t count,
  * reached x2).  Follows or_render_pass for the shared path steps. */
 void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
                           const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                          const pt_dir_light* lights, uint32_t nlights,
                           const pt_settings* settings, int w, int h, const float* dirs,
                           const float* eye, uint32_t pass, uint64_t seed, const float* params,
                           const float* cbmin, const float* cbmax, uint32_t ntrain,

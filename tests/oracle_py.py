@@ -33,8 +33,11 @@ _SIGS = {
     "or_mlp_forward": ([_vp, _vp, _u32, _vp, _vp, _vp], None),
     "or_mlp_loss_grad": ([_vp, _vp, _vp, _u32, _vp, _vp, _vp], ctypes.c_double),
     "or_adam_step": ([_vp, _vp, _vp, _vp, _u32, _u32, _f, _f, _f, _f], None),
-    "or_render_cache_pass": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp, _vp, _vp,
-                              _u32, _u32, _vp, _vp, _vp, _i], None),
+    "or_render_cache_pass": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp,
+                              _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i], None),
+    "or_render_pass_lit": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp, _vp,
+                            _i], None),
+    "or_pow_pos": ([_f, _f], _f),
     "or_pcg4d": ([_vp, _vp], None),
     "or_sincos2pi": ([_f, _vp, _vp], None),
     "or_to_hemisphere": ([_vp, _vp, _vp], None),
@@ -206,8 +209,33 @@ def adam_step(w, m, v, g, t, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
     _lib.or_adam_step(_p(w), _p(m), _p(v), _p(g), len(w), t, lr, b1, b2, eps)
 
 
+def _lights(lights):
+    if lights is None or len(lights) == 0:
+        return None, 0
+    la = np.ascontiguousarray(np.asarray(lights, np.float32).reshape(-1, 7))
+    return la, len(la)
+
+
+def render_pass_lit(tris, bvh, infos, lights, settings, w, h, dirs, eye, pass_index, seed, accum=None,
+                    threads=THREADS):
+    """or_render_pass with directional lights: rows (color.rgb, direction.xyz, intensity)."""
+    if accum is None:
+        accum = np.zeros((h, w, 3), np.float64)
+    la, nl = _lights(lights)
+    segs = np.zeros(1, np.uint64)
+    dirs, eye = _f32(dirs), _f32(eye)
+    _lib.or_render_pass_lit(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
+                            _p(infos), _p(la), nl, ctypes.byref(settings), w, h, _p(dirs), _p(eye),
+                            pass_index, seed, _p(accum), _p(segs), threads)
+    return accum, int(segs[0])
+
+
+def pow_pos(x, y):
+    return float(_lib.or_pow_pos(float(np.float32(x)), float(np.float32(y))))
+
+
 def render_cache_pass(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, seed, params, cbmin, cbmax,
-                      ntrain, extra_bounces, accum=None, threads=THREADS):
+                      ntrain, extra_bounces, accum=None, threads=THREADS, lights=None):
     """One cache frame: returns (accum, train_raw (ntrain, 12), train_target (ntrain, 3))."""
     if accum is None:
         accum = np.zeros((h, w, 3), np.float64)
@@ -215,8 +243,10 @@ def render_cache_pass(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, s
     tgt = np.zeros((ntrain, 3), np.float32)
     dirs, eye, params = _f32(dirs), _f32(eye), _f32(params)
     cbmin, cbmax = _f32(cbmin), _f32(cbmax)
+    la, nl = _lights(lights)
     _lib.or_render_cache_pass(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
-                              _p(infos), ctypes.byref(settings), w, h, _p(dirs), _p(eye), pass_index, seed,
+                              _p(infos), _p(la), nl, ctypes.byref(settings), w, h, _p(dirs), _p(eye),
+                              pass_index, seed,
                               _p(params), _p(cbmin), _p(cbmax), ntrain, extra_bounces, _p(accum), _p(raw),
                               _p(tgt), threads)
     return accum, raw, tgt

@@ -159,3 +159,33 @@ def test_cache_lowers_rmse_at_equal_frames(ctx, room):
     r_cache = float(np.sqrt(np.mean((cached - ref) ** 2)))
     say(f"  RMSE after {frames} frames: plain {r_plain:.5f}, cache {r_cache:.5f}")
     assert np.isfinite(r_cache) and r_cache < r_plain
+
+
+def test_cache_frame_with_lights_matches_oracle(ctx):
+    from test_gpu_render import lit_scene
+    tris, infos, lights = lit_scene()
+    st = ptamd.Settings(6, 3, 1.0, 1)
+    eye, tgt = (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)
+    ctx.upload_scene(tris, infos, settings=st, lights=lights)
+    dirs = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, W / H)
+    ctx.set_primary_rays(dirs, eye)
+    ctx.reset_frames()
+    bmin, bmax = ptamd.scene_bounds(tris)
+    ctx.cache_init(5, bmin, bmax)
+    p = random_params(23, bias=0.2)
+    ctx.cache_set_params(p)
+    ntrain = 900
+    say("[cache frame] soup with directional lights")
+    ctx.render_cache_frame(W, H, 4242, ptamd.CacheSettings(ntrain, 1, 5, train=False))
+    raw, tgt_g = ctx.cache_last_batch(ntrain)
+    bvh = orc.build(tris, bmin, bmax)
+    acc_o, oraw, otgt = orc.render_cache_pass(tris, bvh, infos, st, W, H, dirs, eye, 0, 4242, fp16_weights(p),
+                                              bmin, bmax, ntrain, 5, lights=lights)
+    valid = ~np.isnan(otgt[:, 0])
+    assert valid.sum() > 50
+    np.testing.assert_array_equal(np.isnan(tgt_g[:, 0]), ~valid)
+    np.testing.assert_array_equal(raw[valid], oraw[valid])
+    np.testing.assert_array_equal(tgt_g[valid], otgt[valid])
+    acc_g, _ = ctx.read_accumulator(W, H)
+    assert max(rel_err(acc_g, acc_o)) <= 1e-3
+

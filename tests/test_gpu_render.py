@@ -144,3 +144,39 @@ def test_batched_scene_frame(ctx):
         segs += sg
     assert np.array_equal(acc, o)
     assert rays == segs
+
+
+def lit_scene():
+    from test_oracle_lights import light
+    tris, infos = ptamd.gen_soup(30000, 4)
+    rng = np.random.default_rng(1)
+    infos["specularity"] = rng.choice(np.array([0.0, 4.0, 32.0], np.float32), len(infos))
+    lights = np.array([light((1.0, 0.9, 0.8), (0.3, -1.0, 0.2), 1.5),
+                       light((0.2, 0.3, 1.0), (-0.5, -0.4, 1.0), 0.7)], np.float32)
+    return tris, infos, lights
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_render_with_lights_matches_oracle(ctx, graphs):
+    """Directional lights: Blinn-Phong at primary hits and the light term at
+    bounce hits, each behind a shadow ray, bit-identical to the oracle."""
+    tris, infos, lights = lit_scene()
+    st = ptamd.Settings(3, 2, 1.0, 1)
+    ctx.upload_scene(tris, infos, settings=st, lights=lights)
+    n = 3
+    seeds = np.arange(300, 300 + n, dtype=np.uint64)
+    dirs = np.stack([pass_dirs(p) for p in range(n)])
+    ptamd.tune_set("render.graphs", graphs)
+    try:
+        say(f"[lights] {n}-pass frame, graphs={graphs}")
+        ctx.reset_frames()
+        ctx.render_passes(W, H, seeds, dirs)
+        acc, spp = ctx.read_accumulator(W, H)
+    finally:
+        ptamd.tune_set("render.graphs", 1)
+    b = orc.build(tris, *ptamd.scene_bounds(tris))
+    o = np.zeros((H, W, 3))
+    for p in range(n):
+        orc.render_pass_lit(tris, b, infos, lights, st, W, H, dirs[p], EYE, p, int(seeds[p]), o)
+    assert spp == n and o.sum() > 0
+    assert np.array_equal(acc, o)
