@@ -7,8 +7,10 @@ rounded to fp16 between layers, so
   gradient   per-layer relative L2 error <= 2e-2 and cosine >= 0.999 against
              the oracle's FP64 gradient;
   Adam       the masters after one step equal the oracle's Adam step applied
-             to the GPU's own gradient within 4 fp32 ulp (same formula, fp32
-             on the GPU);
+             to the GPU's own gradient within 4 fp32 ulp of the weight plus
+             1e-6 of the learning rate (the GPU evaluates the update in fp32,
+             the oracle in fp64; near a zero crossing the weight's ulp is
+             smaller than the update's rounding);
   training   deterministic (two gradient calls are bit-identical) and a few
              hundred steps cut the relative L2 loss of a smooth field 10x.
 """
@@ -127,7 +129,8 @@ def test_adam_step_matches_oracle(cache):
         orc.adam_step(w, m, v, g.astype(np.float64), t, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8)
         got = cache.cache_get_params()
         ulp = np.spacing(np.abs(w)).astype(np.float64)
-        assert np.all(np.abs(got.astype(np.float64) - w) <= 4 * ulp + 1e-12), t
+        excess = np.abs(got.astype(np.float64) - w) - (4 * ulp + 1e-6 * 1e-3)
+        assert excess.max() <= 0, (t, int(np.argmax(excess)), float(excess.max()))
 
 
 def test_training_reduces_loss(cache):
@@ -140,12 +143,15 @@ def test_training_reduces_loss(cache):
                     0.2 + raw[:, 4] ** 2], axis=1).astype(np.float32)
     cache.cache_init(77, BMIN, BMAX)
     cache.cache_set_adam(lr=3e-3)
-    first = float(np.mean(cache.cache_train(raw, tgt)))
-    say(f"[cache] training, first loss {first:.4f}")
-    last = first
-    for it in range(300):
-        idx = rng.integers(0, n, 4096)
-        last = float(np.mean(cache.cache_train(raw[idx], tgt[idx])))
+    try:
+        first = float(np.mean(cache.cache_train(raw, tgt)))
+        say(f"[cache] training, first loss {first:.4f}")
+        last = first
+        for it in range(300):
+            idx = rng.integers(0, n, 4096)
+            last = float(np.mean(cache.cache_train(raw[idx], tgt[idx])))
+    finally:
+        cache.cache_set_adam()
     y = cache.cache_infer(raw)
     full = float(np.mean(np.sum((y - tgt) ** 2 / (y * y + 0.01), axis=1)))
     say(f"[cache] loss {first:.4f} -> {last:.4f} (full batch {full:.4f})")
@@ -154,10 +160,8 @@ def test_training_reduces_loss(cache):
 
 
 def test_cache_errors(ctx):
-    import ctypes
     with pytest.raises(ptamd.PtError):
         ctx.cache_gradient_device(0, 0, 0, 0)
     with pytest.raises(ptamd.PtError):
         ctx.cache_infer_device(0, 5, 0)
     ctx.cache_infer_device(0, 0, 0)  # empty batch is a no-op
-    del ctypes

@@ -174,7 +174,7 @@ def test_cache_frame_with_lights_matches_oracle(ctx):
     ctx.cache_init(5, bmin, bmax)
     p = random_params(23, bias=0.2)
     ctx.cache_set_params(p)
-    ntrain = 900
+    ntrain = W * H  # the soup is open: few training paths reach x2
     say("[cache frame] soup with directional lights")
     ctx.render_cache_frame(W, H, 4242, ptamd.CacheSettings(ntrain, 1, 5, train=False))
     raw, tgt_g = ctx.cache_last_batch(ntrain)
@@ -182,7 +182,7 @@ def test_cache_frame_with_lights_matches_oracle(ctx):
     acc_o, oraw, otgt = orc.render_cache_pass(tris, bvh, infos, st, W, H, dirs, eye, 0, 4242, fp16_weights(p),
                                               bmin, bmax, ntrain, 5, lights=lights)
     valid = ~np.isnan(otgt[:, 0])
-    assert valid.sum() > 50
+    assert valid.sum() > 60
     np.testing.assert_array_equal(np.isnan(tgt_g[:, 0]), ~valid)
     np.testing.assert_array_equal(raw[valid], oraw[valid])
     np.testing.assert_array_equal(tgt_g[valid], otgt[valid])
