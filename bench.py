@@ -56,6 +56,9 @@ def cache_leg(ctx, stream, args, seed_mesh):
     def dirs(p, seed):
         return ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, *ptamd.pass_jitter(seed, p, w, h))
 
+    # The eye of every pass (render_passes takes per-pass corner rays only).
+    ctx.set_primary_rays(dirs(0, 31), eye)
+
     # Reference: two halves of ref_spp / 2 passes each (16 passes per frame).
     halves = []
     for half in range(2):

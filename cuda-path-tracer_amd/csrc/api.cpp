@@ -520,4 +520,9 @@ int pt_cache_last_batch(pt_context* ctx, float* raw, float* target, uint32_t n) 
     return guarded(ctx, [&] { pt::render_last_batch(ctx->renderer, raw, target, n, ctx->stream); });
 }
 
+int pt_cache_last_queries(pt_context* ctx, float* raw, float* pred, uint32_t* valid) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { pt::render_last_queries(ctx->renderer, raw, pred, valid, ctx->stream); });
+}
+
 }  // extern "C"

@@ -220,6 +220,8 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
                         const pt_cache_settings& cs, hipStream_t stream);
 // The first n samples (raw, target) of the last cache frame's training batch.
 void render_last_batch(Renderer& r, float* raw, float* target, uint32_t n, hipStream_t stream);
+// The last cache frame's pixel queries: raw[12 w h], predictions[3 w h], valid[w h].
+void render_last_queries(Renderer& r, float* raw, float* pred, uint32_t* valid, hipStream_t stream);
 
 // Learned radiance cache: the MLP's masters, Adam moments, fp16 fragments
 // and training scratch (csrc/mlp.hip).

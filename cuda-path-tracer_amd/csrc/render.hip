@@ -809,6 +809,19 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     r.last_train = ntrain;
 }
 
+void render_last_queries(Renderer& r, float* raw, float* pred, uint32_t* valid, hipStream_t st) {
+    if (r.last_train == 0 && r.cache_io.capacity() == 0) throw Error(PT_E_INVALID_ARG, "no cache frame rendered");
+    const size_t npix = static_cast<size_t>(r.w) * r.h;
+    auto bytes = [](size_t k) { return (k * 4 + 255) & ~size_t(255); };
+    const char* base = r.cache_io.as<char>();
+    if (raw) PT_HIP(hipMemcpyAsync(raw, base, npix * 48, hipMemcpyDeviceToHost, st));
+    base += bytes(npix * 12);
+    if (pred) PT_HIP(hipMemcpyAsync(pred, base, npix * 12, hipMemcpyDeviceToHost, st));
+    base += bytes(npix * 3);
+    if (valid) PT_HIP(hipMemcpyAsync(valid, base, npix * 4, hipMemcpyDeviceToHost, st));
+    PT_HIP(hipStreamSynchronize(st));
+}
+
 void render_last_batch(Renderer& r, float* raw, float* target, uint32_t n, hipStream_t st) {
     if (n > r.last_train) throw Error(PT_E_INVALID_ARG, "more samples than the last cache frame's batch");
     if (n == 0) return;

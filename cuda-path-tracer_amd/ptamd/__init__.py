@@ -115,6 +115,7 @@ _SIGS = {
     "pt_cache_gradient_device": [_vp, _vp, _vp, _u32, _vp, _vp],
     "pt_render_cache_frame": [_vp, _i, _i, _u64, _vp],
     "pt_cache_last_batch": [_vp, _vp, _vp, _u32],
+    "pt_cache_last_queries": [_vp, _vp, _vp, _vp],
     "pt_tune_set": [ctypes.c_char_p, _u32],
     "pt_tune_get": [ctypes.c_char_p, _vp],
 }
@@ -487,3 +488,11 @@ class Context:
         tgt = np.zeros((n, 3), np.float32)
         self._c(_lib.pt_cache_last_batch(self._h, _ptr(raw), _ptr(tgt), n))
         return raw, tgt
+
+    def cache_last_queries(self, w, h):
+        """(raw (h*w, 12), predictions (h*w, 3), valid (h*w,) bool) of the last cache frame."""
+        raw = np.zeros((w * h, 12), np.float32)
+        pred = np.zeros((w * h, 3), np.float32)
+        valid = np.zeros(w * h, np.uint32)
+        self._c(_lib.pt_cache_last_queries(self._h, _ptr(raw), _ptr(pred), _ptr(valid)))
+        return raw, pred, valid.astype(bool)

@@ -177,7 +177,9 @@ int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed);
  * launches one pass per frame): the accumulator ends up exactly as after
  * npasses calls of pt_render_pass with the same corner rays and seeds.
  * dirs = 12 floats per pass (NULL: the corner rays of pt_set_primary_rays for
- * every pass), seeds[npasses]; 1 <= npasses <= 1024.  scenes (may be NULL =
+ * every pass), seeds[npasses]; 1 <= npasses <= 1024.  Every pass starts at
+ * the eye given to pt_set_primary_rays (set it before the first call).
+ * scenes (may be NULL =
  * all scene 0) picks, per pass, the scene of a pt_upload_scenes batch that
  * the pass traces; all passes add into the one accumulator. */
 int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const float* dirs,
@@ -240,6 +242,10 @@ int pt_render_cache_frame(pt_context* ctx, int w, int h, uint64_t pass_seed,
 /* The first n (raw, target) samples of the last cache frame's training batch:
  * raw 12 floats, target 3 floats (NaN: the path never reached x2). */
 int pt_cache_last_batch(pt_context* ctx, float* raw, float* target, uint32_t n);
+/* The last cache frame's pixel queries (raw 12 floats per pixel), the
+ * cache's predictions (3 per pixel) and whether the pixel's path reached x2
+ * (valid[w*h]); any pointer may be NULL.  For inspecting the cache. */
+int pt_cache_last_queries(pt_context* ctx, float* raw, float* pred, uint32_t* valid);
 
 /* ---- launch tuning (no reference counterpart) -------------------------------
  * Process-wide launch-shape knobs read by the kernels' host launchers; results

@@ -77,6 +77,9 @@ def test_cache_frame_matches_oracle(ctx, room, rr_start, strat):
         say(f"[cache frame] rr_start={rr_start} frame {f}")
         ctx.render_cache_frame(W, H, seed, cs)
         raw, tgt = ctx.cache_last_batch(ntrain)
+        qraw, qpred, qvalid = ctx.cache_last_queries(W, H)
+        assert qvalid.any()
+        np.testing.assert_allclose(qpred[qvalid], ctx.cache_infer(qraw[qvalid]), rtol=0, atol=0)
         acc_o, oraw, otgt = orc.render_cache_pass(tris, bvh, infos, st, W, H, dirs_for(W, H), EYE, f, seed,
                                                   fp16_weights(p), bmin, bmax, ntrain, extra, acc_o)
         valid = ~np.isnan(otgt[:, 0])
