@@ -172,8 +172,87 @@ __device__ __forceinline__ V3 blinn_phong(V3 Id, V3 kd, V3 Is, V3 ks, V3 n, V3 l
     return add(diffuse, specular);
 }
 
+// Shading data of a hit (cuda_path_tracer.cu:190-191, 298-300): position,
+// interpolated normal, albedo, emission.
+struct HitShade {
+    V3 pos, nrm, albedo, e;
+    float spec;
+    bool emissive;
+};
+
+__device__ __forceinline__ HitShade hit_shade(const pt_triangle_info* __restrict__ infos, uint32_t id, float t,
+                                              float bu, float bv, V3 o, V3 d) {
+    const pt_triangle_info& in = infos[id];
+    HitShade hs;
+    hs.pos = add(scale(d, t), o);
+    const float bw = (1.0f - bu) - bv;
+    hs.nrm = normalize(add(add(scale(ld3(in.n0), bw), scale(ld3(in.n1), bu)), scale(ld3(in.n2), bv)));
+    hs.albedo = ld3(in.albedo);
+    hs.e = ld3(in.emission);
+    hs.spec = in.specularity;
+    hs.emissive = ((hs.e.x + hs.e.y) + hs.e.z) > 1e-7f;
+    return hs;
+}
+
+// Bounce hit (level >= 1, cuda_path_tracer.cu:298-325): emitter cosine term
+// (albedo 1 on emitters) or the directional lights' term behind shadow rays;
+// throughput *= albedo, radiance += throughput * L * intensity.  LIGHTS:
+// whether the scene has directional lights (kept out of the no-light kernels,
+// whose registers and stack it would otherwise size).
+template <bool LIGHTS>
+__device__ __forceinline__ void bounce_hit(HitShade& hs, V3 d, const FrameConst& fc,
+                                           const pt_dir_light* __restrict__ lights,
+                                           const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                           uint32_t sb, uint32_t se, V3& absorbed, V3& acc) {
+    V3 L = v3(0.0f, 0.0f, 0.0f);
+    if (hs.emissive) {
+        const float c = gmax(dot(neg(d), hs.nrm), 0.0f);
+        L = scale(hs.e, c);
+        hs.albedo = v3(1.0f, 1.0f, 1.0f);
+    } else if (LIGHTS) {
+        for (uint32_t k = 0; k < fc.nlights; ++k) {
+            const V3 ld = ld3(lights[k].direction);
+            L = add(L, scale(light_L(nodes, tris, sb, se, hs.pos, lights[k]), gmax(dot(neg(ld), hs.nrm), 0.0f)));
+        }
+    }
+    absorbed = mul(absorbed, hs.albedo);
+    acc = add(acc, scale(mul(absorbed, L), fc.intensity));
+}
+
+// Next segment of bounce b (cuda_path_tracer.cu:267-290): Russian roulette
+// from rr_start (survive with p = max albedo, throughput / p), stratified
+// first bounce, cosine-weighted direction about the shading normal.  Returns
+// whether the path continues; no/nd are the new origin and direction.
+__device__ __forceinline__ bool next_bounce(uint32_t key, uint32_t pass, uint32_t b, uint32_t seedw,
+                                            const FrameConst& fc, const HitShade& hs, V3& absorbed, V3& no,
+                                            V3& nd) {
+    const U4 rv = pcg4d(U4{key, pass, b, seedw});
+    float u1 = u24(rv.x), u2 = u24(rv.y);
+    if (b >= fc.rr_start) {
+        float p = gmax(gmax(hs.albedo.x, hs.albedo.y), hs.albedo.z);
+        p = gmin(p, 1.0f);
+        if (!(u24(rv.z) < p)) return false;
+        absorbed = v3(absorbed.x / p, absorbed.y / p, absorbed.z / p);
+    }
+    no = add(hs.pos, scale(hs.nrm, 1e-5f));
+    if (fc.stratified != 0u && b == 0) {
+        // 8x8 strata, stratum = pass % 64 (cuda_path_tracer.cu:273-280).
+        const uint32_t p1 = pass % 64u;
+        u1 = (static_cast<float>(p1 % 8u) + u1) / 8.0f;
+        u2 = (static_cast<float>(p1 / 8u) + u2) / 8.0f;
+    }
+    // Cosine-weighted hemisphere sample (random.h:95-102).
+    const float r = __builtin_sqrtf(u1);
+    float sn, cs;
+    sincos2pi(u2, sn, cs);
+    const V3 local = v3(r * cs, r * sn, __builtin_sqrtf(1.0f - u1));
+    nd = to_hemisphere(local, hs.nrm);
+    return true;
+}
+
 // CACHE: a cache frame (one pass; paths >= npix are training paths).
-template <bool CACHE>
+// LIGHTS: the scene has directional lights.
+template <bool CACHE, bool LIGHTS>
 __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                                                     const PassParams* __restrict__ pp,
                                                     const pt_triangle_info* __restrict__ infos,
@@ -203,66 +282,41 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
             const uint32_t seedw = train ? (P.seedw ^ kTrainSalt) : P.seedw;
             const V3 o = v3(qin.ox[i], qin.oy[i], qin.oz[i]);
             const V3 d = v3(qin.dx[i], qin.dy[i], qin.dz[i]);
-            const float t = hq.t[i];
-            const pt_triangle_info& in = infos[hq.id[i]];
-            // Hit point and shading data (cuda_path_tracer.cu:190-191, 298-300).
-            const V3 pos = add(scale(d, t), o);
-            const float bu = hq.u[i], bv = hq.v[i];
-            const float bw = (1.0f - bu) - bv;
-            const V3 nrm = normalize(add(add(scale(ld3(in.n0), bw), scale(ld3(in.n1), bu)),
-                                         scale(ld3(in.n2), bv)));
-            V3 albedo = ld3(in.albedo);
-            const V3 e = ld3(in.emission);
-            const bool emissive = ((e.x + e.y) + e.z) > 1e-7f;
+            HitShade hs = hit_shade(infos, hq.id[i], hq.t[i], hq.u[i], hq.v[i], o, d);
+            const uint32_t sb = off[P.scene], se = off[P.scene + 1];
             V3 absorbed, acc;
             if (level == 0) {
                 // Primary hit: emission, else Blinn-Phong from the directional
                 // lights with shadow rays, the light term L summed over the
                 // lights seen so far (cuda_path_tracer.cu:194-214, 258-259).
                 acc = v3(0.0f, 0.0f, 0.0f);
-                if (emissive) {
-                    acc = e;
-                    albedo = v3(1.0f, 1.0f, 1.0f);
-                } else if (fc.nlights) {
-                    const uint32_t sb = off[P.scene], se = off[P.scene + 1];
+                if (hs.emissive) {
+                    acc = hs.e;
+                    hs.albedo = v3(1.0f, 1.0f, 1.0f);
+                } else if (LIGHTS) {
                     V3 Lc = v3(0.0f, 0.0f, 0.0f);
                     for (uint32_t k = 0; k < fc.nlights; ++k) {
                         const V3 ld = ld3(lights[k].direction);
-                        Lc = add(Lc, light_L(nodes, tris, sb, se, pos, lights[k]));
+                        Lc = add(Lc, light_L(nodes, tris, sb, se, hs.pos, lights[k]));
                         const V3 hw = scale(sub(neg(d), ld), 0.5f);
-                        acc = add(acc, blinn_phong(Lc, albedo, Lc, v3(0.5f, 0.5f, 0.5f), nrm, neg(ld), hw,
-                                                   in.specularity));
+                        acc = add(acc, blinn_phong(Lc, hs.albedo, Lc, v3(0.5f, 0.5f, 0.5f), hs.nrm, neg(ld), hw,
+                                                   hs.spec));
                     }
                 }
-                absorbed = albedo;
+                absorbed = hs.albedo;
             } else {
                 absorbed = v3(ps.ar[path], ps.ag[path], ps.ab[path]);
                 acc = v3(ps.cr[path], ps.cg[path], ps.cb[path]);
-                V3 L = v3(0.0f, 0.0f, 0.0f);
-                if (emissive) {
-                    // Emitter cosine (cuda_path_tracer.cu:304-312).
-                    const float c = gmax(dot(neg(d), nrm), 0.0f);
-                    L = scale(e, c);
-                    albedo = v3(1.0f, 1.0f, 1.0f);
-                } else if (fc.nlights) {
-                    // Shadow rays to the lights with the receiver cosine (:314-321).
-                    const uint32_t sb = off[P.scene], se = off[P.scene + 1];
-                    for (uint32_t k = 0; k < fc.nlights; ++k) {
-                        const V3 ld = ld3(lights[k].direction);
-                        L = add(L, scale(light_L(nodes, tris, sb, se, pos, lights[k]), gmax(dot(neg(ld), nrm), 0.0f)));
-                    }
-                }
-                absorbed = mul(absorbed, albedo);
-                acc = add(acc, scale(mul(absorbed, L), fc.intensity));
+                bounce_hit<LIGHTS>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
             }
             if (CACHE && level == 1) {
                 // x2: a pixel path's cache query, or a training sample whose
                 // suffix radiance is gathered from here with throughput 1.
                 float* f = train ? io.traw + 12 * static_cast<size_t>(pix) : io.qraw + 12 * static_cast<size_t>(path);
-                f[0] = pos.x; f[1] = pos.y; f[2] = pos.z;
-                f[3] = nrm.x; f[4] = nrm.y; f[5] = nrm.z;
-                f[6] = d.x;   f[7] = d.y;   f[8] = d.z;
-                f[9] = albedo.x; f[10] = albedo.y; f[11] = albedo.z;
+                f[0] = hs.pos.x; f[1] = hs.pos.y; f[2] = hs.pos.z;
+                f[3] = hs.nrm.x; f[4] = hs.nrm.y; f[5] = hs.nrm.z;
+                f[6] = d.x;      f[7] = d.y;      f[8] = d.z;
+                f[9] = hs.albedo.x; f[10] = hs.albedo.y; f[11] = hs.albedo.z;
                 if (train) {
                     io.tvalid[pix] = 1u;
                     absorbed = v3(1.0f, 1.0f, 1.0f);
@@ -276,35 +330,7 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
             // train_levels - 1 bounces.
             const uint32_t b = level;
             const uint32_t limit = CACHE ? (train ? io.train_levels - 1u : 1u) : fc.bounces;
-            if (b < limit) {
-                const U4 rv = pcg4d(U4{pix, P.pass, b, seedw});
-                float u1 = u24(rv.x), u2 = u24(rv.y);
-                cont = true;
-                if (b >= fc.rr_start) {
-                    float p = gmax(gmax(albedo.x, albedo.y), albedo.z);
-                    p = gmin(p, 1.0f);
-                    if (!(u24(rv.z) < p)) {
-                        cont = false;
-                    } else {
-                        absorbed = v3(absorbed.x / p, absorbed.y / p, absorbed.z / p);
-                    }
-                }
-                if (cont) {
-                    no = add(pos, scale(nrm, 1e-5f));
-                    if (fc.stratified != 0u && b == 0) {
-                        // 8x8 strata, stratum = pass % 64 (cuda_path_tracer.cu:273-280).
-                        const uint32_t p1 = P.pass % 64u;
-                        u1 = (static_cast<float>(p1 % 8u) + u1) / 8.0f;
-                        u2 = (static_cast<float>(p1 / 8u) + u2) / 8.0f;
-                    }
-                    // Cosine-weighted hemisphere sample (random.h:95-102).
-                    const float r = __builtin_sqrtf(u1);
-                    float sn, cs;
-                    sincos2pi(u2, sn, cs);
-                    const V3 local = v3(r * cs, r * sn, __builtin_sqrtf(1.0f - u1));
-                    nd = to_hemisphere(local, nrm);
-                }
-            }
+            if (b < limit) cont = next_bounce(pix, P.pass, b, seedw, fc, hs, absorbed, no, nd);
             ps.ar[path] = absorbed.x;
             ps.ag[path] = absorbed.y;
             ps.ab[path] = absorbed.z;
@@ -329,6 +355,58 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                 qout.path[s] = path;
             }
         }
+    }
+}
+
+// Cache frames: the training paths past level `level0` (only training paths
+// are left there), one lane per path to its end -- the per-level launches
+// of a few thousand rays are latency bound (each level waits for its slowest
+// ray), one lane per path waits only for the slowest path.  Traversal,
+// shading and sampling are k_extend / k_shade's, so the results are the same
+// bits; counters[L] still count the rays of level L.
+constexpr int kSuffixThreads = 64;
+
+__global__ __launch_bounds__(kSuffixThreads) void k_train_suffix(const FrameConst fc,
+                                                                 const PassParams* __restrict__ pp,
+                                                                 const pt_triangle_info* __restrict__ infos,
+                                                                 RayQueue q, PathState ps,
+                                                                 uint32_t* __restrict__ counters, uint32_t level0,
+                                                                 CacheIO io, const pt_dir_light* __restrict__ lights,
+                                                                 const float4* __restrict__ nodes,
+                                                                 const float4* __restrict__ tris,
+                                                                 const uint32_t* __restrict__ off) {
+    const uint32_t count = counters[level0];
+    const PassParams& P = pp[0];
+    const uint32_t seedw = P.seedw ^ kTrainSalt;
+    const uint32_t sb = off[P.scene], se = off[P.scene + 1];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t path = q.path[i];
+        const uint32_t key = path - fc.npix;
+        V3 o = v3(q.ox[i], q.oy[i], q.oz[i]);
+        V3 d = v3(q.dx[i], q.dy[i], q.dz[i]);
+        V3 absorbed = v3(ps.ar[path], ps.ag[path], ps.ab[path]);
+        V3 acc = v3(ps.cr[path], ps.cg[path], ps.cb[path]);
+        for (uint32_t b = level0;; ++b) {
+            const TraceHit h = trace_ray_in(nodes, tris, sb, se, o, d);
+            if (h.id == kInvalid) break;
+            HitShade hs = hit_shade(infos, h.id, h.t, h.u, h.v, o, d);
+            if (fc.nlights)
+                bounce_hit<true>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+            else
+                bounce_hit<false>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+            if (!(b < io.train_levels - 1u)) break;
+            V3 no, nd;
+            if (!next_bounce(key, P.pass, b, seedw, fc, hs, absorbed, no, nd)) break;
+            atomicAdd(counters + b + 1, 1u);
+            o = no;
+            d = nd;
+        }
+        ps.ar[path] = absorbed.x;
+        ps.ag[path] = absorbed.y;
+        ps.ab[path] = absorbed.z;
+        ps.cr[path] = acc.x;
+        ps.cg[path] = acc.y;
+        ps.cb[path] = acc.z;
     }
 }
 
@@ -514,12 +592,13 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
     const uint32_t grid = grid_for(npaths);
     const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
+    auto shade = fc.nlights ? k_shade<false, true> : k_shade<false, false>;
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt, nlev);
     for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
         hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
                            scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
-        hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(kThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, CacheIO{}, scene.lights.as<pt_dir_light>(),
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
@@ -688,16 +767,26 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     const uint32_t grid = grid_for(npaths);
     const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
+    auto shade = fc.nlights ? k_shade<true, true> : k_shade<true, false>;
     hipLaunchKernelGGL(k_generate_cache, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt,
                        nlev, io);
-    for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
+    // Levels 0 and 1 hold every path; past x2 only training paths are left,
+    // traced to their ends by k_train_suffix.
+    for (uint32_t lvl = 0; lvl < 2; ++lvl) {
         hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
                            scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
-        hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(kThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
+    }
+    if (nlev > 2) {
+        const uint32_t sgrid = std::max(1u, std::min(4096u, (io.ntrain + kSuffixThreads - 1) / kSuffixThreads));
+        hipLaunchKernelGGL(k_train_suffix, dim3(sgrid), dim3(kSuffixThreads), 0, st, fc, pp,
+                           scene.infos.as<pt_triangle_info>(), L.q[0], L.ps, cnt, 2u, io,
+                           scene.lights.as<pt_dir_light>(), scene.bvh.nodes.as<float4>(),
+                           scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
     }
     cache.infer(io.qraw, fc.npix, io.qout, st);
     hipLaunchKernelGGL(k_cache_resolve, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, io, L.ps);
