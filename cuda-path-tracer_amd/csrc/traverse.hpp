@@ -60,19 +60,23 @@ __device__ __forceinline__ TraceHit traverse_impl(const float4* __restrict__ nod
             float h0, h1, u0 = 0.0f, v0 = 0.0f, u1 = 0.0f, v1 = 0.0f;
             uint32_t id0 = kInvalid, id1 = kInvalid;
             if (COUNT) cnt.iters++;
-            if (in0) {
-                if (COUNT) cnt.boxes++;
-                h0 = child_box<COUNT, FINITE>(o, dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
-            } else {
+            // Both child boxes for every lane (a leaf child's value is
+            // discarded), then the leaf children's triangle tests in one loop.
+            h0 = child_box<COUNT, FINITE>(o, dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+            h1 = child_box<COUNT, FINITE>(o, dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
+            if (COUNT) cnt.boxes += static_cast<uint32_t>(in0) + static_cast<uint32_t>(in1);
+            const uint32_t nleaf = static_cast<uint32_t>(!in0) + static_cast<uint32_t>(!in1);
+            for (uint32_t j = 0; j < nleaf; ++j) {
+                const bool first = j == 0 && !in0;
+                float u, v;
+                uint32_t id;
+                const float h = tri_hit(tris, first ? l : l + 1, o, d, u, v, id);
                 if (COUNT) cnt.tris++;
-                h0 = tri_hit(tris, l, o, d, u0, v0, id0);
-            }
-            if (in1) {
-                if (COUNT) cnt.boxes++;
-                h1 = child_box<COUNT, FINITE>(o, dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
-            } else {
-                if (COUNT) cnt.tris++;
-                h1 = tri_hit(tris, l + 1, o, d, u1, v1, id1);
+                if (first) {
+                    h0 = h; u0 = u; v0 = v; id0 = id;
+                } else {
+                    h1 = h; u1 = u; v1 = v; id1 = id;
+                }
             }
             if (!(h0 <= minT) && !(h1 <= minT)) break;
             const bool sw = h1 < h0;  // left child wins ties (trace.h:125)
@@ -196,21 +200,30 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
     float h0, h1, u0 = 0.0f, v0 = 0.0f, u1 = 0.0f, v1 = 0.0f;
     uint32_t id0 = kInvalid, id1 = kInvalid;
     if (COUNT) cnt.iters++;
-    if (in0) {
-        if (COUNT) cnt.boxes++;
-        h0 = r.finite ? slab_test_finite(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y)
-                      : slab_test(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+    // Both child boxes in one code path for every lane (a leaf child's slot
+    // holds no box; its value is discarded), then the triangle tests of the
+    // leaf children, first leaf then second, in one loop body, so lanes at
+    // different depths share the instructions.
+    if (r.finite) {
+        h0 = slab_test_finite(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+        h1 = slab_test_finite(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
     } else {
-        if (COUNT) cnt.tris++;
-        h0 = tri_hit(tris, l, r.o, r.d, u0, v0, id0);
+        h0 = slab_test(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+        h1 = slab_test(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
     }
-    if (in1) {
-        if (COUNT) cnt.boxes++;
-        h1 = r.finite ? slab_test_finite(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w)
-                      : slab_test(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
-    } else {
+    if (COUNT) cnt.boxes += static_cast<uint32_t>(in0) + static_cast<uint32_t>(in1);
+    const uint32_t nleaf = static_cast<uint32_t>(!in0) + static_cast<uint32_t>(!in1);
+    for (uint32_t j = 0; j < nleaf; ++j) {
+        const bool first = j == 0 && !in0;  // this test is child 0's
+        float u, v;
+        uint32_t id;
+        const float h = tri_hit(tris, first ? l : l + 1, r.o, r.d, u, v, id);
         if (COUNT) cnt.tris++;
-        h1 = tri_hit(tris, l + 1, r.o, r.d, u1, v1, id1);
+        if (first) {
+            h0 = h; u0 = u; v0 = v; id0 = id;
+        } else {
+            h1 = h; u1 = u; v1 = v; id1 = id;
+        }
     }
     if (h0 <= r.minT || h1 <= r.minT) {
         const bool sw = h1 < h0;  // left child wins ties (trace.h:125)
