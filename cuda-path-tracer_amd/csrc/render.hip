@@ -154,12 +154,12 @@ __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]);
 
 // Light reaching `pos` from a directional light (computeL, LightAlgorithms.h:
 // 12-20): a shadow ray from pos - dir * 1e-3 towards the light; any hit
-// (closest-hit t < FLT_MAX, trace.h:45-50) shadows it.
+// (closest-hit t < FLT_MAX, trace.h:45-50) shadows it.  The any-hit search
+// answers the same question and stops at its first hit.
 __device__ __forceinline__ V3 light_L(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                       uint32_t b, uint32_t e, V3 pos, const pt_dir_light& lt) {
     const V3 ld = ld3(lt.direction);
-    const TraceHit h = trace_ray_in(nodes, tris, b, e, sub(pos, scale(ld, 1e-3f)), neg(ld));
-    if (h.t < kFltMax) return v3(0.0f, 0.0f, 0.0f);
+    if (any_hit_in(nodes, tris, b, e, sub(pos, scale(ld, 1e-3f)), neg(ld))) return v3(0.0f, 0.0f, 0.0f);
     return scale(ld3(lt.color), lt.intensity);
 }
 

@@ -165,6 +165,46 @@ __device__ __forceinline__ TraceHit trace_ray_in(const float4* __restrict__ node
     return traverse_impl<false, false>(nodes, tris, o, d, dinv, cnt, b);
 }
 
+// Whether a ray hits any triangle of the scene in [b, e) that the closest-hit
+// search would accept (the shadow-ray question of computeL,
+// LightAlgorithms.h:12-20: closest-hit t < FLT_MAX).  Until its first hit
+// the closest-hit search culls nothing (its bound is still the initial
+// one), so it visits the same boxes as this search does and finds a hit
+// exactly when this one does; this one stops at the first.
+__device__ __forceinline__ bool any_hit_in(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          uint32_t b, uint32_t e, V3 o, V3 d) {
+    float u, v;
+    uint32_t id;
+    if (e - b == 0) return false;
+    if (e - b == 1) return tri_hit(tris, b, o, d, u, v, id) <= kMinTInit;
+    const V3 dinv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    uint32_t stack[kStackMax];
+    int sp = 0;
+    uint32_t node = b;
+    for (;;) {
+        const float4* rec = nodes + 4 * static_cast<size_t>(node);
+        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+        const uint32_t w = __float_as_uint(r3.x);
+        const uint32_t l = w >> 2;
+        const bool in0 = (w & 1u) != 0, in1 = (w & 2u) != 0;
+        const float h0 = in0 ? slab_test(o, dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y)
+                             : tri_hit(tris, l, o, d, u, v, id);
+        const float h1 = in1 ? slab_test(o, dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w)
+                             : tri_hit(tris, l + 1, o, d, u, v, id);
+        if ((!in0 && h0 <= kMinTInit) || (!in1 && h1 <= kMinTInit)) return true;
+        const bool go0 = in0 && h0 <= kMinTInit, go1 = in1 && h1 <= kMinTInit;
+        if (go0 && go1) {
+            if (sp < kStackMax) stack[sp++] = l + 1;
+            node = l;
+        } else if (go0 || go1) {
+            node = go0 ? l : l + 1;
+        } else {
+            if (sp == 0) return false;
+            node = stack[--sp];
+        }
+    }
+}
+
 // ---- persistent traversal -------------------------------------------------
 //
 // One lane = one ray at a time, advanced one node visit per loop iteration;

@@ -116,6 +116,8 @@ _SIGS = {
     "pt_render_cache_frame": [_vp, _i, _i, _u64, _vp],
     "pt_cache_last_batch": [_vp, _vp, _vp, _u32],
     "pt_cache_last_queries": [_vp, _vp, _vp, _vp],
+    "pt_load_obj": [ctypes.c_char_p, _vp, _vp, _vp, _vp],
+    "pt_asset_error": [ctypes.c_char_p, ctypes.c_size_t],
     "pt_tune_set": [ctypes.c_char_p, _u32],
     "pt_tune_get": [ctypes.c_char_p, _vp],
 }
@@ -227,6 +229,32 @@ def pass_jitter(seed, pass_index, w, h):
     jy = np.zeros(1, np.float32)
     _check(_lib.pt_pass_jitter(seed, pass_index, w, h, _ptr(jx), _ptr(jy)))
     return float(jx[0]), float(jy[0])
+
+
+class ObjOptions(ctypes.Structure):
+    """pt_obj_options: left_handed (the reference's import flags: z mirrored,
+    winding reversed), normalize_albedo (albedo = Kd / |Kd|)."""
+    _fields_ = [("left_handed", ctypes.c_uint32), ("normalize_albedo", ctypes.c_uint32)]
+
+
+def load_obj(path, left_handed=True, normalize_albedo=True):
+    """OBJ/MTL file -> (tris, infos) in file order, ids 0..n-1."""
+    opt = ObjOptions(1 if left_handed else 0, 1 if normalize_albedo else 0)
+    p = os.fsencode(path)
+    n = np.zeros(1, np.uint32)
+    rc = _lib.pt_load_obj(p, ctypes.byref(opt), None, None, _ptr(n))
+    if rc != PT_OK:
+        buf = ctypes.create_string_buffer(512)
+        _lib.pt_asset_error(buf, len(buf))
+        raise PtError(rc, buf.value.decode(errors="replace"))
+    tris = np.zeros(int(n[0]), TRI_DTYPE)
+    infos = np.zeros(int(n[0]), INFO_DTYPE)
+    rc = _lib.pt_load_obj(p, ctypes.byref(opt), _ptr(tris), _ptr(infos), _ptr(n))
+    if rc != PT_OK:
+        buf = ctypes.create_string_buffer(512)
+        _lib.pt_asset_error(buf, len(buf))
+        raise PtError(rc, buf.value.decode(errors="replace"))
+    return tris, infos
 
 
 def tune_set(name, value):
