@@ -425,6 +425,33 @@ def main():
                          f"OpenMP {threads} threads"}
         log(f"[bench] CPU oracle: {cpu['value']} Mrays/s on {threads} threads ({m} rays, {dt:.1f}s)")
 
+        # The other metrics' CPU restatements on the same host (bounded runs):
+        # the LBVH build of the 2^20-triangle soup and the two render legs.
+        soup, _ = ptamd.gen_soup(1 << 20, 1, emitter=False)
+        sb0, sb1 = ptamd.scene_bounds(soup)
+        t1 = time.time()
+        orc.build(soup, sb0, sb1)
+        cpu_build_ms = (time.time() - t1) * 1e3
+        cpu_render = {}
+        for name, (rt, ri), st, (w, h), passes, eye, tgt in (
+            ("c0_soup_256x256_b4", ptamd.gen_soup(65536, 0), ptamd.Settings(4, ptamd.INVALID, 1.0, 1), (256, 256),
+             4, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)),
+            ("c1_room_1024x1024_b8_rr3", ptamd.gen_sphere(660, seed_mesh, room=True), ptamd.Settings(8, 3, 1.0, 1),
+             (1024, 1024), 1, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))):
+            ob = orc.build(rt, *ptamd.scene_bounds(rt))
+            acc = np.zeros((h, w, 3))
+            t1 = time.time()
+            for p in range(passes):
+                d = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, *ptamd.pass_jitter(5, p, w, h))
+                orc.render_pass(rt, ob, ri, st, w, h, d, eye, p, 1000 + p, acc, threads=threads)
+            dt = time.time() - t1
+            cpu_render[name] = {"ms_per_pass": round(dt * 1e3 / passes, 1), "spp_per_s": round(passes / dt, 3),
+                                "Msamples_per_s": round(w * h * passes / dt / 1e6, 3), "passes": passes}
+        cpu["other_metrics"] = {"lbvh_build_ms_soup_2^20": round(cpu_build_ms, 1), "render": cpu_render,
+                                "threads": threads}
+        log(f"[bench] CPU oracle: LBVH build {cpu_build_ms:.0f} ms @ 2^20; render "
+            + ", ".join(f"{k} {v['ms_per_pass']} ms/pass" for k, v in cpu_render.items()))
+
     # HBM bytes per launch of the same kernel from the committed rocprofv3
     # FETCH_SIZE/WRITE_SIZE runs (calibrated against a 256 MiB copy).
     traffic = None
