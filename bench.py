@@ -313,6 +313,15 @@ def main():
     hits = d_id.to_host()
     hit_rate = float((hits != ptamd.INVALID).mean())
     log(f"[bench] {mrays:.1f} Mrays/s ({ms_per_step:.3f} ms/launch, wall {wall:.2f}s), hit rate {hit_rate:.3f}")
+    # The same launch through the host-buffer entry point (pageable copies
+    # over PCIe both ways included): not the metric, recorded beside it.
+    host_mrays = None
+    if not args.profile:
+        ctx.trace_closest(rays[: 1 << 20])  # staging buffers
+        t1 = time.time()
+        ctx.trace_closest(rays)
+        host_mrays = round(n / (time.time() - t1) / 1e6, 1)
+        log(f"[bench] host-buffer entry point (PCIe copies included): {host_mrays} Mrays/s")
 
     # LBVH build time (HIP events around the build kernels).
     build_ms_mesh = float(np.median([ctx.build_bvh() for _ in range(5)]))
@@ -346,9 +355,12 @@ def main():
         r1.record(stream)
         r1.synchronize()
         rms = r0.elapsed_ms(r1)
+        frame_rays = ctx.last_frame_rays()  # path segments of the last frame
         render[name] = {"ms_per_pass": round(rms / passes, 3), "spp_per_s": round(passes / (rms * 1e-3), 2),
                         "Msamples_per_s": round(w * h * passes / (rms * 1e-3) / 1e6, 2), "passes": passes,
-                        "passes_per_frame": per_frame}
+                        "passes_per_frame": per_frame,
+                        "segments_per_pass": int(frame_rays // per_frame),
+                        "Msegments_per_s": round(frame_rays / per_frame * passes / (rms * 1e-3) / 1e6, 1)}
         log(f"[bench] render {name}: {rms / passes:.3f} ms/pass ({per_frame} passes/frame)")
 
     # Config 3: 512 soups (log-uniform 1,024..262,144 triangles), one batched
@@ -485,6 +497,7 @@ def main():
         "render": render,
         "batched_scenes": batched,
         "radiance_cache": cache,
+        "host_buffers_Mrays_per_s": host_mrays,
         "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),
                       "tri_tests_per_ray": round(tri_tests / n, 3),
                       "algorithmic_bytes_per_ray": round(bytes_per_launch / n, 1), "hit_rate": round(hit_rate, 4)},
