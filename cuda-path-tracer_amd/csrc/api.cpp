@@ -78,10 +78,7 @@ int pt_create(pt_context** out, int device) {
 int pt_destroy(pt_context* ctx) {
     if (!ctx) return PT_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    delete ctx;  // releases device buffers
+    delete ctx;  // waits for its work, releases events, stream and buffers
     return PT_OK;
 }
 

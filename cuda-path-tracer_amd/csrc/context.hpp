@@ -263,4 +263,15 @@ struct pt_context {
     pt::RadianceCache cache;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pt::DevBuf io_rays, io_id, io_t, io_uv;  // staging for host-pointer traces
+    pt_context() = default;
+    pt_context(const pt_context&) = delete;
+    pt_context& operator=(const pt_context&) = delete;
+    // Waits for the context's work, then releases its events and own stream
+    // (device buffers are released by their members afterwards).
+    ~pt_context() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
 };
