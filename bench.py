@@ -278,7 +278,7 @@ def main():
     if not args.no_tune:
         from ptamd import autotune
 
-        best, table = autotune.tune(step, stream, autotune.trace_space(), reps=2)
+        best, table = autotune.tune_staged(step, stream, autotune.trace_stages(), reps=2)
         tuning = {"chosen": best, "candidates": len(table),
                   "best_ms": round(min(ms for _, ms in table), 4),
                   "worst_ms": round(max(ms for _, ms in table), 4)}

@@ -423,14 +423,23 @@ uint32_t* tune_slot(const char* name) {
     const std::string k(name);
     if (k == "trace.blocks") return &pt::trace_tuning().blocks;
     if (k == "trace.refill") return &pt::trace_tuning().refill;
+    if (k == "trace.lds_stack") return &pt::trace_tuning().lds;
+    if (k == "trace.pool") return &pt::trace_tuning().pool;
     if (k == "render.graphs") return &pt::render_tuning().graphs;
     return nullptr;
+}
+
+bool tune_valid(const uint32_t* slot, uint32_t v) {
+    if (slot == &pt::render_tuning().graphs) return true;
+    if (slot == &pt::trace_tuning().lds) return v == 8 || v == 16 || v == 32;
+    if (slot == &pt::trace_tuning().pool) return v >= 64 && v <= 4096;
+    return v != 0;
 }
 }  // namespace
 
 int pt_tune_set(const char* name, uint32_t value) {
     uint32_t* slot = tune_slot(name);
-    if (!slot || (value == 0 && slot != &pt::render_tuning().graphs)) return PT_E_INVALID_ARG;
+    if (!slot || !tune_valid(slot, value)) return PT_E_INVALID_ARG;
     *slot = value;
     return PT_OK;
 }

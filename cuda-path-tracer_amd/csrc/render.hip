@@ -120,6 +120,7 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
 // Closest hit for every ray of queue `level` (count on the device): the
 // persistent traversal, rays handed out from ray_counters[level].
 // Each path traces the scene of its pass (root internal node off[scene]).
+template <int LDS>
 __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
                                                             const PassParams* __restrict__ pp,
                                                             const float4* __restrict__ nodes,
@@ -128,8 +129,8 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
                                                             RayQueue q, HitQueue hq,
                                                             const uint32_t* __restrict__ counters,
                                                             uint32_t* __restrict__ ray_counters,
-                                                            uint32_t level, uint32_t refill) {
-    __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
+                                                            uint32_t level, uint32_t refill, uint32_t pool) {
+    __shared__ uint32_t s_stack[LDS][kPersistThreads];
     const uint32_t count = counters[level];
     TraceCount cnt{0, 0, 0};
     auto load = [&](uint32_t i, V3& o, V3& d, uint32_t& single) -> uint32_t {
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
         hq.u[i] = h.u;
         hq.v[i] = h.v;
     };
-    persistent_trace<false>(nodes, tris, count, ray_counters + level, refill, s_stack, load, store, cnt);
+    persistent_trace<false, LDS>(nodes, tris, count, ray_counters + level, refill, pool, s_stack, load, store, cnt);
 }
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
@@ -577,7 +578,7 @@ struct GraphKey {
     const void* accum;
     const void* counters;
     const void* lights;
-    uint32_t blocks, refill;
+    uint32_t blocks, refill, lds, pool;
     hipStream_t stream;
 };
 
@@ -592,12 +593,14 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
     const uint32_t grid = grid_for(npaths);
     const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
+    const uint32_t pool = pool_for(tune.pool, static_cast<uint32_t>(std::min<size_t>(npaths, 0xffffffffu)), pgrid);
+    auto extend = tune.lds <= 8 ? k_extend<8> : (tune.lds <= 16 ? k_extend<16> : k_extend<32>);
     auto shade = fc.nlights ? k_shade<false, true> : k_shade<false, false>;
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt, nlev);
     for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
-        hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
-                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
+                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill, pool);
         hipLaunchKernelGGL(shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, CacheIO{}, scene.lights.as<pt_dir_light>(),
@@ -736,6 +739,8 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         key.lights = scene.lights.as<void>();
         key.blocks = trace_tuning().blocks;
         key.refill = trace_tuning().refill;
+        key.lds = trace_tuning().lds;
+        key.pool = trace_tuning().pool;
         key.stream = st;
         replay(r.graph_exec, r.graph_key, sizeof r.graph_key, key, st,
                [&] { launch_frame(r, scene, fc, st); });
@@ -767,15 +772,17 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     const uint32_t grid = grid_for(npaths);
     const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
+    const uint32_t pool = pool_for(tune.pool, static_cast<uint32_t>(std::min<size_t>(npaths, 0xffffffffu)), pgrid);
+    auto extend = tune.lds <= 8 ? k_extend<8> : (tune.lds <= 16 ? k_extend<16> : k_extend<32>);
     auto shade = fc.nlights ? k_shade<true, true> : k_shade<true, false>;
     hipLaunchKernelGGL(k_generate_cache, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt,
                        nlev, io);
     // Levels 0 and 1 hold every path; past x2 only training paths are left,
     // traced to their ends by k_train_suffix.
     for (uint32_t lvl = 0; lvl < 2; ++lvl) {
-        hipLaunchKernelGGL(k_extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
-                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill);
+                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill, pool);
         hipLaunchKernelGGL(shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
@@ -808,7 +815,7 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
 struct CacheGraphKey {
     FrameConst fc;
     CacheIO io;
-    uint32_t steps, train, blocks, refill;
+    uint32_t steps, train, blocks, refill, lds, pool;
     const void* ptrs[13];
     float box[6], adam[4];
     hipStream_t stream;
@@ -875,6 +882,8 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
         key.train = cf.train;
         key.blocks = trace_tuning().blocks;
         key.refill = trace_tuning().refill;
+        key.lds = trace_tuning().lds;
+        key.pool = trace_tuning().pool;
         const void* ptrs[13] = {scene.bvh.nodes.as<void>(), scene.bvh.tris.as<void>(), scene.infos.as<void>(),
                                 scene.off.as<void>(),       r.queues.as<void>(),     r.params.as<void>(),
                                 r.accum.as<void>(),         r.counters.as<void>(),   cache.master.as<void>(),

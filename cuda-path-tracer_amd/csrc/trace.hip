@@ -46,16 +46,16 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
 }
 
 // Persistent closest-hit kernel over an AoS ray batch (ntri >= 2).
-template <bool COUNT>
+template <bool COUNT, int LDS>
 __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
     const float4* __restrict__ nodes, const float4* __restrict__ tris, const float* __restrict__ rays,
     uint32_t nrays, uint32_t* __restrict__ out_id, float* __restrict__ out_t,
-    float2* __restrict__ out_uv, uint32_t* __restrict__ ray_counter, uint32_t refill,
+    float2* __restrict__ out_uv, uint32_t* __restrict__ ray_counter, uint32_t refill, uint32_t pool,
     unsigned long long* __restrict__ counters) {
-    __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
+    __shared__ uint32_t s_stack[LDS][kPersistThreads];
     TraceCount cnt{0, 0, 0};
-    persistent_trace<COUNT>(
-        nodes, tris, nrays, ray_counter, refill, s_stack,
+    persistent_trace<COUNT, LDS>(
+        nodes, tris, nrays, ray_counter, refill, pool, s_stack,
         [&](uint32_t i, V3& o, V3& d, uint32_t&) -> uint32_t {
             const float* rp = rays + 6 * static_cast<size_t>(i);
             o = v3(rp[0], rp[1], rp[2]);
@@ -85,15 +85,16 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
 
 // Persistent closest hit over rays tagged with a scene of a batch: the ray
 // starts at its scene's root (internal node off[s]).
+template <int LDS>
 __global__ __launch_bounds__(kPersistThreads) void k_trace_scenes(
     const float4* __restrict__ nodes, const float4* __restrict__ tris, const uint32_t* __restrict__ off,
     uint32_t nscenes, const float* __restrict__ rays, const uint32_t* __restrict__ scene, uint32_t nrays,
     uint32_t* __restrict__ out_id, float* __restrict__ out_t, float2* __restrict__ out_uv,
-    uint32_t* __restrict__ ray_counter, uint32_t refill) {
-    __shared__ uint32_t s_stack[kLdsStack][kPersistThreads];
+    uint32_t* __restrict__ ray_counter, uint32_t refill, uint32_t pool) {
+    __shared__ uint32_t s_stack[LDS][kPersistThreads];
     TraceCount cnt{0, 0, 0};
-    persistent_trace<false>(
-        nodes, tris, nrays, ray_counter, refill, s_stack,
+    persistent_trace<false, LDS>(
+        nodes, tris, nrays, ray_counter, refill, pool, s_stack,
         [&](uint32_t i, V3& o, V3& d, uint32_t& single) -> uint32_t {
             const float* rp = rays + 6 * static_cast<size_t>(i);
             o = v3(rp[0], rp[1], rp[2]);
@@ -133,10 +134,12 @@ void launch_trace(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_
     const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
     const uint32_t blocks =
         static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
-    hipLaunchKernelGGL(k_trace_persistent<COUNT>, dim3(blocks), dim3(kPersistThreads), 0, st,
+    auto kern = tune.lds <= 8 ? k_trace_persistent<COUNT, 8>
+                : (tune.lds <= 16 ? k_trace_persistent<COUNT, 16> : k_trace_persistent<COUNT, 32>);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kPersistThreads), 0, st,
                        s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), d_rays, n, d_id, d_t,
                        reinterpret_cast<float2*>(d_uv), s.bvh.ray_counter.as<uint32_t>(),
-                       std::min(64u, std::max(1u, tune.refill)), d_counters);
+                       std::min(64u, std::max(1u, tune.refill)), pool_for(tune.pool, n, blocks), d_counters);
     PT_HIP(hipGetLastError());
 }
 
@@ -163,10 +166,11 @@ void trace_closest_scenes(const DeviceScene& s, const float* d_rays, const uint3
     const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
     const uint32_t blocks =
         static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
-    hipLaunchKernelGGL(k_trace_scenes, dim3(blocks), dim3(kPersistThreads), 0, st,
+    auto kern = tune.lds <= 8 ? k_trace_scenes<8> : (tune.lds <= 16 ? k_trace_scenes<16> : k_trace_scenes<32>);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kPersistThreads), 0, st,
                        s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), s.off.as<uint32_t>(),
                        s.nscenes(), d_rays, d_scene, n, d_id, d_t, reinterpret_cast<float2*>(d_uv),
-                       s.bvh.ray_counter.as<uint32_t>(), std::min(64u, std::max(1u, tune.refill)));
+                       s.bvh.ray_counter.as<uint32_t>(), std::min(64u, std::max(1u, tune.refill)), pool_for(tune.pool, n, blocks));
     PT_HIP(hipGetLastError());
 }
 

@@ -6,6 +6,7 @@
 // equally near with a lower original id.
 #pragma once
 
+#include <algorithm>
 #include "devmath.hpp"
 
 namespace pt {
@@ -214,7 +215,7 @@ __device__ __forceinline__ bool any_hit_in(const float4* __restrict__ nodes, con
 // kLdsStack entries in LDS ([depth][thread] so a wave's lanes hit distinct
 // banks) and the rest in a private array (scratch), touched only by deep rays.
 
-constexpr int kLdsStack = 16;
+constexpr int kLdsStack = 16;  // default LDS depth (tuned: 8, 16 or 32 entries)
 constexpr int kPersistThreads = 256;
 
 struct RayState {
@@ -227,7 +228,7 @@ struct RayState {
     bool finite;
 };
 
-template <bool COUNT>
+template <bool COUNT, int LDS>
 __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
                                            const float4* __restrict__ tris, RayState& r,
                                            uint32_t (*s_stack)[kPersistThreads],
@@ -277,11 +278,11 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
             if (hf <= r.minT) {
                 if (jf) {
                     const int sp = r.sp;
-                    if (sp < kLdsStack) {
+                    if (sp < LDS) {
                         s_stack[sp][threadIdx.x] = cf;
                         r.sp = sp + 1;
                     } else if (sp < kStackMax) {
-                        spill[sp - kLdsStack] = cf;
+                        spill[sp - LDS] = cf;
                         r.sp = sp + 1;
                     }
                 } else if (hf < r.minT || idf < r.best) {
@@ -305,31 +306,40 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
     }
     if (r.sp == 0) return false;
     const int sp = r.sp - 1;
-    r.node = sp < kLdsStack ? s_stack[sp][threadIdx.x] : spill[sp - kLdsStack];
+    r.node = sp < LDS ? s_stack[sp][threadIdx.x] : spill[sp - LDS];
     r.sp = sp;
     return true;
 }
 
 // Persistent closest-hit loop over rays [0, nrays), handed out in per-wave
-// pools from *ray_counter (zeroed by the caller): 256 rays when there are at
-// least four per lane of the grid, else 64 so that a short queue still
+// pools from *ray_counter (zeroed by the caller): pool_large rays (tuned,
+// 256 by default) when there are at least four per lane of the grid, else 64
+// so that a short queue still
 // spreads over every wave.  Idle lanes are refilled when fewer than `refill`
 // (1..64) lanes of the wave are busy.  load(i, o, d) fetches ray i;
 // store(i, hit) writes its result.  load returns the ray's root internal
 // node, or kInvalid for a scene without one (fewer than two triangles); then
 // `single` names its one triangle (or kInvalid) and the ray is answered at
 // once without entering the loop.
-constexpr uint32_t kRayPool = 256;
 constexpr uint32_t kRayPoolSmall = 64;
 
-template <bool COUNT, class Load, class Store>
+// The tuned pool size a launch of `blocks` workgroups over at most `nmax` rays
+// may use: every wave adds one pool to *ray_counter past the last ray before
+// it stops, so nmax + (waves + 1) * pool must stay below 2^32.
+inline uint32_t pool_for(uint32_t pool, uint32_t nmax, uint32_t blocks) {
+    const uint64_t waves = static_cast<uint64_t>(blocks) * (kPersistThreads / 64) + 1;
+    const uint64_t room = (0xffffffffull - nmax) / waves;
+    return static_cast<uint32_t>(std::max<uint64_t>(kRayPoolSmall, std::min<uint64_t>(pool, room)));
+}
+
+template <bool COUNT, int LDS, class Load, class Store>
 __device__ __forceinline__ void persistent_trace(const float4* __restrict__ nodes,
                                                  const float4* __restrict__ tris, uint32_t nrays,
                                                  uint32_t* __restrict__ ray_counter,
-                                                 uint32_t refill,
+                                                 uint32_t refill, uint32_t pool_large,
                                                  uint32_t (*s_stack)[kPersistThreads], Load load,
                                                  Store store, TraceCount& cnt) {
-    uint32_t spill[kStackMax - kLdsStack];
+    uint32_t spill[kStackMax - LDS];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     RayState r;
@@ -338,7 +348,7 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
     uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
     bool drained = false;                  // wave-uniform: no rays left to hand out
     const uint32_t pool =
-        nrays / 4u >= gridDim.x * blockDim.x ? kRayPool : kRayPoolSmall;
+        nrays / 4u >= gridDim.x * blockDim.x ? pool_large : kRayPoolSmall;
     for (;;) {
         const uint64_t busy = __ballot(active);
         if (!drained && static_cast<uint32_t>(__popcll(busy)) < refill) {
@@ -392,7 +402,7 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
             }
         }
         if (__ballot(active) == 0ull) break;
-        if (active && !visit_step<COUNT>(nodes, tris, r, s_stack, spill, cnt)) {
+        if (active && !visit_step<COUNT, LDS>(nodes, tris, r, s_stack, spill, cnt)) {
             TraceHit h;
             h.id = r.best;
             h.t = r.best == kInvalid ? kFltMax : r.minT;

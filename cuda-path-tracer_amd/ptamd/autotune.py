@@ -50,8 +50,34 @@ def tune(run, stream, space, reps=3):
     return best, table
 
 
+def tune_staged(run, stream, stages, reps=3):
+    """Coordinate search: grid-search each space of `stages` in turn, the
+    knobs of earlier stages held at their best.  Returns (best, table) over
+    all stages, like tune()."""
+    best, table = {}, []
+    for space in stages:
+        b, t = tune(run, stream, space, reps)
+        best.update(b or {})
+        table.extend(t)
+    return best, table
+
+
 def trace_space(num_cus=256):
-    """Default closest-hit search space: persistent workgroups per CU and the
+    """Closest-hit launch shape: persistent workgroups per CU and the
     lane-refill threshold."""
     return {"trace.blocks": [num_cus * k for k in (2, 4, 8, 12, 16)],
             "trace.refill": [16, 32, 48, 56, 64]}
+
+
+def stack_space():
+    """Closest-hit traversal stack and queue: LDS stack entries per lane
+    (more entries, fewer spills to scratch, lower occupancy) and the rays a
+    wave takes from the queue at a time."""
+    return {"trace.lds_stack": [8, 16, 32],
+            "trace.pool": [64, 256, 1024, 4096]}
+
+
+def trace_stages(num_cus=256):
+    """The default staged closest-hit search: launch shape, then stack and
+    queue sizes."""
+    return [trace_space(num_cus), stack_space()]

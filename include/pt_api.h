@@ -252,9 +252,13 @@ int pt_cache_last_queries(pt_context* ctx, float* raw, float* pred, uint32_t* va
  * do not depend on them (tests check this).  Names:
  *   "trace.blocks"  persistent closest-hit workgroups (256 threads each)
  *   "trace.refill"  refill idle lanes when fewer than this many of 64 are busy
+ *   "trace.lds_stack" traversal stack entries per lane kept in LDS (8, 16 or
+ *                   32; deeper entries spill to scratch)
+ *   "trace.pool"    rays a wave takes from the ray queue at a time on long
+ *                   queues (64..4096)
  *   "render.graphs" 1: replay each frame's kernels as one captured hipGraph,
  *                   0: launch them one by one
- * Unknown names return PT_E_INVALID_ARG. */
+ * Unknown names and values outside a knob's range return PT_E_INVALID_ARG. */
 int pt_tune_set(const char* name, uint32_t value);
 int pt_tune_get(const char* name, uint32_t* value);
 

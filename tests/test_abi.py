@@ -52,6 +52,15 @@ def test_tuning_knobs():
         ptamd.tune_set(name, v)
     assert ptamd._lib.pt_tune_set(b"no.such.knob", 1) == ptamd.PT_E_INVALID_ARG
     assert ptamd._lib.pt_tune_set(b"trace.blocks", 0) == ptamd.PT_E_INVALID_ARG
+    for v in (8, 16, 32):
+        ptamd.tune_set("trace.lds_stack", v)
+        assert ptamd.tune_get("trace.lds_stack") == v
+    for v in (0, 4, 12, 64):
+        assert ptamd._lib.pt_tune_set(b"trace.lds_stack", v) == ptamd.PT_E_INVALID_ARG
+    ptamd.tune_set("trace.lds_stack", 16)
+    for v in (63, 4097, 0):
+        assert ptamd._lib.pt_tune_set(b"trace.pool", v) == ptamd.PT_E_INVALID_ARG
+    assert ptamd.tune_get("trace.pool") == 256
 
 
 def test_null_arguments_rejected():

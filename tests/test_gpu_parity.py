@@ -176,18 +176,20 @@ def test_trace_invariant_under_tuning(ctx):
     ctx.upload_scene(tris, infos)
     b = orc.build(tris, bmin, bmax)
     oids, ot, ouv = orc.trace(tris, b, rays)
-    saved = (ptamd.tune_get("trace.blocks"), ptamd.tune_get("trace.refill"))
+    knobs = ("trace.blocks", "trace.refill", "trace.lds_stack", "trace.pool")
+    saved = {k: ptamd.tune_get(k) for k in knobs}
     try:
-        for blocks, refill in ((1, 1), (3, 64), (64, 8), (256, 32), (100000, 48), (2048, 63)):
-            say(f"[tuning invariance] blocks={blocks} refill={refill}")
-            ptamd.tune_set("trace.blocks", blocks)
-            ptamd.tune_set("trace.refill", refill)
+        for setting in ((1, 1, 16, 256), (3, 64, 8, 4096), (64, 8, 32, 64), (256, 32, 8, 1024),
+                        (100000, 48, 32, 4096), (2048, 63, 16, 100), (32, 48, 8, 256)):
+            say(f"[tuning invariance] {dict(zip(knobs, setting))}")
+            for k, v in zip(knobs, setting):
+                ptamd.tune_set(k, v)
             ids, t, uv = ctx.trace_closest(rays)
             assert np.array_equal(ids, oids) and np.array_equal(t.view(np.uint32), ot.view(np.uint32))
             assert np.array_equal(uv.view(np.uint32), ouv.view(np.uint32))
     finally:
-        ptamd.tune_set("trace.blocks", saved[0])
-        ptamd.tune_set("trace.refill", saved[1])
+        for k, v in saved.items():
+            ptamd.tune_set(k, v)
 
 
 def test_trace_parity_c1_full_mesh(ctx):

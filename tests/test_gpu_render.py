@@ -100,19 +100,20 @@ def test_render_invariant_under_tuning(ctx, scene):
     seeds = np.arange(3, dtype=np.uint64)
     dirs = np.stack([pass_dirs(p) for p in range(3)])
     o = oracle_acc(tris, infos, st, 3, seeds)
-    saved = (ptamd.tune_get("trace.blocks"), ptamd.tune_get("trace.refill"))
+    knobs = ("trace.blocks", "trace.refill", "trace.lds_stack", "trace.pool")
+    saved = {k: ptamd.tune_get(k) for k in knobs}
     try:
-        for blocks, refill in ((1, 64), (17, 5), (4096, 40)):
-            say(f"[render tuning] blocks={blocks} refill={refill}")
-            ptamd.tune_set("trace.blocks", blocks)
-            ptamd.tune_set("trace.refill", refill)
+        for setting in ((1, 64, 8, 256), (17, 5, 32, 64), (4096, 40, 16, 4096)):
+            say(f"[render tuning] {dict(zip(knobs, setting))}")
+            for k, v in zip(knobs, setting):
+                ptamd.tune_set(k, v)
             ctx.reset_frames()
             ctx.render_passes(W, H, seeds, dirs)
             acc, spp = ctx.read_accumulator(W, H)
             assert spp == 3 and np.array_equal(acc, o)
     finally:
-        ptamd.tune_set("trace.blocks", saved[0])
-        ptamd.tune_set("trace.refill", saved[1])
+        for k, v in saved.items():
+            ptamd.tune_set(k, v)
 
 
 def test_batched_scene_frame(ctx):
