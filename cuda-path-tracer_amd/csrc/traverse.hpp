@@ -34,9 +34,12 @@ __device__ __forceinline__ float child_box(V3 o, V3 dinv, float a, float b, floa
 
 __device__ __forceinline__ float tri_hit(const float4* __restrict__ tris, uint32_t k, V3 o, V3 d,
                                          float& u, float& v, uint32_t& id) {
-    const float4 a = tris[3 * static_cast<size_t>(k)];
+    float4 a = tris[3 * static_cast<size_t>(k)];
     const float4 b = tris[3 * static_cast<size_t>(k) + 1];
     const float4 c = tris[3 * static_cast<size_t>(k) + 2];
+    // Keep p0's load beside the edges' (the compiler would otherwise sink it
+    // below the determinant test, a second memory round trip).
+    asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z));
     id = __float_as_uint(a.w);
     return mt_test(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), u, v);
 }
@@ -228,7 +231,7 @@ struct RayState {
     bool finite;
 };
 
-template <bool COUNT, int LDS>
+template <bool COUNT, int LDS, bool FINITE>
 __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
                                            const float4* __restrict__ tris, RayState& r,
                                            uint32_t (*s_stack)[kPersistThreads],
@@ -238,75 +241,56 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
     const uint32_t w = __float_as_uint(r3.x);
     const uint32_t l = w >> 2;
     const bool in0 = (w & 1u) != 0, in1 = (w & 2u) != 0;
-    float h0, h1, u0 = 0.0f, v0 = 0.0f, u1 = 0.0f, v1 = 0.0f;
-    uint32_t id0 = kInvalid, id1 = kInvalid;
     if (COUNT) cnt.iters++;
     // Both child boxes in one code path for every lane (a leaf child's slot
-    // holds no box; its value is discarded), then the triangle tests of the
-    // leaf children, first leaf then second, in one loop body, so lanes at
-    // different depths share the instructions.
-    if (r.finite) {
-        h0 = slab_test_finite(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
-        h1 = slab_test_finite(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
-    } else {
-        h0 = slab_test(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
-        h1 = slab_test(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
-    }
+    // holds no box; its value is discarded).
+    const float h0 = child_box<COUNT, FINITE>(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+    const float h1 = child_box<COUNT, FINITE>(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
     if (COUNT) cnt.boxes += static_cast<uint32_t>(in0) + static_cast<uint32_t>(in1);
+    // Leaf children first.  The reference orders a node's two children by
+    // distance and lets a leaf replace the best hit when nearer, or equally
+    // near with a lower id (trace.h:125-160): over the leaves of one node
+    // that is the lexicographic minimum of (t, id), whatever their order.
+    // Taking it before deciding on the internal children visits the same
+    // nodes: an internal child nearer than a leaf stays within the updated
+    // best distance, and one farther than a leaf is culled against the
+    // distance the leaf set, as in the reference.
     const uint32_t nleaf = static_cast<uint32_t>(!in0) + static_cast<uint32_t>(!in1);
     for (uint32_t j = 0; j < nleaf; ++j) {
-        const bool first = j == 0 && !in0;  // this test is child 0's
         float u, v;
         uint32_t id;
-        const float h = tri_hit(tris, first ? l : l + 1, r.o, r.d, u, v, id);
+        const float t = tri_hit(tris, (j == 0 && !in0) ? l : l + 1, r.o, r.d, u, v, id);
         if (COUNT) cnt.tris++;
-        if (first) {
-            h0 = h; u0 = u; v0 = v; id0 = id;
-        } else {
-            h1 = h; u1 = u; v1 = v; id1 = id;
+        if (t < r.minT || (t == r.minT && id < r.best)) {
+            r.minT = t; r.best = id; r.bu = u; r.bv = v;
         }
     }
-    if (h0 <= r.minT || h1 <= r.minT) {
-        const bool sw = h1 < h0;  // left child wins ties (trace.h:125)
-        const float hc = sw ? h1 : h0, hf = sw ? h0 : h1;
-        const uint32_t cc = sw ? l + 1 : l, cf = sw ? l : l + 1;
-        const bool ic = sw ? in1 : in0, jf = sw ? in0 : in1;
-        const float uc = sw ? u1 : u0, vc = sw ? v1 : v0, uf = sw ? u0 : u1, vf = sw ? v0 : v1;
-        const uint32_t idc = sw ? id1 : id0, idf = sw ? id0 : id1;
-        if (ic) {
-            r.node = cc;
-            if (hf <= r.minT) {
-                if (jf) {
-                    const int sp = r.sp;
-                    if (sp < LDS) {
-                        s_stack[sp][threadIdx.x] = cf;
-                        r.sp = sp + 1;
-                    } else if (sp < kStackMax) {
-                        spill[sp - LDS] = cf;
-                        r.sp = sp + 1;
-                    }
-                } else if (hf < r.minT || idf < r.best) {
-                    r.minT = hf; r.best = idf; r.bu = uf; r.bv = vf;
-                }
-            }
-            return true;
-        }
-        if (hc < r.minT || idc < r.best) {
-            r.minT = hc; r.best = idc; r.bu = uc; r.bv = vc;
-        }
-        if (hf <= r.minT) {
-            if (jf) {
-                r.node = cf;
-                return true;
-            }
-            if (hf < r.minT || idf < r.best) {
-                r.minT = hf; r.best = idf; r.bu = uf; r.bv = vf;
+    const bool g0 = in0 && h0 <= r.minT, g1 = in1 && h1 <= r.minT;
+    if (g0 || g1) {
+        const bool sw = g1 && (!g0 || h1 < h0);  // left child wins ties (trace.h:125)
+        r.node = sw ? l + 1 : l;
+        if (g0 && g1) {
+            const uint32_t far = sw ? l : l + 1;
+            const int sp = r.sp;
+            if (sp < LDS) {
+                s_stack[sp][threadIdx.x] = far;
+                r.sp = sp + 1;
+            } else if (sp < kStackMax) {
+                spill[sp - LDS] = far;
+                r.sp = sp + 1;
             }
         }
+        return true;
     }
     if (r.sp == 0) return false;
     const int sp = r.sp - 1;
-    r.node = sp < LDS ? s_stack[sp][threadIdx.x] : spill[sp - LDS];
+    // The LDS entry is read by every lane and passed through an empty asm
+    // statement, so the compiler keeps the LDS and scratch reads apart (a
+    // select of the two addresses would become one slower flat load).
+    uint32_t next = s_stack[sp < LDS ? sp : LDS - 1][threadIdx.x];
+    asm volatile("" : "+v"(next));
+    if (sp >= LDS) next = spill[sp - LDS];
+    r.node = next;
     r.sp = sp;
     return true;
 }
@@ -402,7 +386,8 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
             }
         }
         if (__ballot(active) == 0ull) break;
-        if (active && !visit_step<COUNT, LDS>(nodes, tris, r, s_stack, spill, cnt)) {
+        if (active && !(r.finite ? visit_step<COUNT, LDS, true>(nodes, tris, r, s_stack, spill, cnt)
+                                 : visit_step<COUNT, LDS, false>(nodes, tris, r, s_stack, spill, cnt))) {
             TraceHit h;
             h.id = r.best;
             h.t = r.best == kInvalid ? kFltMax : r.minT;
