@@ -107,7 +107,7 @@ struct DeviceBvh {
     DevBuf nodes, tris, root_box;
     DevBuf tile_hist, digit_total;  // radix sort scratch
     DevBuf span_list, span_top, span_count;  // refit: per-level node lists
-    DevBuf ray_counter;                      // persistent traversal: next ray
+    DevBuf ray_counter;                      // persistent traversal: ray queue block
 };
 
 // One or more scenes: scene s owns triangles [off[s], off[s+1]) and its own

@@ -25,6 +25,7 @@
 
 #include "context.hpp"
 #include "devmath.hpp"
+#include "traverse.hpp"
 
 namespace pt {
 namespace {
@@ -623,7 +624,7 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     b.nodes.reserve(static_cast<size_t>(n) * 64);
     b.tris.reserve(static_cast<size_t>(n) * 48);
     b.root_box.reserve(static_cast<size_t>(nscenes) * 6 * sizeof(float));
-    b.ray_counter.reserve(sizeof(uint32_t));
+    b.ray_counter.reserve(kQueueWords * sizeof(uint32_t));
     const uint32_t num_tiles = div_up(n, kSortTile);
     b.tile_hist.reserve(static_cast<size_t>(num_tiles) * 256 * 4);
     b.digit_total.reserve(256 * 4);

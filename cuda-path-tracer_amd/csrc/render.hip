@@ -77,6 +77,7 @@ struct CacheIO {
 constexpr uint32_t kTrainSalt = 0x5bd1e995u;
 
 constexpr int kThreads = 256;
+constexpr uint32_t kMaxLevels = 64;  // ray queue blocks: at most 61 path levels (60 bounces)
 
 __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
                                                        const PassParams* __restrict__ pp,
@@ -89,8 +90,8 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
     if (gid == 0) {
         counters[0] = npaths;
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
-        for (uint32_t k = 0; k <= nlevels; ++k) ray_counters[k] = 0u;
     }
+    if (blockIdx.x == 0) zero_queue(ray_counters, nlevels);
     for (uint32_t p = gid; p < npaths; p += gridDim.x * blockDim.x) {
         const PassParams& P = pp[p / fc.npix];
         const uint32_t pix = p % fc.npix;
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
 }
 
 // Closest hit for every ray of queue `level` (count on the device): the
-// persistent traversal, rays handed out from ray_counters[level].
+// persistent traversal, rays handed out from the level's ray queue block.
 // Each path traces the scene of its pass (root internal node off[scene]).
 template <int LDS>
 __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
@@ -148,7 +149,8 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
         hq.u[i] = h.u;
         hq.v[i] = h.v;
     };
-    persistent_trace<false, LDS>(nodes, tris, count, ray_counters + level, refill, pool, s_stack, load, store, cnt);
+    persistent_trace<false, LDS>(nodes, tris, count, ray_counters + level * kQueueWords, refill, pool, s_stack,
+                                 load, store, cnt);
 }
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
@@ -446,8 +448,8 @@ __global__ __launch_bounds__(kThreads) void k_generate_cache(const FrameConst fc
     if (gid == 0) {
         counters[0] = npaths;
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
-        for (uint32_t k = 0; k <= nlevels; ++k) ray_counters[k] = 0u;
     }
+    if (blockIdx.x == 0) zero_queue(ray_counters, nlevels);
     const PassParams& P = pp[0];
     const uint32_t w = static_cast<uint32_t>(fc.w), h = static_cast<uint32_t>(fc.h);
     const uint32_t ox = min(P.pass % 4u, w - 1u), oy = min((P.pass / 4u) % 4u, h - 1u);
@@ -628,7 +630,8 @@ void render_resize(Renderer& r, int w, int h, hipStream_t st) {
     r.w = w;
     r.h = h;
     r.accum.reserve(np * 3 * sizeof(double));
-    r.counters.reserve(128 * sizeof(uint32_t));  // queue lengths, then ray hand-out counters
+    // Queue lengths (64 words), then one ray queue block per level.
+    r.counters.reserve((64 + static_cast<size_t>(kMaxLevels) * kQueueWords) * sizeof(uint32_t));
     render_reset(r, st);
 }
 

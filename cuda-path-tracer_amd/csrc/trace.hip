@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_scenes(
         cnt);
 }
 
-__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+__global__ __launch_bounds__(64) void k_zero_queue(uint32_t* queue) { zero_queue(queue, 1u); }
 
 template <bool COUNT>
 void launch_trace(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
@@ -130,7 +130,7 @@ void launch_trace(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_
         return;
     }
     const TraceTuning& tune = trace_tuning();
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s.bvh.ray_counter.as<uint32_t>());
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, st, s.bvh.ray_counter.as<uint32_t>());
     const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
     const uint32_t blocks =
         static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
@@ -162,7 +162,7 @@ void trace_closest_scenes(const DeviceScene& s, const float* d_rays, const uint3
                           uint32_t n, uint32_t* d_id, float* d_t, float* d_uv, hipStream_t st) {
     if (n == 0) return;
     const TraceTuning& tune = trace_tuning();
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, s.bvh.ray_counter.as<uint32_t>());
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, st, s.bvh.ray_counter.as<uint32_t>());
     const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
     const uint32_t blocks =
         static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));

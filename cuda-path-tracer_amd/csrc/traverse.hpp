@@ -295,21 +295,55 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
     return true;
 }
 
+// Ray hand-out.  The rays [0, nrays) are split into nq contiguous slices
+// (nq <= 64, about one per 16K rays), each with its own counter 256 bytes
+// from the next: one address sustains only about 90 M returning atomics/s on
+// MI355X, so a single counter caps how small the pools can be, and large
+// pools leave waves unevenly loaded at the end.  A wave starts on slice
+// (wave index mod nq), takes pools from it and, once it is used up, moves on
+// to the next slice that is neither marked in a 64-bit mask of used-up
+// slices nor found used up by its own probes.  A slice is marked by the one
+// wave whose pool reaches its end, after every ray of it has been handed out;
+// the mask only saves probes.  A wave stops when every slice is marked or
+// found used up.  Slices hold at least 16K rays, so none is empty.  Counters
+// and mask live in one queue block of kQueueWords words that the caller
+// zeroes (zero_queue) before the launch.
+constexpr uint32_t kQueueSlices = 64;
+constexpr uint32_t kQueueStride = 64;  // words between counters (256 bytes)
+constexpr uint32_t kQueueMask = kQueueSlices * kQueueStride;  // word offset of the mask
+constexpr uint32_t kQueueWords = kQueueMask + 64;
+constexpr uint32_t kSliceRays = 16384;
+
+__device__ __forceinline__ uint32_t queue_slices(uint32_t nrays) {
+    return min(kQueueSlices, max(1u, nrays / kSliceRays));
+}
+
+
+// Zeroes the counters and the mask of `levels` consecutive queue blocks with
+// the threads of one workgroup.
+__device__ __forceinline__ void zero_queue(uint32_t* __restrict__ queue, uint32_t levels) {
+    const uint32_t per = kQueueSlices + 2;
+    for (uint32_t i = threadIdx.x; i < levels * per; i += blockDim.x) {
+        const uint32_t lvl = i / per, k = i % per;
+        queue[lvl * kQueueWords + (k < kQueueSlices ? k * kQueueStride : kQueueMask + (k - kQueueSlices))] = 0u;
+    }
+}
+
 // Persistent closest-hit loop over rays [0, nrays), handed out in per-wave
-// pools from *ray_counter (zeroed by the caller): pool_large rays (tuned,
+// pools from the queue block `queue` (see above): pool_large rays (tuned,
 // 256 by default) when there are at least four per lane of the grid, else 64
-// so that a short queue still
-// spreads over every wave.  Idle lanes are refilled when fewer than `refill`
-// (1..64) lanes of the wave are busy.  load(i, o, d) fetches ray i;
-// store(i, hit) writes its result.  load returns the ray's root internal
-// node, or kInvalid for a scene without one (fewer than two triangles); then
-// `single` names its one triangle (or kInvalid) and the ray is answered at
-// once without entering the loop.
+// so that a short queue still spreads over every wave.  Idle lanes are
+// refilled when fewer than `refill` (1..64) lanes of the wave are busy.
+// load(i, o, d) fetches ray i; store(i, hit) writes its result.  load returns
+// the ray's root internal node, or kInvalid for a scene without one (fewer
+// than two triangles); then `single` names its one triangle (or kInvalid) and
+// the ray is answered at once without entering the loop.
 constexpr uint32_t kRayPoolSmall = 64;
 
 // The tuned pool size a launch of `blocks` workgroups over at most `nmax` rays
-// may use: every wave adds one pool to *ray_counter past the last ray before
-// it stops, so nmax + (waves + 1) * pool must stay below 2^32.
+// may use: every wave adds at most one pool to a slice counter past the
+// slice's end before the slice is marked, so nmax + (waves + 1) * pool must
+// stay below 2^32.
 inline uint32_t pool_for(uint32_t pool, uint32_t nmax, uint32_t blocks) {
     const uint64_t waves = static_cast<uint64_t>(blocks) * (kPersistThreads / 64) + 1;
     const uint64_t room = (0xffffffffull - nmax) / waves;
@@ -319,7 +353,7 @@ inline uint32_t pool_for(uint32_t pool, uint32_t nmax, uint32_t blocks) {
 template <bool COUNT, int LDS, class Load, class Store>
 __device__ __forceinline__ void persistent_trace(const float4* __restrict__ nodes,
                                                  const float4* __restrict__ tris, uint32_t nrays,
-                                                 uint32_t* __restrict__ ray_counter,
+                                                 uint32_t* __restrict__ queue,
                                                  uint32_t refill, uint32_t pool_large,
                                                  uint32_t (*s_stack)[kPersistThreads], Load load,
                                                  Store store, TraceCount& cnt) {
@@ -333,21 +367,39 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
     bool drained = false;                  // wave-uniform: no rays left to hand out
     const uint32_t pool =
         nrays / 4u >= gridDim.x * blockDim.x ? pool_large : kRayPoolSmall;
+    const uint32_t nq = queue_slices(nrays);
+    const uint32_t per = nrays / nq, extra = nrays % nq;  // slice q: per rays, +1 if q < extra
+    const uint64_t all = nq == 64u ? ~0ull : (1ull << nq) - 1ull;
+    uint64_t* mask = reinterpret_cast<uint64_t*>(queue + kQueueMask);
+    // Wave-uniform state is read through readfirstlane so that it lives in
+    // scalar registers (the compiler cannot tell that threadIdx.x / 64 is
+    // uniform over a wave).
+    uint32_t q = __builtin_amdgcn_readfirstlane((blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u) % nq);
+    uint64_t seen = ~all;  // wave-uniform: slices this wave found used up
+    bool used_up = false;  // wave-uniform: the refill loop found slice q used up
     for (;;) {
         const uint64_t busy = __ballot(active);
         if (!drained && static_cast<uint32_t>(__popcll(busy)) < refill) {
             uint64_t idle = ~busy;
             while (idle != 0ull) {
                 if (pool_next >= pool_end) {
+                    const uint32_t beg = q * per + min(q, extra);
+                    const uint32_t len = per + (q < extra ? 1u : 0u);
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(ray_counter, pool);
-                    base = __shfl(base, 0, 64);
-                    if (base >= nrays) {
-                        drained = true;
+                    if (lane == 0) {
+                        base = atomicAdd(queue + q * kQueueStride, pool);
+                        // Only the pool that reaches the slice's end marks it:
+                        // one atomic per slice (every wave marking the slices
+                        // it finds used up would serialise on the mask).
+                        if (base < len && len - base <= pool) atomicOr(mask, 1ull << q);
+                    }
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    if (base >= len) {
+                        used_up = true;
                         break;
                     }
-                    pool_next = base;
-                    pool_end = min(base + pool, nrays);
+                    pool_next = beg + base;
+                    pool_end = beg + base + min(pool, len - base);
                 }
                 const uint32_t avail = pool_end - pool_next;
                 const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
@@ -384,8 +436,34 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
                 pool_next += took;
                 idle = ~__ballot(active);
             }
+            if (used_up) {
+                // Slice q is used up: move to the next slice not known to be
+                // used up (kept out of the refill loop above, which stays
+                // short).  Every visit here adds a slice to `seen`, so after
+                // at most nq of them the wave has a slice with rays or knows
+                // that all are used up.
+                used_up = false;
+                seen |= 1ull << q;
+                uint32_t lo = 0, hi = 0;
+                if (lane == 0) {
+                    const uint64_t m = __hip_atomic_load(mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    lo = static_cast<uint32_t>(m);
+                    hi = static_cast<uint32_t>(m >> 32);
+                }
+                lo = __builtin_amdgcn_readfirstlane(lo);
+                hi = __builtin_amdgcn_readfirstlane(hi);
+                const uint64_t open = ~(seen | (static_cast<uint64_t>(hi) << 32) | lo);
+                if (open == 0ull) {
+                    drained = true;
+                } else {
+                    const uint64_t above = q < 63u ? open & (~0ull << (q + 1u)) : 0ull;
+                    q = static_cast<uint32_t>(__builtin_ctzll(above ? above : open));
+                }
+            }
         }
-        if (__ballot(active) == 0ull) break;
+        // With no busy lane the wave stops once no slice has rays left; until
+        // then the next pass probes the slice just chosen.
+        if (__ballot(active) == 0ull && drained) break;
         if (active && !(r.finite ? visit_step<COUNT, LDS, true>(nodes, tris, r, s_stack, spill, cnt)
                                  : visit_step<COUNT, LDS, false>(nodes, tris, r, s_stack, spill, cnt))) {
             TraceHit h;
