@@ -143,7 +143,7 @@ void dump_bvh(DeviceScene& scene, uint32_t scene_index, hipStream_t stream, uint
 // Launch shape of the persistent closest-hit kernel (tunable).
 struct TraceTuning {
     uint32_t blocks = 2048;  // persistent workgroups of 256 threads
-    uint32_t refill = 48;    // refill idle lanes when fewer than this many are busy
+    uint32_t refill = 32;    // refill idle lanes when fewer than this many are busy
     uint32_t lds = 16;       // traversal stack entries kept in LDS: 8, 16 or 32
     uint32_t pool = 256;     // rays a wave takes per hand-out on long queues (64..4096)
 };
