@@ -317,6 +317,48 @@ int pt_trace_closest_counted(pt_context* ctx, const float* d_rays, uint32_t n, u
     });
 }
 
+namespace {
+// Writes the running-mean image into the caller's screen buffer, when one is
+// set, after every frame (the reference writes its surface in raytrace_kernel,
+// cuda_path_tracer.cu:411-420).
+void present(pt_context* ctx) {
+    const pt::Renderer& r = ctx->renderer;
+    if (r.screen) pt::render_display(r, r.screen, r.screen_w, r.screen_h, r.screen_pitch, ctx->stream);
+}
+}  // namespace
+
+int pt_set_screen_buffer(pt_context* ctx, void* d_rgba, uint32_t w, uint32_t h, size_t row_pitch) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        pt::Renderer& r = ctx->renderer;
+        if (!d_rgba) {
+            r.screen = nullptr;
+            r.screen_w = r.screen_h = 0;
+            r.screen_pitch = 0;
+            return;
+        }
+        require(w > 0 && h > 0, "screen size must be positive");
+        require(row_pitch >= static_cast<size_t>(w) * 16 && row_pitch % 16 == 0,
+                "row pitch must hold w float4 texels and be a multiple of 16 bytes");
+        require(reinterpret_cast<uintptr_t>(d_rgba) % 16 == 0, "screen buffer must be 16-byte aligned");
+        r.screen = d_rgba;
+        r.screen_w = w;
+        r.screen_h = h;
+        r.screen_pitch = row_pitch;
+    });
+}
+
+int pt_read_image_device(pt_context* ctx, void* d_rgba, uint32_t w, uint32_t h, size_t row_pitch,
+                         void* stream) {
+    if (!ctx || !d_rgba) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        require(row_pitch >= static_cast<size_t>(w) * 16 && row_pitch % 16 == 0,
+                "row pitch must hold w float4 texels and be a multiple of 16 bytes");
+        require(reinterpret_cast<uintptr_t>(d_rgba) % 16 == 0, "image buffer must be 16-byte aligned");
+        pt::render_display(ctx->renderer, d_rgba, w, h, row_pitch, pick(ctx, stream));
+    });
+}
+
 int pt_set_primary_rays(pt_context* ctx, const float* dirs, const float* eye) {
     if (!ctx || !dirs || !eye) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
@@ -335,6 +377,7 @@ int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed) {
         pt::render_passes(ctx->renderer, ctx->scene, 1, nullptr, &pass_seed, nullptr, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
         ctx->renderer.timed = true;
+        present(ctx);
     });
 }
 
@@ -349,6 +392,7 @@ int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const floa
         pt::render_passes(ctx->renderer, ctx->scene, npasses, dirs, seeds, scenes, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
         ctx->renderer.timed = true;
+        present(ctx);
     });
 }
 
@@ -518,6 +562,7 @@ int pt_render_cache_frame(pt_context* ctx, int w, int h, uint64_t pass_seed,
         pt::render_cache_frame(ctx->renderer, ctx->scene, ctx->cache, pass_seed, *settings, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
         ctx->renderer.timed = true;
+        present(ctx);
     });
 }
 

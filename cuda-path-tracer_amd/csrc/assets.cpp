@@ -2,7 +2,9 @@
 // assimp (AssetImporter.cpp:12-25), which is not part of this build; this is
 // a small parser of the OBJ/MTL subset a path tracer needs: v, vn, f (v,
 // v/vt, v//vn, v/vt/vn, negative indices), usemtl, mtllib; newmtl, Kd, Ke, Ns.
+#include <algorithm>
 #include <cerrno>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -208,6 +210,109 @@ int pt_load_obj(const char* path, const pt_obj_options* options, pt_triangle* tr
             }
         }
         *count = id;
+        return PT_OK;
+    } catch (const ParseError& e) {
+        g_error = e.msg;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+    }
+    return PT_E_INVALID_ARG;
+}
+
+namespace {
+
+// PNG (ISO/IEC 15948) with a zlib stream (RFC 1950) of stored deflate blocks
+// (RFC 1951, BTYPE 00): no compression, no dependency.
+uint32_t crc32_update(uint32_t crc, const uint8_t* p, size_t n) {
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t k = 0; k < 256; ++k) {
+            uint32_t c = k;
+            for (int b = 0; b < 8; ++b) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[k] = c;
+        }
+        ready = true;
+    }
+    for (size_t i = 0; i < n; ++i) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    return crc;
+}
+
+void put_be32(std::vector<uint8_t>& v, uint32_t x) {
+    for (int s = 24; s >= 0; s -= 8) v.push_back(static_cast<uint8_t>(x >> s));
+}
+
+void put_chunk(std::vector<uint8_t>& out, const char type[4], const std::vector<uint8_t>& data) {
+    put_be32(out, static_cast<uint32_t>(data.size()));
+    const size_t at = out.size();
+    out.insert(out.end(), type, type + 4);
+    out.insert(out.end(), data.begin(), data.end());
+    const uint32_t crc = crc32_update(0xFFFFFFFFu, out.data() + at, 4 + data.size()) ^ 0xFFFFFFFFu;
+    put_be32(out, crc);
+}
+
+uint8_t quantize(float v, bool srgb) {
+    if (!(v > 0.0f)) return 0;  // also NaN
+    if (v >= 1.0f) return 255;
+    double x = v;
+    if (srgb) x = x <= 0.0031308 ? 12.92 * x : 1.055 * std::pow(x, 1.0 / 2.4) - 0.055;
+    return static_cast<uint8_t>(std::floor(x * 255.0 + 0.5));
+}
+
+}  // namespace
+
+int pt_write_png(const char* path, const float* rgba, uint32_t w, uint32_t h, uint32_t flags) {
+    try {
+        if (!path || !rgba || w == 0 || h == 0) throw ParseError{"pt_write_png: NULL argument or empty image"};
+        if (static_cast<uint64_t>(w) * 3 + 1 > 0x7FFFFFFFull || static_cast<uint64_t>(h) > 0x7FFFFFFFull)
+            throw ParseError{"pt_write_png: image too large"};
+        const bool srgb = (flags & PT_PNG_SRGB) != 0, top_down = (flags & PT_PNG_TOP_DOWN) != 0;
+        // Filtered scanlines: filter type 0, then RGB bytes; PNG rows run top to bottom.
+        const size_t row = static_cast<size_t>(w) * 3 + 1;
+        std::vector<uint8_t> raw(row * h);
+        for (uint32_t y = 0; y < h; ++y) {
+            const uint32_t src = top_down ? y : h - 1 - y;
+            uint8_t* d = raw.data() + row * y;
+            d[0] = 0;
+            const float* s = rgba + static_cast<size_t>(src) * w * 4;
+            for (uint32_t x = 0; x < w; ++x)
+                for (int c = 0; c < 3; ++c) d[1 + 3 * x + c] = quantize(s[4 * x + c], srgb);
+        }
+        std::vector<uint8_t> z;
+        z.reserve(raw.size() + raw.size() / 65535 * 5 + 16);
+        z.push_back(0x78);
+        z.push_back(0x01);
+        size_t at = 0;
+        do {
+            const size_t n = std::min<size_t>(65535, raw.size() - at);
+            z.push_back(at + n == raw.size() ? 1 : 0);
+            z.push_back(static_cast<uint8_t>(n));
+            z.push_back(static_cast<uint8_t>(n >> 8));
+            z.push_back(static_cast<uint8_t>(~n));
+            z.push_back(static_cast<uint8_t>(~n >> 8));
+            z.insert(z.end(), raw.begin() + static_cast<std::ptrdiff_t>(at),
+                     raw.begin() + static_cast<std::ptrdiff_t>(at + n));
+            at += n;
+        } while (at < raw.size());
+        uint32_t a = 1, b = 0;  // Adler-32
+        for (uint8_t v : raw) {
+            a = (a + v) % 65521u;
+            b = (b + a) % 65521u;
+        }
+        put_be32(z, (b << 16) | a);
+        std::vector<uint8_t> out = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+        std::vector<uint8_t> ihdr;
+        put_be32(ihdr, w);
+        put_be32(ihdr, h);
+        ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8-bit RGB, deflate, filter 0, no interlace
+        put_chunk(out, "IHDR", ihdr);
+        put_chunk(out, "IDAT", z);
+        put_chunk(out, "IEND", {});
+        std::FILE* f = std::fopen(path, "wb");
+        if (!f) throw ParseError{std::string("cannot open ") + path + " for writing"};
+        const size_t put = std::fwrite(out.data(), 1, out.size(), f);
+        const int closed = std::fclose(f);
+        if (put != out.size() || closed != 0) throw ParseError{std::string("cannot write ") + path};
         return PT_OK;
     } catch (const ParseError& e) {
         g_error = e.msg;

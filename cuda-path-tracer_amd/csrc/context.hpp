@@ -195,6 +195,9 @@ struct Renderer {
     uint32_t last_train = 0;   // training paths of the last cache frame
     void* cache_graph_exec = nullptr;  // hipGraphExec_t of the last cache frame
     alignas(8) char cache_graph_key[320] = {};
+    void* screen = nullptr;    // caller's RGBA32F screen buffer (device), or none
+    uint32_t screen_w = 0, screen_h = 0;
+    size_t screen_pitch = 0;   // bytes between rows
     Renderer() = default;
     Renderer(const Renderer&) = delete;
     Renderer& operator=(const Renderer&) = delete;
@@ -212,6 +215,11 @@ void render_reset(Renderer& r, hipStream_t stream);
 // renderer's corner rays for every pass); seeds[npasses].
 void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, const float* dirs,
                    const uint64_t* seeds, const uint32_t* scenes, hipStream_t stream);
+// Running-mean image (sum / spp as float, alpha 1) of the accumulator's
+// min(w, out_w) x min(h, out_h) corner into a device RGBA32F buffer whose rows
+// are `pitch` bytes apart.
+void render_display(const Renderer& r, void* out, uint32_t out_w, uint32_t out_h, size_t pitch,
+                    hipStream_t stream);
 // Rays traced by the last frame (sum of its queue lengths); synchronises.
 uint64_t render_last_rays(Renderer& r, hipStream_t stream);
 

@@ -635,6 +635,32 @@ void render_resize(Renderer& r, int w, int h, hipStream_t st) {
     render_reset(r, st);
 }
 
+namespace {
+// One thread per pixel: rgba = (float(sum * (1 / spp)), 1), the value
+// pt_read_image computes on the host (same double product, same rounding).
+__global__ __launch_bounds__(kThreads) void k_display(const double* __restrict__ accum, int w,
+                                                      uint32_t spp, char* __restrict__ out,
+                                                      uint32_t ow, uint32_t oh, size_t pitch) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= ow || y >= oh) return;
+    const double inv = spp ? 1.0 / static_cast<double>(spp) : 0.0;
+    const double* s = accum + 3 * (static_cast<size_t>(y) * w + x);
+    float4* row = reinterpret_cast<float4*>(out + static_cast<size_t>(y) * pitch);
+    row[x] = make_float4(static_cast<float>(s[0] * inv), static_cast<float>(s[1] * inv),
+                         static_cast<float>(s[2] * inv), 1.0f);
+}
+}  // namespace
+
+void render_display(const Renderer& r, void* out, uint32_t out_w, uint32_t out_h, size_t pitch,
+                    hipStream_t st) {
+    const uint32_t w = std::min(out_w, static_cast<uint32_t>(r.w));
+    const uint32_t h = std::min(out_h, static_cast<uint32_t>(r.h));
+    if (w == 0 || h == 0) return;
+    hipLaunchKernelGGL(k_display, dim3((w + kThreads - 1) / kThreads, h), dim3(kThreads), 0, st,
+                       r.accum.as<double>(), r.w, r.spp, static_cast<char*>(out), w, h, pitch);
+    PT_HIP(hipGetLastError());
+}
+
 void render_reset(Renderer& r, hipStream_t st) {
     r.spp = 0;
     if (r.w <= 0) return;

@@ -97,6 +97,9 @@ _SIGS = {
     "pt_read_accumulator": [_vp, _vp, _vp],
     "pt_write_accumulator": [_vp, _i, _i, _vp, _u32],
     "pt_read_image": [_vp, _vp],
+    "pt_read_image_device": [_vp, _vp, _u32, _u32, ctypes.c_size_t, _vp],
+    "pt_set_screen_buffer": [_vp, _vp, _u32, _u32, ctypes.c_size_t],
+    "pt_write_png": [ctypes.c_char_p, _vp, _u32, _u32, _u32],
     "pt_last_render_ms": [_vp, _vp],
     "pt_scene_bounds": [_vp, _u32, _vp, _vp],
     "pt_gen_soup": [_u64, _u32, _i, _u32, _vp, _vp],
@@ -255,6 +258,24 @@ def load_obj(path, left_handed=True, normalize_albedo=True):
         _lib.pt_asset_error(buf, len(buf))
         raise PtError(rc, buf.value.decode(errors="replace"))
     return tris, infos
+
+
+PT_PNG_SRGB = 1
+PT_PNG_TOP_DOWN = 2
+
+
+def write_png(path, rgba, srgb=False, top_down=False):
+    """8-bit RGB PNG of an (h, w, 4) float32 image (pt_read_image's layout:
+    row 0 at the bottom unless top_down)."""
+    img = np.ascontiguousarray(rgba, np.float32)
+    if img.ndim != 3 or img.shape[2] != 4:
+        raise ValueError("rgba must have shape (h, w, 4)")
+    flags = (PT_PNG_SRGB if srgb else 0) | (PT_PNG_TOP_DOWN if top_down else 0)
+    rc = _lib.pt_write_png(os.fsencode(path), _ptr(img), img.shape[1], img.shape[0], flags)
+    if rc != PT_OK:
+        buf = ctypes.create_string_buffer(512)
+        _lib.pt_asset_error(buf, len(buf))
+        raise PtError(rc, buf.value.decode(errors="replace"))
 
 
 def tune_set(name, value):
@@ -433,6 +454,15 @@ class Context:
         img = np.zeros((h, w, 4), np.float32)
         self._c(_lib.pt_read_image(self._h, _ptr(img)))
         return img
+
+    def read_image_device(self, d_rgba, w, h, row_pitch=None, stream=None):
+        """Running-mean image into device memory (float4 texels)."""
+        self._c(_lib.pt_read_image_device(self._h, d_rgba, w, h, row_pitch or 16 * w, stream))
+
+    def set_screen_buffer(self, d_rgba, w=0, h=0, row_pitch=None):
+        """Device RGBA32F buffer the image is written into after every frame
+        (d_rgba None or 0 detaches)."""
+        self._c(_lib.pt_set_screen_buffer(self._h, d_rgba or None, w, h, row_pitch or 16 * w))
 
     def last_render_ms(self):
         ms = np.zeros(1, np.float32)

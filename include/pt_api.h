@@ -192,8 +192,22 @@ int pt_reset_frames(pt_context* ctx);
 int pt_read_accumulator(pt_context* ctx, double* rgb_sum, uint32_t* spp);
 int pt_write_accumulator(pt_context* ctx, int w, int h, const double* rgb_sum, uint32_t spp);
 /* Running-mean image as the reference's RGBA32F surface holds it
- * (cuda_path_tracer.cu:413-420): rgba[4*w*h] = sum/spp, alpha 1. */
+ * (cuda_path_tracer.cu:413-420): rgba[4*w*h] = sum/spp, alpha 1.  Row 0 is
+ * the bottom row of the image (pixel y = 0, the reference's tc.y = 0). */
 int pt_read_image(pt_context* ctx, float* rgba);
+/* The same image into device memory: the min(w, width) x min(h, height)
+ * corner of it as float4 texels, rows row_pitch bytes apart (a multiple of
+ * 16 and at least 16 w; buffer 16-byte aligned), on `stream` (NULL: the
+ * context's stream).  Values equal pt_read_image's bit for bit. */
+int pt_read_image_device(pt_context* ctx, void* d_rgba, uint32_t w, uint32_t h, size_t row_pitch,
+                         void* stream);
+/* Replaces CUDAPathTracer::setScreenTexture (cuda_path_tracer.h:18): after
+ * every pt_render_pass, pt_render_passes and pt_render_cache_frame the
+ * running-mean image is written into this caller-owned device RGBA32F buffer
+ * (same layout and clipping as pt_read_image_device) on the context's stream,
+ * outside the frame's timed region.  A graphics API maps the buffer as its
+ * screen texture through its own interop.  d_rgba NULL detaches it. */
+int pt_set_screen_buffer(pt_context* ctx, void* d_rgba, uint32_t w, uint32_t h, size_t row_pitch);
 /* Wall time in ms of the last pt_render_pass (HIP events). */
 int pt_last_render_ms(pt_context* ctx, float* ms);
 

@@ -1,6 +1,6 @@
 /*
  * pt_assets.h -- Wavefront OBJ/MTL import into pt_triangle / pt_triangle_info
- * (host only, no GPU).  Reference: AssetImporter::import
+ * and PNG image output (host only, no GPU).  Reference: AssetImporter::import
  * (source/engine/util/AssetImporter.cpp:12-25, assimp with
  * aiProcessPreset_TargetRealtime_Fast | MakeLeftHanded | FlipWindingOrder),
  * material defaults (:98-102) and PathTracingApp::extractScene
@@ -40,7 +40,19 @@ typedef struct pt_obj_options {
 int pt_load_obj(const char* path, const pt_obj_options* options, pt_triangle* tris,
                 pt_triangle_info* infos, uint32_t* count);
 
-/* The calling thread's last pt_load_obj error message. */
+/* Image output (host only).  Replaces the reference's on-screen display of
+ * its RGBA32F screen texture (PathTracingApp.cpp:282-290) for runs without a
+ * window: writes an 8-bit RGB PNG of a running-mean image such as
+ * pt_read_image returns (rgba[4*w*h], alpha ignored).  Values are clamped to
+ * [0, 1] and rounded to 8 bits, linear as the reference's framebuffer shows
+ * them, or through the sRGB transfer curve with PT_PNG_SRGB.  Row 0 of rgba
+ * is the bottom row (pt_read_image's order) unless PT_PNG_TOP_DOWN.
+ * Returns PT_OK or PT_E_INVALID_ARG (message from pt_asset_error). */
+#define PT_PNG_SRGB 1u
+#define PT_PNG_TOP_DOWN 2u
+int pt_write_png(const char* path, const float* rgba, uint32_t w, uint32_t h, uint32_t flags);
+
+/* The calling thread's last pt_load_obj / pt_write_png error message. */
 int pt_asset_error(char* buf, size_t len);
 
 #ifdef __cplusplus

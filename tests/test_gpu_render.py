@@ -251,3 +251,35 @@ def test_obj_scene_render_matches_oracle(ctx, tmp_path):
         orc.render_pass(tris, b, infos, st, W, H, dirs[p], eye, p, int(seeds[p]), o)
     assert o.sum() > 0
     assert np.array_equal(acc, o)
+
+
+def test_screen_buffer_and_device_image(ctx, scene):
+    """The screen buffer written after every frame, and the device image
+    readback, equal pt_read_image bit for bit; a smaller buffer gets the
+    image's corner, a larger one only the image's extent."""
+    from ptamd import hip
+    tris, infos, st = scene
+    ctx.upload_scene(tris, infos, settings=st)
+    ctx.reset_frames()
+    big = hip.DeviceArray.from_host(np.full((H + 3) * (W + 5) * 4, -7.0, np.float32))
+    small = hip.DeviceArray(10 * 12 * 4, np.float32)
+    try:
+        ctx.set_screen_buffer(big.ptr, W + 5, H + 3)
+        for p in range(3):
+            say(f"[screen] frame {p}")
+            ctx.set_primary_rays(pass_dirs(p), EYE)
+            ctx.render_pass(W, H, 900 + p)
+            ctx.synchronize()
+            want = ctx.read_image(W, H)
+            got = big.to_host().reshape(H + 3, W + 5, 4)
+            np.testing.assert_array_equal(got[:H, :W], want)
+            assert np.all(got[H:] == -7.0) and np.all(got[:, W:] == -7.0)
+        ctx.read_image_device(small.ptr, 12, 10)
+        ctx.synchronize()
+        np.testing.assert_array_equal(small.to_host().reshape(10, 12, 4), ctx.read_image(W, H)[:10, :12])
+        with pytest.raises(ptamd.PtError):
+            ctx.set_screen_buffer(big.ptr, W, H, row_pitch=8 * W)
+    finally:
+        ctx.set_screen_buffer(None)
+    ctx.render_pass(W, H, 999)  # detached: nothing written
+    ctx.synchronize()
