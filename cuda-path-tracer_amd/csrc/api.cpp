@@ -130,6 +130,9 @@ void upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* off
     s.ready = false;
     s.n = n;
     s.ninfos = ninfos;
+    s.max_id = max_id;
+    s.ndetail = 0;  // surface detail belongs to the previous upload
+    s.ntex = 0;
     s.off_host.assign(offs, offs + nscenes + 1);
     s.box_host.assign(static_cast<size_t>(nscenes) * 6, 0.0f);
     std::vector<float> mbox(static_cast<size_t>(nscenes) * 6);
@@ -200,6 +203,66 @@ int pt_upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* s
     if (!ctx) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
         upload_scenes(ctx, tris, scene_offsets, nscenes, infos, ninfos, nullptr, 0, boxes, settings);
+    });
+}
+
+static_assert(sizeof(pt_triangle_detail) == 144, "pt_triangle_detail layout");
+
+int pt_upload_detail(pt_context* ctx, const pt_triangle_detail* detail, uint32_t n,
+                     const pt_texture* textures, uint32_t ntextures) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        pt::DeviceScene& s = ctx->scene;
+        if (!s.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        hipStream_t st = ctx->stream;
+        PT_HIP(hipStreamSynchronize(st));
+        if (n == 0) {
+            s.ndetail = 0;
+            s.ntex = 0;
+            return;
+        }
+        require(detail != nullptr, "detail is NULL");
+        require(s.n == 0 || n > s.max_id, "detail must hold a record for every triangle id");
+        require(ntextures == 0 || textures != nullptr, "textures is NULL");
+        std::vector<uint32_t> info(4 * static_cast<size_t>(ntextures));
+        uint64_t total = 0;
+        for (uint32_t k = 0; k < ntextures; ++k) {
+            const pt_texture& t = textures[k];
+            require(t.width >= 1 && t.width <= 16384 && t.height >= 1 && t.height <= 16384,
+                    "texture sizes must be 1..16384");
+            require(t.rgba != nullptr, "texture texels are NULL");
+            info[4 * k] = static_cast<uint32_t>(total);
+            info[4 * k + 1] = t.width;
+            info[4 * k + 2] = t.height;
+            info[4 * k + 3] = 0;
+            total += static_cast<uint64_t>(t.width) * t.height;
+            require(total < (1ull << 31), "at most 2^31 - 1 texels in all textures");
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            require(detail[i].albedo_tex == PT_NO_TEXTURE || detail[i].albedo_tex < ntextures,
+                    "albedo texture index out of range");
+            require(detail[i].normal_tex == PT_NO_TEXTURE || detail[i].normal_tex < ntextures,
+                    "normal map index out of range");
+        }
+        s.ndetail = 0;
+        s.ntex = 0;
+        s.detail.reserve(sizeof(pt_triangle_detail) * static_cast<size_t>(n));
+        PT_HIP(hipMemcpyAsync(s.detail.as<void>(), detail, sizeof(pt_triangle_detail) * static_cast<size_t>(n),
+                              hipMemcpyHostToDevice, st));
+        if (ntextures) {
+            s.texels.reserve(static_cast<size_t>(total) * 4);
+            s.texinfo.reserve(info.size() * sizeof(uint32_t));
+            for (uint32_t k = 0; k < ntextures; ++k) {
+                PT_HIP(hipMemcpyAsync(s.texels.as<uint32_t>() + info[4 * k], textures[k].rgba,
+                                      static_cast<size_t>(textures[k].width) * textures[k].height * 4,
+                                      hipMemcpyHostToDevice, st));
+            }
+            PT_HIP(hipMemcpyAsync(s.texinfo.as<void>(), info.data(), info.size() * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, st));
+        }
+        PT_HIP(hipStreamSynchronize(st));
+        s.ndetail = n;
+        s.ntex = ntextures;
     });
 }
 

@@ -123,6 +123,11 @@ struct DeviceScene {
     DevBuf infos;                          // pt_triangle_info[ninfos]
     DevBuf lights;                         // pt_dir_light[nlights] (all scenes)
     uint32_t nlights = 0;
+    DevBuf detail;                         // pt_triangle_detail[ndetail], or none
+    DevBuf texels;                         // RGBA8 texels of all textures
+    DevBuf texinfo;                        // uint4 per texture: first texel, w, h, 0
+    uint32_t ndetail = 0, ntex = 0;
+    uint32_t max_id = 0;                   // largest triangle id of the upload
     DeviceBvh bvh;
     bool ready = false;
     uint32_t nscenes() const { return static_cast<uint32_t>(off_host.size()) - 1; }

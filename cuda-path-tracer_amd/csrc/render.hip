@@ -42,6 +42,10 @@ struct FrameConst {
     float intensity;
     uint32_t stratified;
     uint32_t nlights;  // directional lights (shadow rays at every vertex)
+    uint32_t ntex;     // textures of the surface detail
+    const pt_triangle_detail* detail;  // per original id, or null: pt_triangle_info alone
+    const uint32_t* texels;            // RGBA8 texels of all textures
+    const uint4* texinfo;              // per texture: first texel, width, height, 0
 };
 
 // Structure-of-arrays queue of rays (origin, direction, path index).
@@ -183,14 +187,69 @@ struct HitShade {
     bool emissive;
 };
 
-__device__ __forceinline__ HitShade hit_shade(const pt_triangle_info* __restrict__ infos, uint32_t id, float t,
-                                              float bu, float bv, V3 o, V3 d) {
+// barycentricLerp (math_util.h:19-25): v0 * (1 - u - v) + v1 * u + v2 * v.
+__device__ __forceinline__ V3 bary3(const float (*v)[3], float bw, float bu, float bv) {
+    return add(add(scale(ld3(v[0]), bw), scale(ld3(v[1]), bu)), scale(ld3(v[2]), bv));
+}
+
+// One texel as c / 255 per channel (rgb).
+__device__ __forceinline__ V3 texel(const uint32_t* __restrict__ texels, uint32_t k) {
+    const uint32_t c = texels[k];
+    return v3(static_cast<float>(c & 0xFFu) / 255.0f, static_cast<float>((c >> 8) & 0xFFu) / 255.0f,
+              static_cast<float>((c >> 16) & 0xFFu) / 255.0f);
+}
+
+// tex2D on a normalised-coordinate, wrapping, bilinear texture at level 0
+// (Scene.cu:66-73): x = s * width - 0.5, y = t * height - 0.5, the four
+// texels around (x, y) blended with fp32 weights, glm::mix order
+// (p + w * (q - p)), first along x then along y.  Coordinates that are not
+// finite or beyond 1e9 texels read as 0.
+__device__ __forceinline__ V3 tex_sample(const uint32_t* __restrict__ texels, uint4 info, float s, float t) {
+    const uint32_t W = info.y, H = info.z;
+    float x = s * static_cast<float>(W) - 0.5f;
+    float y = t * static_cast<float>(H) - 0.5f;
+    if (!(__builtin_fabsf(x) < 1e9f)) x = 0.0f;
+    if (!(__builtin_fabsf(y) < 1e9f)) y = 0.0f;
+    const float fx = __builtin_floorf(x), fy = __builtin_floorf(y);
+    const float a = x - fx, b = y - fy;
+    int i0 = static_cast<int>(fx) % static_cast<int>(W), j0 = static_cast<int>(fy) % static_cast<int>(H);
+    if (i0 < 0) i0 += static_cast<int>(W);
+    if (j0 < 0) j0 += static_cast<int>(H);
+    const uint32_t i1 = static_cast<uint32_t>(i0) + 1u == W ? 0u : static_cast<uint32_t>(i0) + 1u;
+    const uint32_t j1 = static_cast<uint32_t>(j0) + 1u == H ? 0u : static_cast<uint32_t>(j0) + 1u;
+    const uint32_t r0 = info.x + static_cast<uint32_t>(j0) * W, r1 = info.x + j1 * W;
+    const V3 lo = mix(texel(texels, r0 + static_cast<uint32_t>(i0)), texel(texels, r0 + i1), a);
+    const V3 hi = mix(texel(texels, r1 + static_cast<uint32_t>(i0)), texel(texels, r1 + i1), a);
+    return mix(lo, hi, b);
+}
+
+// DETAIL: the scene has surface detail (fc.detail); kept out of the other
+// kernels, whose registers it would otherwise size.
+template <bool DETAIL>
+__device__ __forceinline__ HitShade hit_shade(const FrameConst& fc, const pt_triangle_info* __restrict__ infos,
+                                              uint32_t id, float t, float bu, float bv, V3 o, V3 d) {
     const pt_triangle_info& in = infos[id];
     HitShade hs;
     hs.pos = add(scale(d, t), o);
     const float bw = (1.0f - bu) - bv;
     hs.nrm = normalize(add(add(scale(ld3(in.n0), bw), scale(ld3(in.n1), bu)), scale(ld3(in.n2), bv)));
-    hs.albedo = ld3(in.albedo);
+    if (DETAIL) {
+        // extractInfo / unpackNormal (cuda_path_tracer.cu:85-138).
+        const pt_triangle_detail& dt = fc.detail[id];
+        const float s = ((dt.uv[0][0] * bw) + (dt.uv[1][0] * bu)) + (dt.uv[2][0] * bv);
+        const float tc = ((dt.uv[0][1] * bw) + (dt.uv[1][1] * bu)) + (dt.uv[2][1] * bv);
+        hs.albedo = dt.albedo_tex != PT_NO_TEXTURE ? tex_sample(fc.texels, fc.texinfo[dt.albedo_tex], s, tc)
+                                                   : normalize(bary3(dt.color, bw, bu, bv));
+        if (dt.normal_tex != PT_NO_TEXTURE) {
+            const V3 ns = tex_sample(fc.texels, fc.texinfo[dt.normal_tex], s, tc);
+            const float nx = 2.0f * ns.x - 1.0f, ny = 2.0f * ns.y - 1.0f, nz = 2.0f * ns.z - 1.0f;
+            const V3 tg = normalize(bary3(dt.tangent, bw, bu, bv));
+            const V3 bt = normalize(bary3(dt.bitangent, bw, bu, bv));
+            hs.nrm = normalize(add(add(scale(tg, nx), scale(bt, ny)), scale(hs.nrm, nz)));
+        }
+    } else {
+        hs.albedo = ld3(in.albedo);
+    }
     hs.e = ld3(in.emission);
     hs.spec = in.specularity;
     hs.emissive = ((hs.e.x + hs.e.y) + hs.e.z) > 1e-7f;
@@ -255,7 +314,7 @@ __device__ __forceinline__ bool next_bounce(uint32_t key, uint32_t pass, uint32_
 
 // CACHE: a cache frame (one pass; paths >= npix are training paths).
 // LIGHTS: the scene has directional lights.
-template <bool CACHE, bool LIGHTS>
+template <bool CACHE, bool LIGHTS, bool DETAIL>
 __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                                                     const PassParams* __restrict__ pp,
                                                     const pt_triangle_info* __restrict__ infos,
@@ -285,7 +344,7 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
             const uint32_t seedw = train ? (P.seedw ^ kTrainSalt) : P.seedw;
             const V3 o = v3(qin.ox[i], qin.oy[i], qin.oz[i]);
             const V3 d = v3(qin.dx[i], qin.dy[i], qin.dz[i]);
-            HitShade hs = hit_shade(infos, hq.id[i], hq.t[i], hq.u[i], hq.v[i], o, d);
+            HitShade hs = hit_shade<DETAIL>(fc, infos, hq.id[i], hq.t[i], hq.u[i], hq.v[i], o, d);
             const uint32_t sb = off[P.scene], se = off[P.scene + 1];
             V3 absorbed, acc;
             if (level == 0) {
@@ -369,6 +428,7 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
 // bits; counters[L] still count the rays of level L.
 constexpr int kSuffixThreads = 64;
 
+template <bool DETAIL>
 __global__ __launch_bounds__(kSuffixThreads) void k_train_suffix(const FrameConst fc,
                                                                  const PassParams* __restrict__ pp,
                                                                  const pt_triangle_info* __restrict__ infos,
@@ -392,7 +452,7 @@ __global__ __launch_bounds__(kSuffixThreads) void k_train_suffix(const FrameCons
         for (uint32_t b = level0;; ++b) {
             const TraceHit h = trace_ray_in(nodes, tris, sb, se, o, d);
             if (h.id == kInvalid) break;
-            HitShade hs = hit_shade(infos, h.id, h.t, h.u, h.v, o, d);
+            HitShade hs = hit_shade<DETAIL>(fc, infos, h.id, h.t, h.u, h.v, o, d);
             if (fc.nlights)
                 bounce_hit<true>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
             else
@@ -569,6 +629,13 @@ uint32_t grid_for(size_t work) {
 }
 
 // Everything a captured frame's kernel arguments depend on.
+void set_detail(FrameConst& fc, const DeviceScene& scene) {
+    fc.ntex = scene.ntex;
+    fc.detail = scene.ndetail ? scene.detail.as<pt_triangle_detail>() : nullptr;
+    fc.texels = scene.ntex ? scene.texels.as<uint32_t>() : nullptr;
+    fc.texinfo = scene.ntex ? scene.texinfo.as<uint4>() : nullptr;
+}
+
 struct GraphKey {
     FrameConst fc;
     const void* nodes;
@@ -597,7 +664,8 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
     const uint32_t pool = pool_for(tune.pool, static_cast<uint32_t>(std::min<size_t>(npaths, 0xffffffffu)), pgrid);
     auto extend = tune.lds <= 8 ? k_extend<8> : (tune.lds <= 16 ? k_extend<16> : k_extend<32>);
-    auto shade = fc.nlights ? k_shade<false, true> : k_shade<false, false>;
+    auto shade = fc.detail ? (fc.nlights ? k_shade<false, true, true> : k_shade<false, false, true>)
+                           : (fc.nlights ? k_shade<false, true, false> : k_shade<false, false, false>);
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt, nlev);
     for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
         hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
@@ -749,6 +817,7 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
     fc.intensity = r.settings.indirect_intensity;
     fc.stratified = r.settings.stratified;
     fc.nlights = scene.nlights;
+    set_detail(fc, scene);
 
     if (!render_tuning().graphs) {
         launch_frame(r, scene, fc, st);
@@ -803,7 +872,9 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
     const uint32_t pool = pool_for(tune.pool, static_cast<uint32_t>(std::min<size_t>(npaths, 0xffffffffu)), pgrid);
     auto extend = tune.lds <= 8 ? k_extend<8> : (tune.lds <= 16 ? k_extend<16> : k_extend<32>);
-    auto shade = fc.nlights ? k_shade<true, true> : k_shade<true, false>;
+    auto shade = fc.detail ? (fc.nlights ? k_shade<true, true, true> : k_shade<true, false, true>)
+                           : (fc.nlights ? k_shade<true, true, false> : k_shade<true, false, false>);
+    auto suffix = fc.detail ? k_train_suffix<true> : k_train_suffix<false>;
     hipLaunchKernelGGL(k_generate_cache, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt,
                        nlev, io);
     // Levels 0 and 1 hold every path; past x2 only training paths are left,
@@ -819,7 +890,7 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     }
     if (nlev > 2) {
         const uint32_t sgrid = std::max(1u, std::min(4096u, (io.ntrain + kSuffixThreads - 1) / kSuffixThreads));
-        hipLaunchKernelGGL(k_train_suffix, dim3(sgrid), dim3(kSuffixThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(suffix, dim3(sgrid), dim3(kSuffixThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[0], L.ps, cnt, 2u, io,
                            scene.lights.as<pt_dir_light>(), scene.bvh.nodes.as<float4>(),
                            scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
@@ -874,6 +945,7 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     cf.fc.intensity = r.settings.indirect_intensity;
     cf.fc.stratified = r.settings.stratified;
     cf.fc.nlights = scene.nlights;
+    set_detail(cf.fc, scene);
     cf.steps = cs.train_steps;
     cf.train = (cs.flags & PT_CACHE_TRAIN) != 0;
 

@@ -39,6 +39,30 @@ TRI_DTYPE = np.dtype([("v0", "<f4", (3,)), ("v1", "<f4", (3,)), ("v2", "<f4", (3
 INFO_DTYPE = np.dtype([("n0", "<f4", (3,)), ("n1", "<f4", (3,)), ("n2", "<f4", (3,)),
                        ("albedo", "<f4", (3,)), ("emission", "<f4", (3,)), ("specularity", "<f4")])
 assert TRI_DTYPE.itemsize == 40 and INFO_DTYPE.itemsize == 64
+# pt_triangle_detail: vertex colours, uvs, tangents, bitangents, texture indices.
+DETAIL_DTYPE = np.dtype([("color", "<f4", (3, 3)), ("uv", "<f4", (3, 2)), ("tangent", "<f4", (3, 3)),
+                         ("bitangent", "<f4", (3, 3)), ("albedo_tex", "<u4"), ("normal_tex", "<u4"),
+                         ("reserved", "<u4")])
+assert DETAIL_DTYPE.itemsize == 144
+NO_TEXTURE = 0xFFFFFFFF
+
+
+class Texture(ctypes.Structure):
+    """pt_texture: width, height, RGBA8 texels (row 0 = t = 0, the bottom row)."""
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba", ctypes.c_void_p)]
+
+
+def texture_array(images):
+    """(h, w, 4) uint8 images -> (ctypes array of Texture, the contiguous
+    copies it points into, which must stay alive while it is used)."""
+    keep = [np.ascontiguousarray(im, np.uint8) for im in images]
+    for im in keep:
+        if im.ndim != 3 or im.shape[2] != 4:
+            raise ValueError("textures must have shape (h, w, 4)")
+    arr = (Texture * max(1, len(keep)))()
+    for k, im in enumerate(keep):
+        arr[k] = Texture(im.shape[1], im.shape[0], im.ctypes.data)
+    return arr, keep
 
 
 class Settings(ctypes.Structure):
@@ -97,6 +121,7 @@ _SIGS = {
     "pt_read_accumulator": [_vp, _vp, _vp],
     "pt_write_accumulator": [_vp, _i, _i, _vp, _u32],
     "pt_read_image": [_vp, _vp],
+    "pt_upload_detail": [_vp, _vp, _u32, _vp, _u32],
     "pt_read_image_device": [_vp, _vp, _u32, _u32, ctypes.c_size_t, _vp],
     "pt_set_screen_buffer": [_vp, _vp, _u32, _u32, ctypes.c_size_t],
     "pt_write_png": [ctypes.c_char_p, _vp, _u32, _u32, _u32],
@@ -454,6 +479,14 @@ class Context:
         img = np.zeros((h, w, 4), np.float32)
         self._c(_lib.pt_read_image(self._h, _ptr(img)))
         return img
+
+    def upload_detail(self, detail, textures=()):
+        """Surface detail (DETAIL_DTYPE per triangle id) and textures ((h, w, 4)
+        uint8, row 0 = bottom); an empty detail removes it."""
+        detail = np.ascontiguousarray(detail, DETAIL_DTYPE)
+        arr, keep = texture_array(textures)
+        self._c(_lib.pt_upload_detail(self._h, _ptr(detail) if len(detail) else None, len(detail),
+                                      ctypes.cast(arr, ctypes.c_void_p) if keep else None, len(keep)))
 
     def read_image_device(self, d_rgba, w, h, row_pitch=None, stream=None):
         """Running-mean image into device memory (float4 texels)."""

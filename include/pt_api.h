@@ -65,6 +65,39 @@ typedef struct pt_triangle_info {
     float specularity;
 } pt_triangle_info;
 
+/* Per-triangle surface detail (reference: the Vertex fields and Material
+ * texture indices of CUDA::TriangleInfo, TriangleInfo.h:14-29, read by
+ * extractInfo / unpackNormal, cuda_path_tracer.cu:85-161).  With a detail
+ * record a hit's albedo is the albedo texture sampled at the interpolated uv
+ * (rgb), or else normalize(interpolated vertex colour) (computeColor,
+ * TriangleInfo.h:39-42); its normal is normalize(interpolated vertex normal),
+ * and with a normal map n = 2 * texel - 1 gives
+ * normalize(n.x * T + n.y * B + n.z * N) with T, B the normalised
+ * interpolated tangent and bitangent.  Interpolation is the reference's
+ * barycentricLerp, v0 * (1 - u - v) + v1 * u + v2 * v (math_util.h:19-25). */
+#define PT_NO_TEXTURE 0xFFFFFFFFu
+typedef struct pt_triangle_detail {
+    float color[3][3];     /* vertex colours */
+    float uv[3][2];        /* vertex texture coordinates */
+    float tangent[3][3];
+    float bitangent[3][3];
+    uint32_t albedo_tex;   /* texture index, or PT_NO_TEXTURE: vertex colours */
+    uint32_t normal_tex;   /* texture index, or PT_NO_TEXTURE: no normal map */
+    uint32_t reserved;     /* zero */
+} pt_triangle_detail;      /* 144 bytes */
+
+/* An RGBA8 texture, texels row by row starting at t = 0 (the bottom row, as
+ * the reference's GL upload stores images after its Invert_Y flip,
+ * Texture2D.cpp:139-145).  Sampling (tex2D on the reference's texture
+ * objects, Scene.cu:57-79: normalised coordinates, wrap, bilinear, level 0;
+ * 8-bit texels read as c / 255): x = s * width - 0.5, y = t * height - 0.5,
+ * weights the fractions of x and y in fp32 (the hardware's are 8-bit fixed
+ * point, so the reference differs in the last bits of a filtered texel). */
+typedef struct pt_texture {
+    uint32_t width, height;
+    const uint8_t* rgba;
+} pt_texture;
+
 /* Directional light (reference: CUDA::DirectionalLight, Lights.h:6-11);
  * direction points from the light into the scene; at most 64 per scene.
  * Primary hits get Blinn-Phong from every light, bounce hits its cosine
@@ -110,6 +143,16 @@ int pt_upload_scene(pt_context* ctx, const pt_triangle* tris, uint32_t n,
                     const pt_dir_light* lights, uint32_t nlights,
                     const float* bmin, const float* bmax,
                     const pt_settings* settings);
+/* Surface detail for the uploaded scene(s) (reference: CUDA::Scene::upload
+ * of the texture arrays and the TriangleInfo vertex data, Scene.cu:57-79,
+ * PathTracingApp.cpp:59-93): detail[k] belongs to the triangle whose id == k
+ * (n >= max id + 1) and replaces that triangle's pt_triangle_info albedo and
+ * normal in shading (emission and specularity stay in pt_triangle_info).
+ * Texture indices must be below ntextures; every texture at most 16384 x
+ * 16384.  The data is copied; n == 0 removes the detail (shading uses
+ * pt_triangle_info alone again).  A new pt_upload_scene(s) also removes it. */
+int pt_upload_detail(pt_context* ctx, const pt_triangle_detail* detail, uint32_t n,
+                     const pt_texture* textures, uint32_t ntextures);
 /* Batched scenes (no reference counterpart; the reference builds one scene
  * per BVHBuilder::build call): scene s owns triangles
  * [scene_offsets[s], scene_offsets[s+1]) and gets its own LBVH, identical to

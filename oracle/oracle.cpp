@@ -277,13 +277,70 @@ struct ShadeInfo {
     V3 normal, albedo, emission;
 };
 
+// Surface detail of a scene (pt_triangle_detail per original id, textures);
+// empty: pt_triangle_info alone.
+struct Detail {
+    const pt_triangle_detail* det = nullptr;
+    const pt_texture* tex = nullptr;
+    uint32_t ntex = 0;
+};
+
+// tex2D on the reference's texture objects (Scene.cu:66-73: normalised
+// coordinates, wrap, bilinear, level 0; texels c / 255), with fp32 weights:
+// x = s * width - 0.5, y = t * height - 0.5, glm::mix along x then y.
+V3 tex_sample(const pt_texture& tx, float s, float t) {
+    const uint32_t W = tx.width, H = tx.height;
+    float x = s * static_cast<float>(W) - 0.5f;
+    float y = t * static_cast<float>(H) - 0.5f;
+    if (!(std::fabs(x) < 1e9f)) x = 0.0f;
+    if (!(std::fabs(y) < 1e9f)) y = 0.0f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float a = x - fx, b = y - fy;
+    int64_t i0 = static_cast<int64_t>(fx) % static_cast<int64_t>(W);
+    int64_t j0 = static_cast<int64_t>(fy) % static_cast<int64_t>(H);
+    if (i0 < 0) i0 += W;
+    if (j0 < 0) j0 += H;
+    const int64_t i1 = (i0 + 1) % W, j1 = (j0 + 1) % H;
+    auto texel = [&](int64_t i, int64_t j) {
+        const uint8_t* c = tx.rgba + 4 * (static_cast<size_t>(j) * W + static_cast<size_t>(i));
+        return V3{static_cast<float>(c[0]) / 255.0f, static_cast<float>(c[1]) / 255.0f,
+                  static_cast<float>(c[2]) / 255.0f};
+    };
+    const V3 lo = mix(texel(i0, j0), texel(i1, j0), a);
+    const V3 hi = mix(texel(i0, j1), texel(i1, j1), a);
+    return mix(lo, hi, b);
+}
+
+// barycentricLerp (math_util.h:19-25): v0 * (1 - u - v) + v1 * u + v2 * v.
+inline V3 bary3(const float (*v)[3], float w, float u, float t) {
+    return add(add(scale(vld(v[0]), w), scale(vld(v[1]), u)), scale(vld(v[2]), t));
+}
+
 // extractInfo (cuda_path_tracer.cu:115-138): interpolated, normalised vertex
 // normal (TriangleInfo.h:44-47 via barycentricLerp, math_util.h:19-25) and
-// the albedo as given.
-inline ShadeInfo extract(const pt_triangle_info& in, float u, float v) {
+// the albedo as given; with surface detail, the albedo texture at the
+// interpolated uv or normalize(interpolated vertex colour) (computeColor,
+// TriangleInfo.h:39-42), and the normal map through the interpolated tangent
+// frame (unpackNormal, cuda_path_tracer.cu:85-104).
+inline ShadeInfo extract(const pt_triangle_info& in, uint32_t id, const Detail& dt, float u, float v) {
     float w = (1.0f - u) - v;
-    V3 nrm = add(add(scale(vld(in.n0), w), scale(vld(in.n1), u)), scale(vld(in.n2), v));
-    return {normalize(nrm), vld(in.albedo), vld(in.emission)};
+    V3 nrm = normalize(add(add(scale(vld(in.n0), w), scale(vld(in.n1), u)), scale(vld(in.n2), v)));
+    V3 albedo = vld(in.albedo);
+    if (dt.det) {
+        const pt_triangle_detail& d = dt.det[id];
+        const float s = ((d.uv[0][0] * w) + (d.uv[1][0] * u)) + (d.uv[2][0] * v);
+        const float t = ((d.uv[0][1] * w) + (d.uv[1][1] * u)) + (d.uv[2][1] * v);
+        albedo = d.albedo_tex != PT_NO_TEXTURE ? tex_sample(dt.tex[d.albedo_tex], s, t)
+                                               : normalize(bary3(d.color, w, u, v));
+        if (d.normal_tex != PT_NO_TEXTURE) {
+            const V3 ns = tex_sample(dt.tex[d.normal_tex], s, t);
+            const float nx = 2.0f * ns.x - 1.0f, ny = 2.0f * ns.y - 1.0f, nz = 2.0f * ns.z - 1.0f;
+            const V3 tg = normalize(bary3(d.tangent, w, u, v));
+            const V3 bt = normalize(bary3(d.bitangent, w, u, v));
+            nrm = normalize(add(add(scale(tg, nx), scale(bt, ny)), scale(nrm, nz)));
+        }
+    }
+    return {nrm, albedo, vld(in.emission)};
 }
 
 inline bool emissive(V3 e) { return ((e.x + e.y) + e.z) > 1e-7f; }  // CUDA_EPSILON
@@ -656,6 +713,18 @@ void or_render_pass_lit(const pt_triangle* tris, const uint32_t* perm, const uin
                         const pt_dir_light* lights, uint32_t nlights, const pt_settings* st, int w,
                         int h, const float* dirs, const float* eye, uint32_t pass, uint64_t seed,
                         double* accum, uint64_t* seg_counts, int threads) {
+    or_render_pass_detail(tris, perm, children, boxes, n, infos, nullptr, nullptr, 0, lights, nlights, st, w,
+                          h, dirs, eye, pass, seed, accum, seg_counts, threads);
+}
+
+void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                           const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                           const pt_triangle_detail* detail, const pt_texture* textures,
+                           uint32_t ntextures, const pt_dir_light* lights, uint32_t nlights,
+                           const pt_settings* st, int w, int h, const float* dirs, const float* eye,
+                           uint32_t pass, uint64_t seed, double* accum, uint64_t* seg_counts,
+                           int threads) {
+    const Detail dt{detail, textures, ntextures};
     Bvh bvh{tris, perm, children, boxes, n};
     const V3 d00 = vld(dirs), d10 = vld(dirs + 3), d11 = vld(dirs + 6), d01 = vld(dirs + 9);
     const V3 e = vld(eye);
@@ -682,7 +751,7 @@ void or_render_pass_lit(const pt_triangle* tris, const uint32_t* perm, const uin
             // blinnPhongShadingShadowTrace (cuda_path_tracer.cu:176-199) with no
             // directional lights: emission or zero.
             V3 pos = add(scale(d, hit.t), o);
-            ShadeInfo si = extract(infos[hit.id], hit.u, hit.v);
+            ShadeInfo si = extract(infos[hit.id], hit.id, dt, hit.u, hit.v);
             V3 albedo = si.albedo;
             V3 nrm = si.normal;
             if (emissive(si.emission)) {
@@ -722,7 +791,7 @@ void or_render_pass_lit(const pt_triangle* tris, const uint32_t* perm, const uin
                 hit = trace_one(bvh, o, d, nullptr);
                 if (hit.id == kInvalid) break;
                 pos = add(scale(d, hit.t), o);
-                si = extract(infos[hit.id], hit.u, hit.v);
+                si = extract(infos[hit.id], hit.id, dt, hit.u, hit.v);
                 nrm = si.normal;
                 albedo = si.albedo;
                 V3 L{0.0f, 0.0f, 0.0f};
@@ -763,6 +832,20 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
                           uint32_t pass, uint64_t seed, const float* params, const float* cbmin,
                           const float* cbmax, uint32_t ntrain, uint32_t extra_bounces, double* accum,
                           float* train_raw, float* train_target, int threads) {
+    or_render_cache_pass_detail(tris, perm, children, boxes, n, infos, nullptr, nullptr, 0, lights, nlights,
+                                st, w, h, dirs, eye, pass, seed, params, cbmin, cbmax, ntrain,
+                                extra_bounces, accum, train_raw, train_target, threads);
+}
+
+void or_render_cache_pass_detail(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                                 const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                                 const pt_triangle_detail* detail, const pt_texture* textures,
+                                 uint32_t ntextures, const pt_dir_light* lights, uint32_t nlights,
+                                 const pt_settings* st, int w, int h, const float* dirs, const float* eye,
+                                 uint32_t pass, uint64_t seed, const float* params, const float* cbmin,
+                                 const float* cbmax, uint32_t ntrain, uint32_t extra_bounces,
+                                 double* accum, float* train_raw, float* train_target, int threads) {
+    const Detail dt{detail, textures, ntextures};
     Bvh bvh{tris, perm, children, boxes, n};
     const V3 d00 = vld(dirs), d10 = vld(dirs + 3), d11 = vld(dirs + 6), d01 = vld(dirs + 9);
     const V3 e = vld(eye);
@@ -790,7 +873,7 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
         Hit hit = trace_one(bvh, o, d, nullptr);
         if (hit.id == kInvalid) return false;
         V3 pos = add(scale(d, hit.t), o);
-        ShadeInfo si = extract(infos[hit.id], hit.u, hit.v);
+        ShadeInfo si = extract(infos[hit.id], hit.id, dt, hit.u, hit.v);
         V3 albedo = si.albedo;
         V3 nrm = si.normal;
         if (emissive(si.emission)) {
@@ -824,7 +907,7 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
             hit = trace_one(bvh, o, d, nullptr);
             if (hit.id == kInvalid) break;
             pos = add(scale(d, hit.t), o);
-            si = extract(infos[hit.id], hit.u, hit.v);
+            si = extract(infos[hit.id], hit.id, dt, hit.u, hit.v);
             nrm = si.normal;
             albedo = si.albedo;
             V3 L{0.0f, 0.0f, 0.0f};
@@ -900,5 +983,25 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
 }
 
 float or_pow_pos(float x, float y) { return pow_pos(x, y); }
+
+void or_tex_sample(const pt_texture* tex, float s, float t, float* rgb) {
+    const V3 c = tex_sample(*tex, s, t);
+    rgb[0] = c.x;
+    rgb[1] = c.y;
+    rgb[2] = c.z;
+}
+
+void or_shade_detail(const pt_triangle_info* info, const pt_triangle_detail* detail,
+                     const pt_texture* textures, uint32_t ntextures, float u, float v, float* normal,
+                     float* albedo) {
+    const Detail dt{detail, textures, ntextures};
+    const ShadeInfo si = extract(*info, 0, dt, u, v);
+    normal[0] = si.normal.x;
+    normal[1] = si.normal.y;
+    normal[2] = si.normal.z;
+    albedo[0] = si.albedo.x;
+    albedo[1] = si.albedo.y;
+    albedo[2] = si.albedo.z;
+}
 
 }  // extern "C"

@@ -96,6 +96,23 @@ void or_render_pass_lit(const pt_triangle* tris, const uint32_t* perm, const uin
                         const pt_dir_light* lights, uint32_t nlights, const pt_settings* settings,
                         int w, int h, const float* dirs, const float* eye, uint32_t pass,
                         uint64_t seed, double* accum, uint64_t* seg_counts, int threads);
+/* The same pass with surface detail (pt_triangle_detail per original id,
+ * textures): extractInfo / unpackNormal, cuda_path_tracer.cu:85-161.
+ * detail NULL: or_render_pass_lit. */
+void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                           const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                           const pt_triangle_detail* detail, const pt_texture* textures,
+                           uint32_t ntextures, const pt_dir_light* lights, uint32_t nlights,
+                           const pt_settings* settings, int w, int h, const float* dirs,
+                           const float* eye, uint32_t pass, uint64_t seed, double* accum,
+                           uint64_t* seg_counts, int threads);
+/* tex2D as pt_texture documents it (bilinear, wrap, level 0), rgb[3]. */
+void or_tex_sample(const pt_texture* tex, float s, float t, float* rgb);
+/* Shading normal and albedo of one hit (u, v barycentrics) on a triangle
+ * with `info` and (if not NULL) detail[0]. */
+void or_shade_detail(const pt_triangle_info* info, const pt_triangle_detail* detail,
+                     const pt_texture* textures, uint32_t ntextures, float u, float v, float* normal,
+                     float* albedo);
 /* x^y for x >= 0, y > 0 as the specular power is computed (own function). */
 float or_pow_pos(float x, float y);
 
@@ -112,6 +129,17 @@ void or_render_cache_pass(const pt_triangle* tris, const uint32_t* perm, const u
                           const float* cbmin, const float* cbmax, uint32_t ntrain,
                           uint32_t extra_bounces, double* accum, float* train_raw,
                           float* train_target, int threads);
+
+/* The cache frame with surface detail (detail NULL: or_render_cache_pass). */
+void or_render_cache_pass_detail(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                                 const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                                 const pt_triangle_detail* detail, const pt_texture* textures,
+                                 uint32_t ntextures, const pt_dir_light* lights, uint32_t nlights,
+                                 const pt_settings* settings, int w, int h, const float* dirs,
+                                 const float* eye, uint32_t pass, uint64_t seed, const float* params,
+                                 const float* cbmin, const float* cbmax, uint32_t ntrain,
+                                 uint32_t extra_bounces, double* accum, float* train_raw,
+                                 float* train_target, int threads);
 
 /* Sampling helpers shared by the render, exposed for tests. */
 void or_pcg4d(const uint32_t* in, uint32_t* out);

@@ -38,6 +38,12 @@ _SIGS = {
     "or_render_pass_lit": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp, _vp,
                             _i], None),
     "or_pow_pos": ([_f, _f], _f),
+    "or_render_pass_detail": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _i, _i, _vp, _vp,
+                               _u32, _u64, _vp, _vp, _i], None),
+    "or_render_cache_pass_detail": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _i, _i, _vp,
+                                     _vp, _u32, _u64, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i], None),
+    "or_tex_sample": ([_vp, _f, _f, _vp], None),
+    "or_shade_detail": ([_vp, _vp, _vp, _u32, _f, _f, _vp, _vp], None),
     "or_pcg4d": ([_vp, _vp], None),
     "or_sincos2pi": ([_f, _vp, _vp], None),
     "or_to_hemisphere": ([_vp, _vp, _vp], None),
@@ -230,12 +236,56 @@ def render_pass_lit(tris, bvh, infos, lights, settings, w, h, dirs, eye, pass_in
     return accum, int(segs[0])
 
 
+def _tex(textures):
+    import ptamd
+    arr, keep = ptamd.texture_array(textures or [])
+    return (ctypes.cast(arr, ctypes.c_void_p).value if keep else None), len(keep), (arr, keep)
+
+
+def render_pass_detail(tris, bvh, infos, detail, textures, settings, w, h, dirs, eye, pass_index, seed,
+                       accum=None, lights=None, threads=THREADS):
+    """or_render_pass with surface detail (DETAIL_DTYPE per id, (h, w, 4) uint8 textures)."""
+    if accum is None:
+        accum = np.zeros((h, w, 3), np.float64)
+    la, nl = _lights(lights)
+    tp, nt, keep = _tex(textures)
+    segs = np.zeros(1, np.uint64)
+    dirs, eye = _f32(dirs), _f32(eye)
+    _lib.or_render_pass_detail(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
+                               _p(infos), _p(detail), tp, nt, _p(la), nl, ctypes.byref(settings), w, h, _p(dirs),
+                               _p(eye), pass_index, seed, _p(accum), _p(segs), threads)
+    del keep
+    return accum, int(segs[0])
+
+
+def tex_sample(image, s, t):
+    """tex2D of one (h, w, 4) uint8 texture at (s, t): rgb float32."""
+    tp, nt, keep = _tex([image])
+    out = np.zeros(3, np.float32)
+    _lib.or_tex_sample(tp, float(np.float32(s)), float(np.float32(t)), _p(out))
+    del keep
+    return out
+
+
+def shade_detail(info, detail, textures, u, v):
+    """(normal, albedo) of a hit at barycentrics (u, v)."""
+    tp, nt, keep = _tex(textures)
+    info = np.ascontiguousarray(info)
+    detail = None if detail is None else np.ascontiguousarray(detail)
+    n = np.zeros(3, np.float32)
+    a = np.zeros(3, np.float32)
+    _lib.or_shade_detail(_p(info), _p(detail), tp, nt, float(np.float32(u)), float(np.float32(v)), _p(n), _p(a))
+    del keep
+    return n, a
+
+
 def pow_pos(x, y):
     return float(_lib.or_pow_pos(float(np.float32(x)), float(np.float32(y))))
 
 
 def render_cache_pass(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, seed, params, cbmin, cbmax,
-                      ntrain, extra_bounces, accum=None, threads=THREADS, lights=None):
+                      ntrain, extra_bounces, accum=None, threads=THREADS, lights=None, detail=None,
+                      textures=None):
     """One cache frame: returns (accum, train_raw (ntrain, 12), train_target (ntrain, 3))."""
     if accum is None:
         accum = np.zeros((h, w, 3), np.float64)
@@ -244,9 +294,11 @@ def render_cache_pass(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, s
     dirs, eye, params = _f32(dirs), _f32(eye), _f32(params)
     cbmin, cbmax = _f32(cbmin), _f32(cbmax)
     la, nl = _lights(lights)
-    _lib.or_render_cache_pass(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
-                              _p(infos), _p(la), nl, ctypes.byref(settings), w, h, _p(dirs), _p(eye),
-                              pass_index, seed,
-                              _p(params), _p(cbmin), _p(cbmax), ntrain, extra_bounces, _p(accum), _p(raw),
-                              _p(tgt), threads)
+    tp, nt, keep = _tex(textures)
+    _lib.or_render_cache_pass_detail(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]),
+                                     len(tris), _p(infos), _p(detail), tp, nt, _p(la), nl, ctypes.byref(settings),
+                                     w, h, _p(dirs), _p(eye), pass_index, seed,
+                                     _p(params), _p(cbmin), _p(cbmax), ntrain, extra_bounces, _p(accum),
+                                     _p(raw), _p(tgt), threads)
+    del keep
     return accum, raw, tgt

@@ -192,3 +192,34 @@ def test_cache_frame_with_lights_matches_oracle(ctx):
     acc_g, _ = ctx.read_accumulator(W, H)
     assert max(rel_err(acc_g, acc_o)) <= 1e-3
 
+
+
+def test_cache_frame_with_detail_matches_oracle(ctx, room):
+    """Surface detail in cache frames: the training batch (whose albedo and
+    normal come from textures, vertex colours and normal maps) is bit-identical
+    to the oracle's, the image within the fp16 tolerance."""
+    from test_gpu_render import random_detail
+    st = ptamd.Settings(8, 3, 1.0, 1)
+    bmin, bmax = setup(ctx, room, st)
+    tris, infos = room
+    det, texs = random_detail(len(infos), 9)
+    ctx.upload_detail(det, texs)
+    ctx.cache_init(5, bmin, bmax)
+    p = random_params(29, bias=0.2)
+    ctx.cache_set_params(p)
+    ntrain, extra = 900, 5
+    say("[cache frame] room with surface detail")
+    ctx.render_cache_frame(W, H, 777, ptamd.CacheSettings(ntrain, 1, extra, train=False))
+    raw, tgt = ctx.cache_last_batch(ntrain)
+    bvh = orc.build(tris, bmin, bmax)
+    acc_o, oraw, otgt = orc.render_cache_pass(tris, bvh, infos, st, W, H, dirs_for(W, H), EYE, 0, 777,
+                                              fp16_weights(p), bmin, bmax, ntrain, extra, detail=det,
+                                              textures=texs)
+    valid = ~np.isnan(otgt[:, 0])
+    assert valid.sum() > ntrain // 2
+    np.testing.assert_array_equal(np.isnan(tgt[:, 0]), ~valid)
+    np.testing.assert_array_equal(raw[valid], oraw[valid])
+    np.testing.assert_array_equal(tgt[valid], otgt[valid])
+    acc_g, _ = ctx.read_accumulator(W, H)
+    assert max(rel_err(acc_g, acc_o)) <= 1e-3
+    ctx.upload_detail(np.zeros(0, ptamd.DETAIL_DTYPE))

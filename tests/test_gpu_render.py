@@ -283,3 +283,68 @@ def test_screen_buffer_and_device_image(ctx, scene):
         ctx.set_screen_buffer(None)
     ctx.render_pass(W, H, 999)  # detached: nothing written
     ctx.synchronize()
+
+
+def random_detail(n, seed):
+    rng = np.random.default_rng(seed)
+    det = np.zeros(n, ptamd.DETAIL_DTYPE)
+    det["color"] = rng.uniform(0.05, 1.0, (n, 3, 3))
+    det["uv"] = rng.uniform(-2.0, 3.0, (n, 3, 2))
+    det["tangent"] = rng.normal(size=(n, 3, 3))
+    det["bitangent"] = rng.normal(size=(n, 3, 3))
+    det["albedo_tex"] = np.where(rng.random(n) < 0.6, rng.integers(0, 3, n), ptamd.NO_TEXTURE)
+    det["normal_tex"] = np.where(rng.random(n) < 0.5, rng.integers(0, 3, n), ptamd.NO_TEXTURE)
+    texs = [rng.integers(0, 256, (8, 16, 4), dtype=np.uint8), rng.integers(0, 256, (1, 1, 4), dtype=np.uint8),
+            rng.integers(0, 256, (32, 32, 4), dtype=np.uint8)]
+    return det, texs
+
+
+@pytest.mark.parametrize("graphs", [1, 0])
+def test_render_with_detail_matches_oracle(ctx, scene, graphs):
+    """Textures, vertex colours and normal maps (extractInfo / unpackNormal):
+    accumulators bit-identical to the oracle, with directional lights."""
+    tris, infos, st = scene
+    det, texs = random_detail(len(infos), 5)
+    from test_oracle_lights import light
+    lights = np.array([light((1.0, 0.9, 0.8), (0.3, -1.0, 0.2), 1.5)], np.float32)
+    ctx.upload_scene(tris, infos, settings=st, lights=lights)
+    ctx.upload_detail(det, texs)
+    ptamd.tune_set("render.graphs", graphs)
+    try:
+        seeds = np.arange(40, 43, dtype=np.uint64)
+        dirs = np.stack([pass_dirs(p) for p in range(3)])
+        ctx.reset_frames()
+        say(f"[detail] 3 passes, graphs={graphs}")
+        ctx.render_passes(W, H, seeds, dirs)
+        acc, spp = ctx.read_accumulator(W, H)
+    finally:
+        ptamd.tune_set("render.graphs", 1)
+    b = orc.build(tris, *ptamd.scene_bounds(tris))
+    o = np.zeros((H, W, 3))
+    for p in range(3):
+        orc.render_pass_detail(tris, b, infos, det, texs, st, W, H, pass_dirs(p), EYE, p, int(seeds[p]), o,
+                               lights=lights)
+    assert spp == 3 and np.array_equal(acc, o)
+    plain = np.zeros((H, W, 3))
+    for p in range(3):
+        orc.render_pass_lit(tris, b, infos, lights, st, W, H, pass_dirs(p), EYE, p, int(seeds[p]), plain)
+    assert not np.array_equal(plain, o)  # the detail changes the image
+    # removing the detail restores the plain shading
+    ctx.upload_detail(np.zeros(0, ptamd.DETAIL_DTYPE))
+    ctx.reset_frames()
+    ctx.render_passes(W, H, seeds, dirs)
+    assert np.array_equal(ctx.read_accumulator(W, H)[0], plain)
+
+
+def test_detail_errors(ctx, scene):
+    tris, infos, st = scene
+    ctx.upload_scene(tris, infos, settings=st)
+    det, texs = random_detail(len(infos), 6)
+    with pytest.raises(ptamd.PtError):
+        ctx.upload_detail(det[:-1], texs)  # fewer records than ids
+    with pytest.raises(ptamd.PtError):
+        ctx.upload_detail(det, texs[:2])  # index 2 out of range
+    bad = det.copy()
+    bad["albedo_tex"][0] = 7
+    with pytest.raises(ptamd.PtError):
+        ctx.upload_detail(bad, texs)
