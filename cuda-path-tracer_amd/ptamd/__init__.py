@@ -145,6 +145,12 @@ _SIGS = {
     "pt_cache_last_batch": [_vp, _vp, _vp, _u32],
     "pt_cache_last_queries": [_vp, _vp, _vp, _vp],
     "pt_load_obj": [ctypes.c_char_p, _vp, _vp, _vp, _vp],
+    "pt_asset_load_obj": [ctypes.c_char_p, _vp, _vp],
+    "pt_asset_info": [_vp, _vp, _vp],
+    "pt_asset_copy": [_vp, _vp, _vp, _vp],
+    "pt_asset_textures": [_vp, _vp],
+    "pt_asset_free": [_vp],
+    "pt_load_tga": [ctypes.c_char_p, _vp, _vp, _vp],
     "pt_asset_error": [ctypes.c_char_p, ctypes.c_size_t],
     "pt_tune_set": [ctypes.c_char_p, _u32],
     "pt_tune_get": [ctypes.c_char_p, _vp],
@@ -283,6 +289,53 @@ def load_obj(path, left_handed=True, normalize_albedo=True):
         _lib.pt_asset_error(buf, len(buf))
         raise PtError(rc, buf.value.decode(errors="replace"))
     return tris, infos
+
+
+def _asset_error(rc):
+    buf = ctypes.create_string_buffer(512)
+    _lib.pt_asset_error(buf, len(buf))
+    raise PtError(rc, buf.value.decode(errors="replace"))
+
+
+def load_obj_asset(path, left_handed=True, normalize_albedo=True):
+    """OBJ/MTL with surface detail -> (tris, infos, detail, textures), the
+    textures as (h, w, 4) uint8 arrays (row 0 = bottom)."""
+    opt = ObjOptions(1 if left_handed else 0, 1 if normalize_albedo else 0)
+    h = ctypes.c_void_p()
+    rc = _lib.pt_asset_load_obj(os.fsencode(path), ctypes.byref(opt), ctypes.byref(h))
+    if rc != PT_OK:
+        _asset_error(rc)
+    try:
+        n = np.zeros(2, np.uint32)
+        _check(_lib.pt_asset_info(h, _ptr(n[:1]), n[1:].ctypes.data))
+        tris = np.zeros(int(n[0]), TRI_DTYPE)
+        infos = np.zeros(int(n[0]), INFO_DTYPE)
+        detail = np.zeros(int(n[0]), DETAIL_DTYPE)
+        _check(_lib.pt_asset_copy(h, _ptr(tris), _ptr(infos), _ptr(detail)))
+        arr = (Texture * max(1, int(n[1])))()
+        _check(_lib.pt_asset_textures(h, ctypes.cast(arr, ctypes.c_void_p)))
+        textures = []
+        for k in range(int(n[1])):
+            t = arr[k]
+            buf = (ctypes.c_uint8 * (4 * t.width * t.height)).from_address(t.rgba)
+            textures.append(np.frombuffer(buf, np.uint8).reshape(t.height, t.width, 4).copy())
+    finally:
+        _lib.pt_asset_free(h)
+    return tris, infos, detail, textures
+
+
+def load_tga(path):
+    """TGA file -> (h, w, 4) uint8 RGBA, row 0 = bottom."""
+    wh = np.zeros(2, np.uint32)
+    p = os.fsencode(path)
+    rc = _lib.pt_load_tga(p, _ptr(wh[:1]), wh[1:].ctypes.data, None)
+    if rc != PT_OK:
+        _asset_error(rc)
+    img = np.zeros((int(wh[1]), int(wh[0]), 4), np.uint8)
+    rc = _lib.pt_load_tga(p, _ptr(wh[:1]), wh[1:].ctypes.data, _ptr(img))
+    if rc != PT_OK:
+        _asset_error(rc)
+    return img
 
 
 PT_PNG_SRGB = 1

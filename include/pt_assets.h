@@ -40,6 +40,31 @@ typedef struct pt_obj_options {
 int pt_load_obj(const char* path, const pt_obj_options* options, pt_triangle* tris,
                 pt_triangle_info* infos, uint32_t* count);
 
+/* Full asset import with surface detail (PathTracingApp.cpp:59-93): a handle
+ * owning the triangles, infos, pt_triangle_detail records (vertex colour from
+ * a `v x y z r g b` line or else the material's Kd; uv from vt, else 0;
+ * a per-face tangent frame from the uv derivatives, T along +u and B along
+ * +v, or normalize(edge 1) and cross(N, T) where the uvs are degenerate) and
+ * the textures the materials name: map_Kd as the albedo texture
+ * (aiTextureType_DIFFUSE, AssetImporter.cpp:92), map_bump / bump as the
+ * normal map (aiTextureType_HEIGHT, :93).  Textures are TGA files (types 2,
+ * 3, 10, 11; 8, 24 or 32 bits; the format of the reference's Sponza
+ * textures), stored bottom row first as pt_texture describes; other formats
+ * fail the load.  Texture paths are relative to the MTL file, '\' read as
+ * '/'.  pt_asset_textures' texel pointers live as long as the handle. */
+typedef struct pt_asset pt_asset;
+int pt_asset_load_obj(const char* path, const pt_obj_options* options, pt_asset** out);
+int pt_asset_info(const pt_asset* asset, uint32_t* ntris, uint32_t* ntextures);
+/* Any output may be NULL; each non-NULL one holds ntris records. */
+int pt_asset_copy(const pt_asset* asset, pt_triangle* tris, pt_triangle_info* infos,
+                  pt_triangle_detail* detail);
+int pt_asset_textures(const pt_asset* asset, pt_texture* textures);
+int pt_asset_free(pt_asset* asset);
+
+/* One TGA file as RGBA8, bottom row first.  Two calls: rgba NULL fills *w,
+ * *h; then rgba holds 4 * w * h bytes (w, h as returned). */
+int pt_load_tga(const char* path, uint32_t* w, uint32_t* h, uint8_t* rgba);
+
 /* Image output (host only).  Replaces the reference's on-screen display of
  * its RGBA32F screen texture (PathTracingApp.cpp:282-290) for runs without a
  * window: writes an 8-bit RGB PNG of a running-mean image such as
@@ -52,7 +77,8 @@ int pt_load_obj(const char* path, const pt_obj_options* options, pt_triangle* tr
 #define PT_PNG_TOP_DOWN 2u
 int pt_write_png(const char* path, const float* rgba, uint32_t w, uint32_t h, uint32_t flags);
 
-/* The calling thread's last pt_load_obj / pt_write_png error message. */
+/* The calling thread's last pt_load_obj / pt_asset_load_obj / pt_load_tga /
+ * pt_write_png error message. */
 int pt_asset_error(char* buf, size_t len);
 
 #ifdef __cplusplus

@@ -31,6 +31,13 @@ v 1 0 1
 v 0 0 1
 v 0 2 0
 vn 0 1 0
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+vt 0.5 0
+vt 0.5 1
+vt 0.25 0.75
 usemtl red
 f 1//1 4//1 3//1 2//1
 usemtl lamp
