@@ -226,6 +226,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched leg")
+    ap.add_argument("--no-hair", action="store_true", help="skip the 2.88M-triangle hairball leg")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache leg")
     ap.add_argument("--cache-full", action="store_true",
                     help="radiance-cache leg at 1920x1080 with 65,536 training samples per frame "
@@ -239,6 +240,7 @@ def main():
         sys.exit(2)
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.no_tune, args.no_batch = 3, 2, True, True, True
+        args.no_hair = True
         args.no_cache = True
 
     import numpy as np
@@ -278,7 +280,7 @@ def main():
     if not args.no_tune:
         from ptamd import autotune
 
-        best, table = autotune.tune_staged(step, stream, autotune.trace_stages(), reps=2)
+        best, table = autotune.tune_staged(step, stream, autotune.trace_stages(), reps=4)
         tuning = {"chosen": best, "candidates": len(table),
                   "best_ms": round(min(ms for _, ms in table), 4),
                   "worst_ms": round(max(ms for _, ms in table), 4)}
@@ -362,6 +364,60 @@ def main():
                         "segments_per_pass": int(frame_rays // per_frame),
                         "Msegments_per_s": round(frame_rays / per_frame * passes / (rms * 1e-3) / 1e6, 1)}
         log(f"[bench] render {name}: {rms / passes:.3f} ms/pass ({per_frame} passes/frame)")
+
+    # Config 2: 10,000-strand hairball (2,880,000 triangles, heavily
+    # overlapping boxes): build, the 16M-ray closest-hit microbench (same ray
+    # generator as config 1) with its algorithmic bytes, and a 1920x1080
+    # 6-bounce render.
+    hair = None
+    if not args.no_hair:
+        ht, hi = ptamd.gen_hairball(10000, 144, 2025, light=True)
+        ctx.upload_scene(ht, hi, settings=ptamd.Settings(6, ptamd.INVALID, 1.0, 1))
+        h_build = float(np.median([ctx.build_bvh() for _ in range(3)]))
+        hb0, hb1 = ptamd.scene_bounds(ht)
+        hn = n
+        hrays = hip.DeviceArray.from_host(ptamd.gen_shell_rays(hn, seed_rays + 1, hb0, hb1, 1.5))
+        hcnt = ctx.trace_closest_counted(hrays.ptr, hn, stream=stream.handle)
+        hbytes = 24 * hn + 8 * hn + 4 * int(hcnt[0]) + 24 * int(hcnt[1]) + 36 * int(hcnt[2])
+
+        def hstep():
+            ctx.trace_closest_device(hrays.ptr, hn, d_id.ptr, d_t.ptr, 0, stream.handle)
+
+        hstep()
+        stream.synchronize()
+        h0, h1 = hip.Event(), hip.Event()
+        h0.record(stream)
+        for _ in range(3):
+            hstep()
+        h1.record(stream)
+        h1.synchronize()
+        h_ms = h0.elapsed_ms(h1) / 3
+        hrays.free()
+        w2, h2, eye2, tgt2 = 1920, 1080, (0.0, 0.3, -3.2), (0.0, 0.0, 0.0)
+        ctx.set_primary_rays(ptamd.camera_corners(eye2, tgt2, (0, 1, 0), 60.0, w2 / h2), eye2)
+        hdirs = np.stack([ptamd.camera_corners(eye2, tgt2, (0, 1, 0), 60.0, w2 / h2, *ptamd.pass_jitter(6, p, w2, h2))
+                          for p in range(3)])
+        hseeds = np.arange(2000, 2003, dtype=np.uint64)
+        ctx.render_passes(w2, h2, hseeds[:1], hdirs[:1])  # warm-up (allocation, capture)
+        ctx.reset_frames()
+        h0.record(stream)
+        for p in range(1, 3):
+            ctx.render_passes(w2, h2, hseeds[p:p + 1], hdirs[p:p + 1])
+        h1.record(stream)
+        h1.synchronize()
+        hr_ms = h0.elapsed_ms(h1) / 2
+        hsegs = ctx.last_frame_rays()
+        hair = {"triangles": int(len(ht)), "build_ms": round(h_build, 3),
+                "closest_hit_Mrays_per_s": round(hn / (h_ms * 1e-3) / 1e6, 1), "closest_hit_ms": round(h_ms, 3),
+                "visits_per_ray": round(int(hcnt[0]) / hn, 3), "box_tests_per_ray": round(int(hcnt[1]) / hn, 3),
+                "tri_tests_per_ray": round(int(hcnt[2]) / hn, 3),
+                "algorithmic_GB_per_s": round(hbytes / (h_ms * 1e-3) / 1e9, 1),
+                "hbm_frac": round(hbytes / (h_ms * 1e-3) / 1e9 / 8000.0, 4),
+                "render_1920x1080_b6": {"ms_per_pass": round(hr_ms, 3), "spp_per_s": round(1e3 / hr_ms, 2),
+                                        "segments_per_pass": int(hsegs),
+                                        "Msegments_per_s": round(hsegs / (hr_ms * 1e-3) / 1e6, 1)}}
+        log(f"[bench] hairball ({len(ht)} tris): build {h_build:.3f} ms; closest hit {hair['closest_hit_Mrays_per_s']} "
+            f"Mrays/s ({hair['visits_per_ray']} visits/ray); render 1080p b6 {hr_ms:.2f} ms/pass")
 
     # Config 3: 512 soups (log-uniform 1,024..262,144 triangles), one batched
     # build vs one build per scene; one frame tracing every scene (a
@@ -496,6 +552,7 @@ def main():
         "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
         "render": render,
         "batched_scenes": batched,
+        "hairball": hair,
         "radiance_cache": cache,
         "host_buffers_Mrays_per_s": host_mrays,
         "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),

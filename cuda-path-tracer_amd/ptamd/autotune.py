@@ -74,7 +74,7 @@ def stack_space():
     (more entries, fewer spills to scratch, lower occupancy) and the rays a
     wave takes from the queue at a time."""
     return {"trace.lds_stack": [8, 16, 32],
-            "trace.pool": [64, 256, 1024, 4096]}
+            "trace.pool": [128, 256, 512, 1024]}
 
 
 def trace_stages(num_cus=256):
