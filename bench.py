@@ -508,13 +508,28 @@ def main():
              (1024, 1024), 1, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))):
             ob = orc.build(rt, *ptamd.scene_bounds(rt))
             acc = np.zeros((h, w, 3))
+            tot = {"segments": 0, "visits": 0, "boxes": 0, "tris": 0, "hits": 0}
             t1 = time.time()
             for p in range(passes):
                 d = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, *ptamd.pass_jitter(5, p, w, h))
-                orc.render_pass(rt, ob, ri, st, w, h, d, eye, p, 1000 + p, acc, threads=threads)
+                _, sts = orc.render_pass_stats(rt, ob, ri, st, w, h, d, eye, p, 1000 + p, acc, threads=threads)
+                for k in tot:
+                    tot[k] += sts[k]
             dt = time.time() - t1
             cpu_render[name] = {"ms_per_pass": round(dt * 1e3 / passes, 1), "spp_per_s": round(passes / dt, 3),
                                 "Msamples_per_s": round(w * h * passes / dt / 1e6, 3), "passes": passes}
+            # Algorithmic bytes of a pass (closest-hit bytes per segment with
+            # the reference-order traversal's counts, FP64 accumulate per
+            # pixel, shading fetch per hit), from the oracle's passes at the
+            # GPU legs' first seeds and jitters; roofline of the GPU pass.
+            pass_bytes = (32 * tot["segments"] + 4 * tot["visits"] + 24 * tot["boxes"] + 36 * tot["tris"]
+                          + 48 * w * h * passes + 24 * tot["hits"]) / passes
+            g = render.get(name)
+            if g:
+                gbs = pass_bytes / (g["ms_per_pass"] * 1e-3) / 1e9
+                g.update({"algorithmic_bytes_per_pass": int(pass_bytes), "algorithmic_GB_per_s": round(gbs, 1),
+                          "hbm_frac": round(gbs / 8000.0, 4),
+                          "bytes_from": f"oracle traversal counts of {passes} pass(es) at the same seeds"})
         cpu["other_metrics"] = {"lbvh_build_ms_soup_2^20": round(cpu_build_ms, 1), "render": cpu_render,
                                 "threads": threads}
         log(f"[bench] CPU oracle: LBVH build {cpu_build_ms:.0f} ms @ 2^20; render "

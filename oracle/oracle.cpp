@@ -717,14 +717,15 @@ void or_render_pass_lit(const pt_triangle* tris, const uint32_t* perm, const uin
                           h, dirs, eye, pass, seed, accum, seg_counts, threads);
 }
 
-void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
-                           const float* boxes, uint32_t n, const pt_triangle_info* infos,
-                           const pt_triangle_detail* detail, const pt_texture* textures,
-                           uint32_t ntextures, const pt_dir_light* lights, uint32_t nlights,
-                           const pt_settings* st, int w, int h, const float* dirs, const float* eye,
-                           uint32_t pass, uint64_t seed, double* accum, uint64_t* seg_counts,
-                           int threads) {
-    const Detail dt{detail, textures, ntextures};
+namespace {
+// One pass (or_render_pass_detail); stats (may be NULL) receives the traced
+// segments, their node visits, box tests and triangle tests (the
+// reference-order traversal's counts), and the segments that hit.
+void render_pass_impl(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                      const float* boxes, uint32_t n, const pt_triangle_info* infos, const Detail& dt,
+                      const pt_dir_light* lights, uint32_t nlights, const pt_settings* st, int w, int h,
+                      const float* dirs, const float* eye, uint32_t pass, uint64_t seed, double* accum,
+                      uint64_t* stats, int threads) {
     Bvh bvh{tris, perm, children, boxes, n};
     const V3 d00 = vld(dirs), d10 = vld(dirs + 3), d11 = vld(dirs + 6), d01 = vld(dirs + 9);
     const V3 e = vld(eye);
@@ -733,11 +734,12 @@ void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const 
     const uint32_t rr_start = st->rr_start;
     const float intensity = st->indirect_intensity;
     const bool strat = st->stratified != 0;
-    uint64_t segs = 0;
+    uint64_t segs = 0, iters = 0, nbox = 0, ntri = 0, hits = 0;
     if (threads <= 0) threads = 1;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) reduction(+ : segs)
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) reduction(+ : segs, iters, nbox, ntri, hits)
     for (int64_t pix = 0; pix < static_cast<int64_t>(w) * h; ++pix) {
         const int x = static_cast<int>(pix % w), y = static_cast<int>(pix / w);
+        Counters cnt;
         // Primary ray (cuda_path_tracer.cu:404-407, 376-379): bilinear blend of
         // the corner directions at the pixel corner (x/w, y/h), not renormalised.
         const float tx = static_cast<float>(x) / static_cast<float>(w);
@@ -746,7 +748,8 @@ void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const 
         V3 o = e;
         V3 accumc{0.0f, 0.0f, 0.0f};
         ++segs;
-        Hit hit = trace_one(bvh, o, d, nullptr);
+        Hit hit = trace_one(bvh, o, d, stats ? &cnt : nullptr);
+        hits += hit.id != kInvalid;
         if (hit.id != kInvalid) {
             // blinnPhongShadingShadowTrace (cuda_path_tracer.cu:176-199) with no
             // directional lights: emission or zero.
@@ -788,8 +791,9 @@ void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const 
                 V3 local{r * cs, r * sn, std::sqrt(1.0f - u1)};
                 d = to_hemisphere(local, nrm);
                 ++segs;
-                hit = trace_one(bvh, o, d, nullptr);
+                hit = trace_one(bvh, o, d, stats ? &cnt : nullptr);
                 if (hit.id == kInvalid) break;
+                ++hits;
                 pos = add(scale(d, hit.t), o);
                 si = extract(infos[hit.id], hit.id, dt, hit.u, hit.v);
                 nrm = si.normal;
@@ -811,8 +815,39 @@ void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const 
         a[0] += static_cast<double>(accumc.x);
         a[1] += static_cast<double>(accumc.y);
         a[2] += static_cast<double>(accumc.z);
+        iters += cnt.iters;
+        nbox += cnt.boxes;
+        ntri += cnt.tris;
     }
-    if (seg_counts) *seg_counts = segs;
+    if (stats) {
+        stats[0] = segs;
+        stats[1] = iters;
+        stats[2] = nbox;
+        stats[3] = ntri;
+        stats[4] = hits;
+    }
+}
+}  // namespace
+
+void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                           const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                           const pt_triangle_detail* detail, const pt_texture* textures,
+                           uint32_t ntextures, const pt_dir_light* lights, uint32_t nlights,
+                           const pt_settings* st, int w, int h, const float* dirs, const float* eye,
+                           uint32_t pass, uint64_t seed, double* accum, uint64_t* seg_counts,
+                           int threads) {
+    uint64_t stats[5];
+    render_pass_impl(tris, perm, children, boxes, n, infos, Detail{detail, textures, ntextures}, lights,
+                     nlights, st, w, h, dirs, eye, pass, seed, accum, seg_counts ? stats : nullptr, threads);
+    if (seg_counts) *seg_counts = stats[0];
+}
+
+void or_render_pass_stats(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                          const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                          const pt_settings* st, int w, int h, const float* dirs, const float* eye,
+                          uint32_t pass, uint64_t seed, double* accum, uint64_t* stats, int threads) {
+    render_pass_impl(tris, perm, children, boxes, n, infos, Detail{}, nullptr, 0, st, w, h, dirs, eye, pass,
+                     seed, accum, stats, threads);
 }
 
 // Learned-cache frame (the build's own addition; no reference counterpart).

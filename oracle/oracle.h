@@ -106,6 +106,14 @@ void or_render_pass_detail(const pt_triangle* tris, const uint32_t* perm, const 
                            const pt_settings* settings, int w, int h, const float* dirs,
                            const float* eye, uint32_t pass, uint64_t seed, double* accum,
                            uint64_t* seg_counts, int threads);
+/* or_render_pass with traversal statistics: stats[5] = traced segments,
+ * node visits, box tests, triangle tests (reference-order traversal) and
+ * segments that hit -- the algorithmic-byte counts of a render pass. */
+void or_render_pass_stats(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
+                          const float* boxes, uint32_t n, const pt_triangle_info* infos,
+                          const pt_settings* settings, int w, int h, const float* dirs,
+                          const float* eye, uint32_t pass, uint64_t seed, double* accum,
+                          uint64_t* stats, int threads);
 /* tex2D as pt_texture documents it (bilinear, wrap, level 0), rgb[3]. */
 void or_tex_sample(const pt_texture* tex, float s, float t, float* rgb);
 /* Shading normal and albedo of one hit (u, v barycentrics) on a triangle

@@ -43,6 +43,7 @@ _SIGS = {
     "or_render_cache_pass_detail": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _u32, _vp, _i, _i, _vp,
                                      _vp, _u32, _u64, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp, _i], None),
     "or_tex_sample": ([_vp, _f, _f, _vp], None),
+    "or_render_pass_stats": ([_vp, _vp, _vp, _vp, _u32, _vp, _vp, _i, _i, _vp, _vp, _u32, _u64, _vp, _vp, _i], None),
     "or_shade_detail": ([_vp, _vp, _vp, _u32, _f, _f, _vp, _vp], None),
     "or_pcg4d": ([_vp, _vp], None),
     "or_sincos2pi": ([_f, _vp, _vp], None),
@@ -256,6 +257,21 @@ def render_pass_detail(tris, bvh, infos, detail, textures, settings, w, h, dirs,
                                _p(eye), pass_index, seed, _p(accum), _p(segs), threads)
     del keep
     return accum, int(segs[0])
+
+
+def render_pass_stats(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, seed, accum=None,
+                      threads=THREADS):
+    """or_render_pass with traversal statistics: (accum, {segments, visits,
+    boxes, tris, hits})."""
+    if accum is None:
+        accum = np.zeros((h, w, 3), np.float64)
+    st = np.zeros(5, np.uint64)
+    dirs, eye = _f32(dirs), _f32(eye)
+    _lib.or_render_pass_stats(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
+                              _p(infos), ctypes.byref(settings), w, h, _p(dirs), _p(eye), pass_index, seed,
+                              _p(accum), _p(st), threads)
+    keys = ("segments", "visits", "boxes", "tris", "hits")
+    return accum, {k: int(v) for k, v in zip(keys, st)}
 
 
 def tex_sample(image, s, t):

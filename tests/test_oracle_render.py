@@ -140,3 +140,27 @@ def test_seeds_are_deterministic():
     assert a.sum() > 0
     assert np.array_equal(a, b)
     assert not np.array_equal(a, c)
+
+
+def test_render_stats_equal_trace_counts():
+    """A 0-bounce pass traces exactly the primary rays: its statistics equal
+    w * h segments and the closest-hit traversal's own counters over the
+    same rays (built here from the corner directions as the pass builds
+    them: pixel corner (x/w, y/h), glm::mix x + (y - x) a in fp32)."""
+    tris, infos = ptamd.gen_soup(4096, 11)
+    b = orc.build(tris, *ptamd.scene_bounds(tris))
+    w, h = 24, 16
+    eye = np.float32([0.5, 0.5, -1.5])
+    dirs = ptamd.camera_corners(tuple(eye), (0.5, 0.5, 0.0), (0, 1, 0), 60.0, w / h).reshape(4, 3)
+    st = ptamd.Settings(0, ptamd.INVALID, 1.0, 1)
+    _, sts = orc.render_pass_stats(tris, b, infos, st, w, h, dirs, eye, 0, 3)
+    mix = lambda x, y, a: x + (y - x) * a
+    y, x = np.mgrid[0:h, 0:w]
+    tx = (x.astype(np.float32) / np.float32(w))[..., None]
+    ty = (y.astype(np.float32) / np.float32(h))[..., None]
+    d = mix(mix(dirs[0], dirs[1], tx), mix(dirs[3], dirs[2], tx), ty).astype(np.float32).reshape(-1, 3)
+    rays = np.concatenate([np.broadcast_to(eye, d.shape), d], axis=1).astype(np.float32)
+    ids, _, _, c = orc.trace(tris, b, rays, counters=True)
+    assert sts["segments"] == w * h
+    assert (sts["visits"], sts["boxes"], sts["tris"]) == tuple(int(v) for v in c)
+    assert sts["hits"] == int((ids != ptamd.INVALID).sum()) > 0
