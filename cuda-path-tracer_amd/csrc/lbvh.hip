@@ -355,13 +355,25 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
     uint32_t level = 0;
     if ((lo / kRefitBlock) != (hi / kRefitBlock)) level = ((lo / kMidBlock) != (hi / kMidBlock)) ? 2u : 1u;
     counters[g] = level << kLevelShift;
-    if (level == 1) {
-        const uint32_t blk = static_cast<uint32_t>(lo / kMidBlock);
-        const uint32_t slot = atomicAdd(span_count + 1 + blk, 1u);
-        if (slot < kMidCap) span_list[static_cast<size_t>(blk) * kMidCap + slot] = g;
-    } else if (level == 2) {
-        span_top[atomicAdd(span_count, 1u)] = g;
+    // List appends, aggregated per wave: one atomic per distinct 65536-block
+    // among the wave's level-1 nodes (neighbouring nodes mostly share it).
+    const uint32_t blk = static_cast<uint32_t>(lo / kMidBlock);
+    const uint64_t lt = (1ull << (threadIdx.x & 63u)) - 1ull;
+    uint64_t pending = __ballot(level == 1);
+    while (pending) {
+        const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(pending));
+        const uint32_t b = __shfl(blk, static_cast<int>(lead), 64);
+        const uint64_t same = pending & __ballot(level == 1 && blk == b);
+        uint32_t first = 0;
+        if ((threadIdx.x & 63u) == lead) first = atomicAdd(span_count + 1 + b, static_cast<uint32_t>(__popcll(same)));
+        first = __shfl(first, static_cast<int>(lead), 64);
+        if ((same >> (threadIdx.x & 63u)) & 1ull) {
+            const uint32_t slot = first + static_cast<uint32_t>(__popcll(same & lt));
+            if (slot < kMidCap) span_list[static_cast<size_t>(b) * kMidCap + slot] = g;
+        }
+        pending &= ~same;
     }
+    if (level == 2) span_top[atomicAdd(span_count, 1u)] = g;
     if (i == 0) parents_internal[g] = kInvalid;  // BVHBuilder.cu:224
 }
 
