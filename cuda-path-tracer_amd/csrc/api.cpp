@@ -390,6 +390,35 @@ void present(pt_context* ctx) {
 }
 }  // namespace
 
+namespace {
+// CUDAPathTracer::raytrace's stop at a frame count (cuda_path_tracer.cu:428-441):
+// how many of `want` passes may still run; at the limit, records the time
+// since the first pass after the last reset.
+uint32_t frames_allowed(pt_context* ctx, uint32_t want) {
+    pt::Renderer& r = ctx->renderer;
+    if (r.spp == 0) r.t_start = std::chrono::steady_clock::now();
+    if (r.stop_at == 0) return want;
+    if (r.spp >= r.stop_at) {
+        if (r.gen_seconds < 0.0) {
+            PT_HIP(hipStreamSynchronize(ctx->stream));
+            r.gen_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - r.t_start).count();
+        }
+        return 0;
+    }
+    return std::min(want, r.stop_at - r.spp);
+}
+}  // namespace
+
+int pt_set_stop_frame(pt_context* ctx, uint32_t frames) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { ctx->renderer.stop_at = frames; });
+}
+
+int pt_generation_time(pt_context* ctx, double* seconds) {
+    if (!ctx || !seconds) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { *seconds = ctx->renderer.gen_seconds; });
+}
+
 int pt_set_screen_buffer(pt_context* ctx, void* d_rgba, uint32_t w, uint32_t h, size_t row_pitch) {
     if (!ctx) return PT_E_INVALID_ARG;
     return guarded(ctx, [&] {
@@ -436,6 +465,7 @@ int pt_render_pass(pt_context* ctx, int w, int h, uint64_t pass_seed) {
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
+        if (frames_allowed(ctx, 1) == 0) return;
         PT_HIP(hipEventRecord(ctx->ev0, st));
         pt::render_passes(ctx->renderer, ctx->scene, 1, nullptr, &pass_seed, nullptr, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
@@ -451,8 +481,10 @@ int pt_render_passes(pt_context* ctx, int w, int h, uint32_t npasses, const floa
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
+        const uint32_t run = npasses ? frames_allowed(ctx, npasses) : npasses;
+        if (npasses && run == 0) return;
         PT_HIP(hipEventRecord(ctx->ev0, st));
-        pt::render_passes(ctx->renderer, ctx->scene, npasses, dirs, seeds, scenes, st);
+        pt::render_passes(ctx->renderer, ctx->scene, run, dirs, seeds, scenes, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
         ctx->renderer.timed = true;
         present(ctx);
@@ -621,6 +653,7 @@ int pt_render_cache_frame(pt_context* ctx, int w, int h, uint64_t pass_seed,
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         hipStream_t st = ctx->stream;
         pt::render_resize(ctx->renderer, w, h, st);
+        if (frames_allowed(ctx, 1) == 0) return;
         PT_HIP(hipEventRecord(ctx->ev0, st));
         pt::render_cache_frame(ctx->renderer, ctx->scene, ctx->cache, pass_seed, *settings, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <utility>
 #include <stdexcept>
@@ -200,6 +201,9 @@ struct Renderer {
     uint32_t last_train = 0;   // training paths of the last cache frame
     void* cache_graph_exec = nullptr;  // hipGraphExec_t of the last cache frame
     alignas(8) char cache_graph_key[320] = {};
+    uint32_t stop_at = 0;      // passes after which render calls stop (0: never)
+    double gen_seconds = -1.0; // time to reach stop_at, once reached
+    std::chrono::steady_clock::time_point t_start;  // first pass after the last reset
     void* screen = nullptr;    // caller's RGBA32F screen buffer (device), or none
     uint32_t screen_w = 0, screen_h = 0;
     size_t screen_pitch = 0;   // bytes between rows

@@ -731,6 +731,7 @@ void render_display(const Renderer& r, void* out, uint32_t out_w, uint32_t out_h
 
 void render_reset(Renderer& r, hipStream_t st) {
     r.spp = 0;
+    r.gen_seconds = -1.0;
     if (r.w <= 0) return;
     const size_t n = static_cast<size_t>(r.w) * r.h * 3;
     hipLaunchKernelGGL(k_zero_f64, dim3(grid_for(n)), dim3(kThreads), 0, st, r.accum.as<double>(), n);

@@ -122,6 +122,8 @@ _SIGS = {
     "pt_write_accumulator": [_vp, _i, _i, _vp, _u32],
     "pt_read_image": [_vp, _vp],
     "pt_upload_detail": [_vp, _vp, _u32, _vp, _u32],
+    "pt_set_stop_frame": [_vp, _u32],
+    "pt_generation_time": [_vp, _vp],
     "pt_read_image_device": [_vp, _vp, _u32, _u32, ctypes.c_size_t, _vp],
     "pt_set_screen_buffer": [_vp, _vp, _u32, _u32, ctypes.c_size_t],
     "pt_write_png": [ctypes.c_char_p, _vp, _u32, _u32, _u32],
@@ -549,6 +551,16 @@ class Context:
         """Device RGBA32F buffer the image is written into after every frame
         (d_rgba None or 0 detaches)."""
         self._c(_lib.pt_set_screen_buffer(self._h, d_rgba or None, w, h, row_pitch or 16 * w))
+
+    def set_stop_frame(self, frames):
+        """Render calls stop once `frames` passes have accumulated (0: never)."""
+        self._c(_lib.pt_set_stop_frame(self._h, int(frames)))
+
+    def generation_time(self):
+        """Seconds to reach the stop frame, or -1 while not reached."""
+        v = np.zeros(1, np.float64)
+        self._c(_lib.pt_generation_time(self._h, _ptr(v)))
+        return float(v[0])
 
     def last_render_ms(self):
         ms = np.zeros(1, np.float32)

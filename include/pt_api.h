@@ -251,6 +251,17 @@ int pt_read_image_device(pt_context* ctx, void* d_rgba, uint32_t w, uint32_t h, 
  * outside the frame's timed region.  A graphics API maps the buffer as its
  * screen texture through its own interop.  d_rgba NULL detaches it. */
 int pt_set_screen_buffer(pt_context* ctx, void* d_rgba, uint32_t w, uint32_t h, size_t row_pitch);
+/* Stop at a frame count (reference: CUDAPathTracer::raytrace with
+ * PathTracerSettings::DEMO.stopAtFrame, cuda_path_tracer.cu:428-441,
+ * PathTracingSettings.h:15; the reference's -1 = never is 0 here, the
+ * default): once `frames` passes have accumulated since the last reset or
+ * upload, pt_render_pass, pt_render_passes and pt_render_cache_frame render
+ * nothing (a multi-pass call renders only the passes up to the limit). */
+int pt_set_stop_frame(pt_context* ctx, uint32_t frames);
+/* The reference's "Picture generation time": seconds from the first pass
+ * after the last reset to the render call that found the limit reached, or -1
+ * while it has not been reached. */
+int pt_generation_time(pt_context* ctx, double* seconds);
 /* Wall time in ms of the last pt_render_pass (HIP events). */
 int pt_last_render_ms(pt_context* ctx, float* ms);
 

@@ -393,3 +393,33 @@ def test_textured_obj_scene_matches_oracle(ctx, tmp_path):
         orc.render_pass_detail(tris, b, infos, det, texs, st, W, H, dirs[p], eye, p, int(seeds[p]), o)
     assert o.sum() > 0
     assert np.array_equal(acc, o)
+
+
+def test_stop_at_frame(ctx, scene):
+    """CUDAPathTracer::raytrace's stop frame: passes beyond the limit render
+    nothing; a multi-pass call renders up to it; the generation time is set
+    once the limit is found reached and cleared by a reset."""
+    tris, infos, st = scene
+    ctx.upload_scene(tris, infos, settings=st)
+    seeds = np.arange(60, 66, dtype=np.uint64)
+    dirs = np.stack([pass_dirs(p) for p in range(6)])
+    try:
+        ctx.set_primary_rays(dirs[0], EYE)  # the eye every pass starts at
+        ctx.reset_frames()
+        ctx.render_passes(W, H, seeds[:3], dirs[:3])
+        want = ctx.read_accumulator(W, H)[0]
+        ctx.set_stop_frame(3)
+        ctx.reset_frames()
+        for p in range(5):
+            ctx.set_primary_rays(dirs[p], EYE)
+            ctx.render_pass(W, H, int(seeds[p]))
+        acc, spp = ctx.read_accumulator(W, H)
+        assert spp == 3 and np.array_equal(acc, want)
+        assert ctx.generation_time() > 0.0
+        ctx.reset_frames()
+        assert ctx.generation_time() == -1.0
+        ctx.render_passes(W, H, seeds, dirs)  # 6 requested, 3 rendered
+        acc, spp = ctx.read_accumulator(W, H)
+        assert spp == 3 and np.array_equal(acc, want)
+    finally:
+        ctx.set_stop_frame(0)
