@@ -58,7 +58,8 @@ int pt_gen_batch_counts(uint64_t seed, uint32_t nscenes, uint32_t lo, uint32_t h
 /* Pinhole camera corner directions (normalised) in the order bottom-left,
  * bottom-right, top-right, top-left, through NDC corners shifted by
  * (-jx, -jy) as the reference's jitter translation does
- * (PathTracingApp.cpp:225-242).  up must not be parallel to target-eye. */
+ * (PathTracingApp.cpp:225-242).  up must not be parallel to target-eye.
+ * Returns the number of corners written (4). */
 int pt_camera_corners(const float* eye, const float* target, const float* up,
                       float fovy_deg, float aspect, float jx, float jy, float* dirs);
 

@@ -66,3 +66,15 @@ def test_tuning_knobs():
 def test_null_arguments_rejected():
     assert ptamd._lib.pt_upload_scene(None, None, 0, None, 0, None, 0, None, None, None) == ptamd.PT_E_INVALID_ARG
     assert ptamd._lib.pt_scene_bounds(None, 0, None, None) == ptamd.PT_E_INVALID_ARG
+
+
+def test_cpp_facade_header_compiles(tmp_path):
+    """include/pt_tracer.hpp compiles warning-free on its own (C++14 and 17)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "pt_tracer.hpp"\nint main() { ptamd::vec3 v{1, 2, 3}; return v.x > 0 ? 0 : 1; }\n')
+    for std in ("c++14", "c++17"):
+        r = subprocess.run(["g++", f"-std={std}", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                            "-I" + os.path.join(root, "include"), str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
