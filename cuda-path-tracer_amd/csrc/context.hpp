@@ -194,13 +194,13 @@ struct Renderer {
     bool ring_used[kParamRing] = {};
     int ring_next = 0;
     void* graph_exec = nullptr;   // hipGraphExec_t of the last captured frame
-    alignas(8) char graph_key[160] = {};
+    alignas(8) char graph_key[192] = {};
     bool timed = false;   // a pass has been bracketed by the context events
     uint32_t last_levels = 0;  // queue levels of the last frame
     DevBuf cache_io;           // cache-frame queries and training batch
     uint32_t last_train = 0;   // training paths of the last cache frame
     void* cache_graph_exec = nullptr;  // hipGraphExec_t of the last cache frame
-    alignas(8) char cache_graph_key[320] = {};
+    alignas(8) char cache_graph_key[384] = {};
     uint32_t stop_at = 0;      // passes after which render calls stop (0: never)
     double gen_seconds = -1.0; // time to reach stop_at, once reached
     std::chrono::steady_clock::time_point t_start;  // first pass after the last reset

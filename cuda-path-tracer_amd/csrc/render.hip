@@ -766,10 +766,10 @@ void upload_pass_params(Renderer& r, uint32_t npasses, const float* dirs, const 
 
 // Capture `launch` into the graph slot (exec, key) unless the key matches
 // the last capture, then launch the graph.
-template <typename Key, typename Launch>
-void replay(void*& exec, char* key_store, size_t key_cap, const Key& key, hipStream_t st, Launch launch) {
+template <typename Key, size_t Cap, typename Launch>
+void replay(void*& exec, char (&key_store)[Cap], const Key& key, hipStream_t st, Launch launch) {
     static_assert(std::is_trivially_copyable<Key>::value, "graph key");
-    if (sizeof(Key) > key_cap) throw Error(PT_E_INTERNAL, "graph key storage");
+    static_assert(sizeof(Key) <= Cap, "graph key storage too small for the key");
     if (!exec || std::memcmp(key_store, &key, sizeof key) != 0) {
         if (exec) {
             PT_HIP(hipStreamSynchronize(st));
@@ -841,7 +841,7 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         key.lds = trace_tuning().lds;
         key.pool = trace_tuning().pool;
         key.stream = st;
-        replay(r.graph_exec, r.graph_key, sizeof r.graph_key, key, st,
+        replay(r.graph_exec, r.graph_key, key, st,
                [&] { launch_frame(r, scene, fc, st); });
     }
     r.spp += npasses;
@@ -1001,7 +1001,7 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
         key.adam[2] = cache.beta2;
         key.adam[3] = cache.eps;
         key.stream = st;
-        replay(r.cache_graph_exec, r.cache_graph_key, sizeof r.cache_graph_key, key, st,
+        replay(r.cache_graph_exec, r.cache_graph_key, key, st,
                [&] { launch_cache_frame(r, scene, cache, cf, st); });
     }
     r.spp += 1;
