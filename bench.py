@@ -31,6 +31,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def render_rmse(ctx, img, w, h, eye, tgt, ref_spp, per_frame, seed):
+    """RMSE over all pixels and channels of the running-mean image `img`
+    (3 * w * h) against the plain tracer's FP64 accumulator at `ref_spp`
+    passes, rendered as two independent halves (own seeds and camera
+    jitter) whose difference estimates the reference's own variance, which
+    is subtracted.  Leaves the context's accumulator holding the second
+    half.  Returns (rmse, reference rmse noise)."""
+    import numpy as np
+
+    import ptamd
+
+    halves = []
+    for half in range(2):
+        ctx.reset_frames()
+        for b in range(0, ref_spp // 2, per_frame):
+            ps = range(b, min(b + per_frame, ref_spp // 2))
+            ctx.render_passes(w, h, np.array([seed + 10 ** 7 * half + p for p in ps], np.uint64),
+                              np.stack([ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h,
+                                                             *ptamd.pass_jitter(seed + half, p, w, h)) for p in ps]))
+        acc, spp = ctx.read_accumulator(w, h)
+        halves.append(acc / spp)
+    ref = 0.5 * (halves[0] + halves[1])
+    ref_var = float(np.mean((halves[0] - halves[1]) ** 2)) / 4.0
+    mse = float(np.mean((img - ref) ** 2)) - ref_var
+    return round(max(mse, 0.0) ** 0.5, 6), round(ref_var ** 0.5, 6)
+
+
 def cache_leg(ctx, stream, args, seed_mesh):
     """Time-to-RMSE of cache frames against the plain tracer on the room
     scene (871,200-triangle mesh + room + ceiling emitter), 1 spp per frame,
@@ -334,12 +361,14 @@ def main():
 
     # Progressive render legs (spp/s): config 0 (16 passes per frame) and
     # config 1 (one pass per frame), frames replayed as captured hipGraphs.
+    # Each states the RMSE of its image against a high-spp plain reference
+    # (ref_spp passes, FP64 sums; outside the timed region).
     render = {}
-    for name, (rt, ri), st, (w, h), frames, per_frame, eye, tgt in (
+    for name, (rt, ri), st, (w, h), frames, per_frame, eye, tgt, ref_spp in (
         ("c0_soup_256x256_b4", ptamd.gen_soup(65536, 0), ptamd.Settings(4, ptamd.INVALID, 1.0, 1),
-         (256, 256), 2, 16, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)),
+         (256, 256), 2, 16, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0), 4096),
         ("c1_room_1024x1024_b8_rr3", ptamd.gen_sphere(660, seed_mesh, room=True), ptamd.Settings(8, 3, 1.0, 1),
-         (1024, 1024), 4 if args.profile else 8, 1, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))):
+         (1024, 1024), 4 if args.profile else 8, 1, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0), 256)):
         ctx.upload_scene(rt, ri, settings=st)
         ctx.set_primary_rays(ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h), eye)
         passes = frames * per_frame
@@ -363,6 +392,12 @@ def main():
                         "passes_per_frame": per_frame,
                         "segments_per_pass": int(frame_rays // per_frame),
                         "Msegments_per_s": round(frame_rays / per_frame * passes / (rms * 1e-3) / 1e6, 1)}
+        if not args.profile:
+            acc, spp = ctx.read_accumulator(w, h)
+            err, noise = render_rmse(ctx, acc / spp, w, h, eye, tgt, ref_spp, per_frame, 7 * 10 ** 8)
+            render[name].update({"rmse": err, "rmse_spp": int(spp), "rmse_ref_spp": ref_spp,
+                                 "rmse_ref_noise": noise})
+            log(f"[bench] render {name}: RMSE {err} at {spp} spp (reference {ref_spp} spp, noise {noise})")
         log(f"[bench] render {name}: {rms / passes:.3f} ms/pass ({per_frame} passes/frame)")
 
     # Config 2: 10,000-strand hairball (2,880,000 triangles, heavily
@@ -407,6 +442,10 @@ def main():
         h1.synchronize()
         hr_ms = h0.elapsed_ms(h1) / 2
         hsegs = ctx.last_frame_rays()
+        h_err = None
+        if not args.profile:
+            acc, spp = ctx.read_accumulator(w2, h2)
+            h_err = render_rmse(ctx, acc / spp, w2, h2, eye2, tgt2, 128, 16, 8 * 10 ** 8) + (int(spp),)
         hair = {"triangles": int(len(ht)), "build_ms": round(h_build, 3),
                 "closest_hit_Mrays_per_s": round(hn / (h_ms * 1e-3) / 1e6, 1), "closest_hit_ms": round(h_ms, 3),
                 "visits_per_ray": round(int(hcnt[0]) / hn, 3), "box_tests_per_ray": round(int(hcnt[1]) / hn, 3),
@@ -416,6 +455,9 @@ def main():
                 "render_1920x1080_b6": {"ms_per_pass": round(hr_ms, 3), "spp_per_s": round(1e3 / hr_ms, 2),
                                         "segments_per_pass": int(hsegs),
                                         "Msegments_per_s": round(hsegs / (hr_ms * 1e-3) / 1e6, 1)}}
+        if h_err is not None:
+            hair["render_1920x1080_b6"].update({"rmse": h_err[0], "rmse_spp": h_err[2], "rmse_ref_spp": 128,
+                                                "rmse_ref_noise": h_err[1]})
         log(f"[bench] hairball ({len(ht)} tris): build {h_build:.3f} ms; closest hit {hair['closest_hit_Mrays_per_s']} "
             f"Mrays/s ({hair['visits_per_ray']} visits/ray); render 1080p b6 {hr_ms:.2f} ms/pass")
 
