@@ -81,6 +81,10 @@ struct CacheIO {
 constexpr uint32_t kTrainSalt = 0x5bd1e995u;
 
 constexpr int kThreads = 256;
+// k_shade workgroups: its queue append is one atomic per workgroup pass, and
+// one counter address sustains only ~90 M returning atomics/s, so a wide
+// workgroup (16 waves) keeps a 1M-path level's appends at ~1K atomics.
+constexpr int kShadeThreads = 1024;
 constexpr uint32_t kMaxLevels = 64;  // ray queue blocks: at most 61 path levels (60 bounces)
 
 __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
@@ -315,7 +319,7 @@ __device__ __forceinline__ bool next_bounce(uint32_t key, uint32_t pass, uint32_
 // CACHE: a cache frame (one pass; paths >= npix are training paths).
 // LIGHTS: the scene has directional lights.
 template <bool CACHE, bool LIGHTS, bool DETAIL>
-__global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
+__global__ __launch_bounds__(kShadeThreads) void k_shade(const FrameConst fc,
                                                     const PassParams* __restrict__ pp,
                                                     const pt_triangle_info* __restrict__ infos,
                                                     RayQueue qin, HitQueue hq, RayQueue qout,
@@ -325,11 +329,15 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
                                                     const float4* __restrict__ nodes,
                                                     const float4* __restrict__ tris,
                                                     const uint32_t* __restrict__ off) {
+    __shared__ uint32_t s_off[kShadeThreads / 64];  // per-wave offsets in the workgroup's append
+    __shared__ uint32_t s_base;                      // the workgroup's first slot in qout
     const uint32_t count = counters[level];
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
     const uint32_t stride = gridDim.x * blockDim.x;
-    // Wave-uniform trip count so every lane reaches the ballots below.
+    // Workgroup-uniform trip count so every lane reaches the ballots and
+    // barriers below.
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool cont = false;
@@ -400,22 +408,32 @@ __global__ __launch_bounds__(kThreads) void k_shade(const FrameConst fc,
             ps.cg[path] = acc.y;
             ps.cb[path] = acc.z;
         }
+        // Append the continuing paths to qout: one atomic per workgroup,
+        // waves in order inside it, lanes in order inside a wave.
         const uint64_t m = __ballot(cont);
-        if (m != 0ull) {
-            const int leader = __ffsll(static_cast<long long>(m)) - 1;
-            uint32_t wbase = 0;
-            if (lane == leader) wbase = atomicAdd(counters + level + 1, static_cast<uint32_t>(__popcll(m)));
-            wbase = __shfl(wbase, leader, 64);
-            if (cont) {
-                const uint32_t s = wbase + static_cast<uint32_t>(__popcll(m & lt));
-                qout.ox[s] = no.x;
-                qout.oy[s] = no.y;
-                qout.oz[s] = no.z;
-                qout.dx[s] = nd.x;
-                qout.dy[s] = nd.y;
-                qout.dz[s] = nd.z;
-                qout.path[s] = path;
+        if (lane == 0) s_off[wave] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t total = 0;
+            for (uint32_t k = 0; k < kShadeThreads / 64; ++k) {
+                const uint32_t c = s_off[k];
+                s_off[k] = total;
+                total += c;
             }
+            s_base = total ? atomicAdd(counters + level + 1, total) : 0u;
+        }
+        __syncthreads();
+        // (The next pass rewrites s_off and s_base only after its first
+        // barrier, which every wave reaches after its reads here.)
+        if (cont) {
+            const uint32_t s = s_base + s_off[wave] + static_cast<uint32_t>(__popcll(m & lt));
+            qout.ox[s] = no.x;
+            qout.oy[s] = no.y;
+            qout.oz[s] = no.z;
+            qout.dx[s] = nd.x;
+            qout.dy[s] = nd.y;
+            qout.dz[s] = nd.z;
+            qout.path[s] = path;
         }
     }
 }
@@ -628,6 +646,14 @@ uint32_t grid_for(size_t work) {
     return static_cast<uint32_t>(b);
 }
 
+// k_shade's grid: one workgroup pass covers kShadeThreads paths.
+uint32_t shade_grid_for(size_t work) {
+    size_t b = (work + kShadeThreads - 1) / kShadeThreads;
+    if (b > 2048) b = 2048;
+    if (b < 1) b = 1;
+    return static_cast<uint32_t>(b);
+}
+
 // Everything a captured frame's kernel arguments depend on.
 void set_detail(FrameConst& fc, const DeviceScene& scene) {
     fc.ntex = scene.ntex;
@@ -671,7 +697,7 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, h
         hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
                            scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill, pool);
-        hipLaunchKernelGGL(shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(shade, dim3(shade_grid_for(npaths)), dim3(kShadeThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, CacheIO{}, scene.lights.as<pt_dir_light>(),
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
@@ -884,7 +910,7 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
         hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
                            scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill, pool);
-        hipLaunchKernelGGL(shade, dim3(grid), dim3(kThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(shade, dim3(shade_grid_for(npaths)), dim3(kShadeThreads), 0, st, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
