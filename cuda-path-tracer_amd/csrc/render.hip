@@ -6,7 +6,7 @@
 //   k_generate     primary rays of every path into queue 0, counters reset
 //   k_extend(L)    persistent closest hit for every ray of queue L
 //   k_shade(L)     shade the hits, append surviving paths to queue L+1
-//                  (wave-aggregated atomics), L = 0 .. bounces
+//                  (workgroup-aggregated atomics), L = 0 .. bounces
 //   k_accumulate   per pixel, FP64 sums += each pass's radiance, in pass order
 // Queue lengths live on the device, so the host never waits inside a frame,
 // and a frame's kernel sequence is captured once as a hipGraph and replayed.
