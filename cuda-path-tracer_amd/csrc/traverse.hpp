@@ -357,6 +357,12 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
                                                  uint32_t refill, uint32_t pool_large,
                                                  uint32_t (*s_stack)[kPersistThreads], Load load,
                                                  Store store, TraceCount& cnt) {
+    // Waves past one per kRayPoolSmall rays leave at once: every wave that
+    // stays drains the queue with the others, and on a short queue (a late
+    // render level, a cache frame's training levels) the extra waves would
+    // only add returning atomics on its few slice counters.
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u);
+    if (wave_id > 0u && static_cast<uint64_t>(wave_id) * kRayPoolSmall >= nrays) return;
     uint32_t spill[kStackMax - LDS];
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
@@ -374,7 +380,7 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
     // Wave-uniform state is read through readfirstlane so that it lives in
     // scalar registers (the compiler cannot tell that threadIdx.x / 64 is
     // uniform over a wave).
-    uint32_t q = __builtin_amdgcn_readfirstlane((blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u) % nq);
+    uint32_t q = wave_id % nq;
     uint64_t seen = ~all;  // wave-uniform: slices this wave found used up
     bool used_up = false;  // wave-uniform: the refill loop found slice q used up
     for (;;) {
