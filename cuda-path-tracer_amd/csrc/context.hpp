@@ -198,6 +198,11 @@ struct Renderer {
     bool timed = false;   // a pass has been bracketed by the context events
     uint32_t last_levels = 0;  // queue levels of the last frame
     DevBuf cache_io;           // cache-frame queries and training batch
+    // A cache frame traces its training paths' last bounces on `side` while
+    // the pixel paths are resolved on the frame's stream (fork and join
+    // events); created on the first cache frame, default priority.
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     uint32_t last_train = 0;   // training paths of the last cache frame
     void* cache_graph_exec = nullptr;  // hipGraphExec_t of the last cache frame
     alignas(8) char cache_graph_key[384] = {};

@@ -713,6 +713,9 @@ Renderer::~Renderer() {
     if (cache_graph_exec) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(cache_graph_exec));
     for (auto& e : ring_ev)
         if (e) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (side) (void)hipStreamDestroy(side);
 }
 
 void render_resize(Renderer& r, int w, int h, hipStream_t st) {
@@ -915,17 +918,27 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
     }
+    // The training paths' suffix runs on the side stream beside the pixel
+    // paths' inference, resolve and accumulation: it reads the level-2 queue
+    // and writes path state of training paths (index >= npix) only, and they
+    // touch pixel paths and the cache's query buffers only.  The join comes
+    // before the training targets and the Adam steps, which change the
+    // weights the inference reads.
     if (nlev > 2) {
+        PT_HIP(hipEventRecord(r.fork_ev, st));
+        PT_HIP(hipStreamWaitEvent(r.side, r.fork_ev, 0));
         const uint32_t sgrid = std::max(1u, std::min(4096u, (io.ntrain + kSuffixThreads - 1) / kSuffixThreads));
-        hipLaunchKernelGGL(suffix, dim3(sgrid), dim3(kSuffixThreads), 0, st, fc, pp,
+        hipLaunchKernelGGL(suffix, dim3(sgrid), dim3(kSuffixThreads), 0, r.side, fc, pp,
                            scene.infos.as<pt_triangle_info>(), L.q[0], L.ps, cnt, 2u, io,
                            scene.lights.as<pt_dir_light>(), scene.bvh.nodes.as<float4>(),
                            scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
+        PT_HIP(hipEventRecord(r.join_ev, r.side));
     }
     cache.infer(io.qraw, fc.npix, io.qout, st);
     hipLaunchKernelGGL(k_cache_resolve, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, io, L.ps);
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
                        r.accum.as<double>());
+    if (nlev > 2) PT_HIP(hipStreamWaitEvent(st, r.join_ev, 0));
     if (io.ntrain) {
         hipLaunchKernelGGL(k_train_targets, dim3(grid_for(io.ntrain)), dim3(kThreads), 0, st, fc, io, L.ps);
         if (cf.train) {
@@ -997,6 +1010,11 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     cf.io.train_levels = 2u + cs.extra_bounces;
     if (ntrain && cf.train) cache.prepare_train((ntrain + cf.steps - 1) / cf.steps);
     upload_pass_params(r, 1, nullptr, &seed, nullptr, st);
+    if (!r.side) {
+        PT_HIP(hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking));
+        PT_HIP(hipEventCreateWithFlags(&r.fork_ev, hipEventDisableTiming));
+        PT_HIP(hipEventCreateWithFlags(&r.join_ev, hipEventDisableTiming));
+    }
 
     if (!render_tuning().graphs) {
         launch_cache_frame(r, scene, cache, cf, st);
