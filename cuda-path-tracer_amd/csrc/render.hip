@@ -87,6 +87,21 @@ constexpr int kThreads = 256;
 constexpr int kShadeThreads = 1024;
 constexpr uint32_t kMaxLevels = 64;  // ray queue blocks: at most 61 path levels (60 bounces)
 
+// Queue slot -> pixel for primary rays: 8x8 tiles in bands of 8 rows (the
+// last band and the last tile of a band may be smaller), so the 64 lanes of
+// a wave start on one tile -- coherent rays -- rather than on a 64-pixel row
+// strip.  A bijection of [0, w*h); paths keep their pixel as identity, so
+// the order changes no result.
+__device__ __forceinline__ uint32_t tiled_pixel(uint32_t i, uint32_t w, uint32_t h) {
+    const uint32_t y0 = 8u * (i / (8u * w));
+    const uint32_t hb = min(8u, h - y0);
+    const uint32_t r = i - y0 * w;
+    const uint32_t tx = r / (8u * hb);
+    const uint32_t c = r - tx * 8u * hb;
+    const uint32_t tw = min(8u, w - 8u * tx);
+    return (y0 + c / tw) * w + 8u * tx + c % tw;
+}
+
 __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
                                                        const PassParams* __restrict__ pp,
                                                        RayQueue q, PathState ps,
@@ -100,9 +115,12 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
     }
     if (blockIdx.x == 0) zero_queue(ray_counters, nlevels);
-    for (uint32_t p = gid; p < npaths; p += gridDim.x * blockDim.x) {
-        const PassParams& P = pp[p / fc.npix];
-        const uint32_t pix = p % fc.npix;
+    for (uint32_t slot = gid; slot < npaths; slot += gridDim.x * blockDim.x) {
+        const uint32_t pass = slot / fc.npix;
+        const PassParams& P = pp[pass];
+        const uint32_t pix = tiled_pixel(slot - pass * fc.npix, static_cast<uint32_t>(fc.w),
+                                         static_cast<uint32_t>(fc.h));
+        const uint32_t p = pass * fc.npix + pix;
         const int x = static_cast<int>(pix % static_cast<uint32_t>(fc.w));
         const int y = static_cast<int>(pix / static_cast<uint32_t>(fc.w));
         // Pixel-corner blend of the corner directions (cuda_path_tracer.cu:404-407).
@@ -113,13 +131,13 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
         const V3 d11 = v3(P.dirs[6], P.dirs[7], P.dirs[8]);
         const V3 d01 = v3(P.dirs[9], P.dirs[10], P.dirs[11]);
         const V3 d = mix(mix(d00, d10, tx), mix(d01, d11, tx), ty);
-        q.ox[p] = P.eye[0];
-        q.oy[p] = P.eye[1];
-        q.oz[p] = P.eye[2];
-        q.dx[p] = d.x;
-        q.dy[p] = d.y;
-        q.dz[p] = d.z;
-        q.path[p] = p;
+        q.ox[slot] = P.eye[0];
+        q.oy[slot] = P.eye[1];
+        q.oz[slot] = P.eye[2];
+        q.dx[slot] = d.x;
+        q.dy[slot] = d.y;
+        q.dz[slot] = d.z;
+        q.path[slot] = p;
         ps.cr[p] = 0.0f;
         ps.cg[p] = 0.0f;
         ps.cb[p] = 0.0f;
