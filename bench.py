@@ -376,6 +376,17 @@ def main():
                          for p in range(passes)])
         seeds = np.arange(1000, 1000 + passes, dtype=np.uint64)
         ctx.render_passes(w, h, seeds[:per_frame], dirs[:per_frame])  # warm-up frame (allocation, capture)
+        # The traversal knobs tuned on this render's own frames (restored to
+        # the closest-hit choice after the leg).
+        rtune, saved = None, None
+        if not args.no_tune:
+            from ptamd import autotune
+
+            space = autotune.render_space()
+            saved = {k: ptamd.tune_get(k) for k in space}
+            best, table = autotune.tune(lambda: ctx.render_passes(w, h, seeds[:per_frame], dirs[:per_frame]),
+                                        stream, space, reps=3)
+            rtune = {"chosen": best, "candidates": len(table)}
         ctx.reset_frames()
         stream.synchronize()
         r0, r1 = hip.Event(), hip.Event()
@@ -392,13 +403,19 @@ def main():
                         "passes_per_frame": per_frame,
                         "segments_per_pass": int(frame_rays // per_frame),
                         "Msegments_per_s": round(frame_rays / per_frame * passes / (rms * 1e-3) / 1e6, 1)}
+        if rtune:
+            render[name]["tuning"] = rtune
         if not args.profile:
             acc, spp = ctx.read_accumulator(w, h)
             err, noise = render_rmse(ctx, acc / spp, w, h, eye, tgt, ref_spp, per_frame, 7 * 10 ** 8)
             render[name].update({"rmse": err, "rmse_spp": int(spp), "rmse_ref_spp": ref_spp,
                                  "rmse_ref_noise": noise})
             log(f"[bench] render {name}: RMSE {err} at {spp} spp (reference {ref_spp} spp, noise {noise})")
-        log(f"[bench] render {name}: {rms / passes:.3f} ms/pass ({per_frame} passes/frame)")
+        log(f"[bench] render {name}: {rms / passes:.3f} ms/pass ({per_frame} passes/frame)"
+            + (f", tuned {rtune['chosen']}" if rtune else ""))
+        if saved:
+            for k, v in saved.items():
+                ptamd.tune_set(k, v)
 
     # Config 2: 10,000-strand hairball (2,880,000 triangles, heavily
     # overlapping boxes): build, the 16M-ray closest-hit microbench (same ray

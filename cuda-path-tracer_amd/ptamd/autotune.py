@@ -77,6 +77,16 @@ def stack_space():
             "trace.pool": [128, 256, 512, 1024]}
 
 
+def render_space(num_cus=256):
+    """Traversal knobs for a progressive render, searched together on the
+    render's own frames: its bounce rays traverse deeper than the closest-hit
+    microbench's shell rays and favour other lane and queue settings."""
+    return {"trace.blocks": [num_cus * 8, num_cus * 12],
+            "trace.refill": [32, 48],
+            "trace.lds_stack": [8, 16],
+            "trace.pool": [128, 256]}
+
+
 def trace_stages(num_cus=256):
     """The default staged closest-hit search: launch shape, then stack and
     queue sizes."""
