@@ -1028,10 +1028,10 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     cf.io.train_levels = 2u + cs.extra_bounces;
     if (ntrain && cf.train) cache.prepare_train((ntrain + cf.steps - 1) / cf.steps);
     upload_pass_params(r, 1, nullptr, &seed, nullptr, st);
-    if (!r.side) {
-        PT_HIP(hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking));
-        PT_HIP(hipEventCreateWithFlags(&r.fork_ev, hipEventDisableTiming));
-        PT_HIP(hipEventCreateWithFlags(&r.join_ev, hipEventDisableTiming));
+    if (!r.side || !r.fork_ev || !r.join_ev) {
+        if (!r.side) PT_HIP(hipStreamCreateWithFlags(&r.side, hipStreamNonBlocking));
+        if (!r.fork_ev) PT_HIP(hipEventCreateWithFlags(&r.fork_ev, hipEventDisableTiming));
+        if (!r.join_ev) PT_HIP(hipEventCreateWithFlags(&r.join_ev, hipEventDisableTiming));
     }
 
     if (!render_tuning().graphs) {
