@@ -73,7 +73,7 @@ def cache_leg(ctx, stream, args, seed_mesh):
     w, h = (1920, 1080) if args.cache_full else (960, 540)
     ntrain = 65536 if args.cache_full else 16384
     ref_spp = args.cache_ref_spp or (4096 if args.cache_full else 8192)
-    extra, steps, frames, plain_max = 8, 4, 300, 1024
+    extra, steps, frames, plain_max = 8, args.cache_steps, 300, 1024
     rt, ri = ptamd.gen_sphere(660, seed_mesh, room=True)
     st = ptamd.Settings(1 + extra, 3, 1.0, 1)
     ctx.upload_scene(rt, ri, settings=st)
@@ -258,6 +258,7 @@ def main():
     ap.add_argument("--cache-full", action="store_true",
                     help="radiance-cache leg at 1920x1080 with 65,536 training samples per frame "
                          "(default: 960x540, training scaled to 16,384)")
+    ap.add_argument("--cache-steps", type=int, default=4, help="Adam steps per cache frame (1..64)")
     ap.add_argument("--cache-ref-spp", type=int, default=0,
                     help="reference spp of the cache leg (default 8192, 4096 with --cache-full)")
     args = ap.parse_args()
