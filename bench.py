@@ -546,7 +546,11 @@ def main():
         # About cpu_seconds of host work: a prefix of the rays, or (when all
         # of them take less) the whole set traced `reps` times.
         m = int(min(n, max(len(probe), rate * args.cpu_seconds)))
-        reps = max(1, int(rate * args.cpu_seconds / n)) if m == n else 1
+        reps = 1
+        if m == n:
+            t1 = time.time()
+            orc.trace(tris, o, rays, threads=threads)
+            reps = max(1, int(round(args.cpu_seconds / max(time.time() - t1, 1e-6))))
         sample = rays[:m]
         t1 = time.time()
         for _ in range(reps):
