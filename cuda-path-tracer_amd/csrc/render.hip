@@ -85,6 +85,10 @@ constexpr int kThreads = 256;
 // one counter address sustains only ~90 M returning atomics/s, so a wide
 // workgroup (16 waves) keeps a 1M-path level's appends at ~1K atomics.
 constexpr int kShadeThreads = 1024;
+// Counter slots a cache frame's training paths report their first two
+// levels in (their later levels go to the level counters): the frame's own
+// levels 0 and 1 count pixel paths only, and run at the same time.
+constexpr uint32_t kTrainLevel0Slot = 62, kTrainLevel1Slot = 63;
 constexpr uint32_t kMaxLevels = 64;  // ray queue blocks: at most 61 path levels (60 bounces)
 
 // Queue slot -> pixel for primary rays: 8x8 tiles in bands of 8 rows (the
@@ -113,6 +117,8 @@ __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
     if (gid == 0) {
         counters[0] = npaths;
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
+        counters[kTrainLevel0Slot] = 0u;
+        counters[kTrainLevel1Slot] = 0u;
     }
     if (blockIdx.x == 0) zero_queue(ray_counters, nlevels);
     for (uint32_t slot = gid; slot < npaths; slot += gridDim.x * blockDim.x) {
@@ -334,6 +340,50 @@ __device__ __forceinline__ bool next_bounce(uint32_t key, uint32_t pass, uint32_
     return true;
 }
 
+// Primary hit (level 0): emission, else Blinn-Phong from the directional
+// lights with shadow rays, the light term L summed over the lights seen so
+// far (cuda_path_tracer.cu:194-214, 258-259); the path's throughput starts
+// at the hit's albedo (1 on emitters).
+template <bool LIGHTS>
+__device__ __forceinline__ void primary_hit(HitShade& hs, V3 d, const FrameConst& fc,
+                                            const pt_dir_light* __restrict__ lights,
+                                            const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                            uint32_t sb, uint32_t se, V3& absorbed, V3& acc) {
+    acc = v3(0.0f, 0.0f, 0.0f);
+    if (hs.emissive) {
+        acc = hs.e;
+        hs.albedo = v3(1.0f, 1.0f, 1.0f);
+    } else if (LIGHTS) {
+        V3 Lc = v3(0.0f, 0.0f, 0.0f);
+        for (uint32_t k = 0; k < fc.nlights; ++k) {
+            const V3 ld = ld3(lights[k].direction);
+            Lc = add(Lc, light_L(nodes, tris, sb, se, hs.pos, lights[k]));
+            const V3 hw = scale(sub(neg(d), ld), 0.5f);
+            acc = add(acc, blinn_phong(Lc, hs.albedo, Lc, v3(0.5f, 0.5f, 0.5f), hs.nrm, neg(ld), hw, hs.spec));
+        }
+    }
+    absorbed = hs.albedo;
+}
+
+// A cache frame's second vertex x2: a pixel path's cache query, or a
+// training sample whose suffix radiance is gathered from here with
+// throughput 1.
+__device__ __forceinline__ void record_cache_vertex(const CacheIO& io, bool train, uint32_t pix, uint32_t path,
+                                                    const HitShade& hs, V3 d, V3& absorbed, V3& acc) {
+    float* f = train ? io.traw + 12 * static_cast<size_t>(pix) : io.qraw + 12 * static_cast<size_t>(path);
+    f[0] = hs.pos.x; f[1] = hs.pos.y; f[2] = hs.pos.z;
+    f[3] = hs.nrm.x; f[4] = hs.nrm.y; f[5] = hs.nrm.z;
+    f[6] = d.x;      f[7] = d.y;      f[8] = d.z;
+    f[9] = hs.albedo.x; f[10] = hs.albedo.y; f[11] = hs.albedo.z;
+    if (train) {
+        io.tvalid[pix] = 1u;
+        absorbed = v3(1.0f, 1.0f, 1.0f);
+        acc = v3(0.0f, 0.0f, 0.0f);
+    } else {
+        io.qvalid[path] = 1u;
+    }
+}
+
 // CACHE: a cache frame (one pass; paths >= npix are training paths).
 // LIGHTS: the scene has directional lights.
 template <bool CACHE, bool LIGHTS, bool DETAIL>
@@ -374,45 +424,13 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(const FrameConst fc,
             const uint32_t sb = off[P.scene], se = off[P.scene + 1];
             V3 absorbed, acc;
             if (level == 0) {
-                // Primary hit: emission, else Blinn-Phong from the directional
-                // lights with shadow rays, the light term L summed over the
-                // lights seen so far (cuda_path_tracer.cu:194-214, 258-259).
-                acc = v3(0.0f, 0.0f, 0.0f);
-                if (hs.emissive) {
-                    acc = hs.e;
-                    hs.albedo = v3(1.0f, 1.0f, 1.0f);
-                } else if (LIGHTS) {
-                    V3 Lc = v3(0.0f, 0.0f, 0.0f);
-                    for (uint32_t k = 0; k < fc.nlights; ++k) {
-                        const V3 ld = ld3(lights[k].direction);
-                        Lc = add(Lc, light_L(nodes, tris, sb, se, hs.pos, lights[k]));
-                        const V3 hw = scale(sub(neg(d), ld), 0.5f);
-                        acc = add(acc, blinn_phong(Lc, hs.albedo, Lc, v3(0.5f, 0.5f, 0.5f), hs.nrm, neg(ld), hw,
-                                                   hs.spec));
-                    }
-                }
-                absorbed = hs.albedo;
+                primary_hit<LIGHTS>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
             } else {
                 absorbed = v3(ps.ar[path], ps.ag[path], ps.ab[path]);
                 acc = v3(ps.cr[path], ps.cg[path], ps.cb[path]);
                 bounce_hit<LIGHTS>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
             }
-            if (CACHE && level == 1) {
-                // x2: a pixel path's cache query, or a training sample whose
-                // suffix radiance is gathered from here with throughput 1.
-                float* f = train ? io.traw + 12 * static_cast<size_t>(pix) : io.qraw + 12 * static_cast<size_t>(path);
-                f[0] = hs.pos.x; f[1] = hs.pos.y; f[2] = hs.pos.z;
-                f[3] = hs.nrm.x; f[4] = hs.nrm.y; f[5] = hs.nrm.z;
-                f[6] = d.x;      f[7] = d.y;      f[8] = d.z;
-                f[9] = hs.albedo.x; f[10] = hs.albedo.y; f[11] = hs.albedo.z;
-                if (train) {
-                    io.tvalid[pix] = 1u;
-                    absorbed = v3(1.0f, 1.0f, 1.0f);
-                    acc = v3(0.0f, 0.0f, 0.0f);
-                } else {
-                    io.qvalid[path] = 1u;
-                }
-            }
+            if (CACHE && level == 1) record_cache_vertex(io, train, pix, path, hs, d, absorbed, acc);
             // Next bounce index = level (bounce loop, cuda_path_tracer.cu:267).
             // Cache frames: pixel paths end at x2, training paths after
             // train_levels - 1 bounces.
@@ -456,47 +474,55 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(const FrameConst fc,
     }
 }
 
-// Cache frames: the training paths past level `level0` (only training paths
-// are left there), one lane per path to its end -- the per-level launches
-// of a few thousand rays are latency bound (each level waits for its slowest
-// ray), one lane per path waits only for the slowest path.  Traversal,
-// shading and sampling are k_extend / k_shade's, so the results are the same
-// bits; counters[L] still count the rays of level L.
+// Cache frames: the training paths, one lane per path from its primary ray
+// to its end, on a stream of their own beside the pixel paths' two levels,
+// the inference and the resolve (the per-level launches of a few thousand
+// rays are latency bound -- each level waits for its slowest ray -- and one
+// lane per path waits only for the slowest path).  Traversal, shading and
+// sampling are k_extend / k_shade's (primary_hit, bounce_hit,
+// record_cache_vertex, next_bounce), so the training batches are the same
+// bits as when these paths ran through the levels.
 constexpr int kSuffixThreads = 64;
 
 template <bool DETAIL>
-__global__ __launch_bounds__(kSuffixThreads) void k_train_suffix(const FrameConst fc,
-                                                                 const PassParams* __restrict__ pp,
-                                                                 const pt_triangle_info* __restrict__ infos,
-                                                                 RayQueue q, PathState ps,
-                                                                 uint32_t* __restrict__ counters, uint32_t level0,
-                                                                 CacheIO io, const pt_dir_light* __restrict__ lights,
-                                                                 const float4* __restrict__ nodes,
-                                                                 const float4* __restrict__ tris,
-                                                                 const uint32_t* __restrict__ off) {
-    const uint32_t count = counters[level0];
+__global__ __launch_bounds__(kSuffixThreads) void k_train_paths(const FrameConst fc,
+                                                                const PassParams* __restrict__ pp,
+                                                                const pt_triangle_info* __restrict__ infos,
+                                                                RayQueue q, PathState ps,
+                                                                uint32_t* __restrict__ counters, CacheIO io,
+                                                                const pt_dir_light* __restrict__ lights,
+                                                                const float4* __restrict__ nodes,
+                                                                const float4* __restrict__ tris,
+                                                                const uint32_t* __restrict__ off) {
     const PassParams& P = pp[0];
     const uint32_t seedw = P.seedw ^ kTrainSalt;
     const uint32_t sb = off[P.scene], se = off[P.scene + 1];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const uint32_t path = q.path[i];
-        const uint32_t key = path - fc.npix;
-        V3 o = v3(q.ox[i], q.oy[i], q.oz[i]);
-        V3 d = v3(q.dx[i], q.dy[i], q.dz[i]);
-        V3 absorbed = v3(ps.ar[path], ps.ag[path], ps.ab[path]);
-        V3 acc = v3(ps.cr[path], ps.cg[path], ps.cb[path]);
-        for (uint32_t b = level0;; ++b) {
+    for (uint32_t key = blockIdx.x * blockDim.x + threadIdx.x; key < io.ntrain; key += gridDim.x * blockDim.x) {
+        const uint32_t path = fc.npix + key;  // its slot in the first level's queue
+        V3 o = v3(q.ox[path], q.oy[path], q.oz[path]);
+        V3 d = v3(q.dx[path], q.dy[path], q.dz[path]);
+        V3 absorbed = v3(1.0f, 1.0f, 1.0f);
+        V3 acc = v3(0.0f, 0.0f, 0.0f);
+        atomicAdd(counters + kTrainLevel0Slot, 1u);
+        for (uint32_t b = 0;; ++b) {
             const TraceHit h = trace_ray_in(nodes, tris, sb, se, o, d);
             if (h.id == kInvalid) break;
             HitShade hs = hit_shade<DETAIL>(fc, infos, h.id, h.t, h.u, h.v, o, d);
-            if (fc.nlights)
+            if (b == 0) {
+                if (fc.nlights)
+                    primary_hit<true>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+                else
+                    primary_hit<false>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+            } else if (fc.nlights) {
                 bounce_hit<true>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
-            else
+            } else {
                 bounce_hit<false>(hs, d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+            }
+            if (b == 1) record_cache_vertex(io, true, key, path, hs, d, absorbed, acc);
             if (!(b < io.train_levels - 1u)) break;
             V3 no, nd;
             if (!next_bounce(key, P.pass, b, seedw, fc, hs, absorbed, no, nd)) break;
-            atomicAdd(counters + b + 1, 1u);
+            atomicAdd(counters + (b == 0 ? kTrainLevel1Slot : b + 1), 1u);
             o = no;
             d = nd;
         }
@@ -542,8 +568,12 @@ __global__ __launch_bounds__(kThreads) void k_generate_cache(const FrameConst fc
     const uint32_t npaths = fc.npix + io.ntrain;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
-        counters[0] = npaths;
+        // The frame's levels trace the pixel paths; the training paths' rays
+        // (slots npix ..) are read by k_train_paths.
+        counters[0] = fc.npix;
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
+        counters[kTrainLevel0Slot] = 0u;
+        counters[kTrainLevel1Slot] = 0u;
     }
     if (blockIdx.x == 0) zero_queue(ray_counters, nlevels);
     const PassParams& P = pp[0];
@@ -922,11 +952,26 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     auto extend = tune.lds <= 8 ? k_extend<8> : (tune.lds <= 16 ? k_extend<16> : k_extend<32>);
     auto shade = fc.detail ? (fc.nlights ? k_shade<true, true, true> : k_shade<true, false, true>)
                            : (fc.nlights ? k_shade<true, true, false> : k_shade<true, false, false>);
-    auto suffix = fc.detail ? k_train_suffix<true> : k_train_suffix<false>;
+    auto train_paths = fc.detail ? k_train_paths<true> : k_train_paths<false>;
     hipLaunchKernelGGL(k_generate_cache, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt,
                        nlev, io);
-    // Levels 0 and 1 hold every path; past x2 only training paths are left,
-    // traced to their ends by k_train_suffix.
+    // The training paths run on the side stream from here, beside the pixel
+    // paths' two levels, inference, resolve and accumulation: they read the
+    // first level's queue slots from npix on and write only training-path
+    // state, the training batch and counter slots the levels do not read.
+    // The join comes before the training targets and the Adam steps, which
+    // change the weights the inference reads.
+    if (io.ntrain) {
+        PT_HIP(hipEventRecord(r.fork_ev, st));
+        PT_HIP(hipStreamWaitEvent(r.side, r.fork_ev, 0));
+        const uint32_t sgrid = std::max(1u, std::min(4096u, (io.ntrain + kSuffixThreads - 1) / kSuffixThreads));
+        hipLaunchKernelGGL(train_paths, dim3(sgrid), dim3(kSuffixThreads), 0, r.side, fc, pp,
+                           scene.infos.as<pt_triangle_info>(), L.q[0], L.ps, cnt, io,
+                           scene.lights.as<pt_dir_light>(), scene.bvh.nodes.as<float4>(),
+                           scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
+        PT_HIP(hipEventRecord(r.join_ev, r.side));
+    }
+    // Levels 0 and 1 of the pixel paths (a pixel path ends in the cache at x2).
     for (uint32_t lvl = 0; lvl < 2; ++lvl) {
         hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
@@ -936,27 +981,11 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
                            L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
                            scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
     }
-    // The training paths' suffix runs on the side stream beside the pixel
-    // paths' inference, resolve and accumulation: it reads the level-2 queue
-    // and writes path state of training paths (index >= npix) only, and they
-    // touch pixel paths and the cache's query buffers only.  The join comes
-    // before the training targets and the Adam steps, which change the
-    // weights the inference reads.
-    if (nlev > 2) {
-        PT_HIP(hipEventRecord(r.fork_ev, st));
-        PT_HIP(hipStreamWaitEvent(r.side, r.fork_ev, 0));
-        const uint32_t sgrid = std::max(1u, std::min(4096u, (io.ntrain + kSuffixThreads - 1) / kSuffixThreads));
-        hipLaunchKernelGGL(suffix, dim3(sgrid), dim3(kSuffixThreads), 0, r.side, fc, pp,
-                           scene.infos.as<pt_triangle_info>(), L.q[0], L.ps, cnt, 2u, io,
-                           scene.lights.as<pt_dir_light>(), scene.bvh.nodes.as<float4>(),
-                           scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
-        PT_HIP(hipEventRecord(r.join_ev, r.side));
-    }
     cache.infer(io.qraw, fc.npix, io.qout, st);
     hipLaunchKernelGGL(k_cache_resolve, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, io, L.ps);
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
                        r.accum.as<double>());
-    if (nlev > 2) PT_HIP(hipStreamWaitEvent(st, r.join_ev, 0));
+    if (io.ntrain) PT_HIP(hipStreamWaitEvent(st, r.join_ev, 0));
     if (io.ntrain) {
         hipLaunchKernelGGL(k_train_targets, dim3(grid_for(io.ntrain)), dim3(kThreads), 0, st, fc, io, L.ps);
         if (cf.train) {
@@ -1098,12 +1127,14 @@ void render_last_batch(Renderer& r, float* raw, float* target, uint32_t n, hipSt
 
 uint64_t render_last_rays(Renderer& r, hipStream_t st) {
     if (r.last_levels == 0) return 0;
+    // Levels 0 .. last_levels-1 plus the cache frame's training paths at
+    // their first two levels (slots 62 and 63, zeroed by every frame).
     uint32_t h[64];
-    PT_HIP(hipMemcpyAsync(h, r.counters.as<void>(), r.last_levels * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    PT_HIP(hipMemcpyAsync(h, r.counters.as<void>(), 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     PT_HIP(hipStreamSynchronize(st));
     uint64_t total = 0;
-    for (uint32_t k = 0; k < r.last_levels; ++k) total += h[k];
-    return total;
+    for (uint32_t k = 0; k < std::min(r.last_levels, kTrainLevel0Slot); ++k) total += h[k];
+    return total + h[kTrainLevel0Slot] + h[kTrainLevel1Slot];
 }
 
 RenderTuning& render_tuning() {
