@@ -7,19 +7,27 @@ fBm (871,200 fp32 triangles).  Rays, BVH and outputs are resident in HBM before
 the timed region.  Beside it: LBVH build ms (2^20-triangle soup and the mesh)
 and the CPU oracle on a bounded sample of the same rays.
 
-Further legs: progressive render spp/s, the 512-scene batched build and
-frame, and the radiance cache's time-to-RMSE against the plain tracer.
+Further legs: progressive render spp/s, the 2.88M-triangle hairball, the
+512-scene batched build and frame, and the radiance cache's time-to-RMSE
+against the plain tracer (1920x1080, 65,536 training paths per frame).  The
+CPU leg checks the timed results against the oracle at full size (the 16M
+closest hits of the mesh and of the hairball, render passes at the legs'
+first seeds) and reports a "parity" object beside the roofline.
 
 Prints one JSON line.  Options: --gpus N (only 1 is supported), --steps K,
---warmup W, --rays R, --no-cpu (skip the CPU baseline), --no-batch,
---no-cache, --cache-full, --profile (short run for rocprofv3: 2 warmup + 3
-steps, no CPU baseline, no batched or cache legs).
+--warmup W, --rays R, --no-cpu (skip the CPU baseline and the parity
+checks), --no-batch, --no-hair, --no-cache, --cache-small (960x540 with
+16,384 training paths), --profile (short run for rocprofv3: 2 warmup + 3
+steps, no CPU baseline, no hairball, batched or cache legs).
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cuda-path-tracer_amd"))
@@ -70,9 +78,9 @@ def cache_leg(ctx, stream, args, seed_mesh):
     import ptamd
     from ptamd import hip
 
-    w, h = (1920, 1080) if args.cache_full else (960, 540)
-    ntrain = 65536 if args.cache_full else 16384
-    ref_spp = args.cache_ref_spp or (4096 if args.cache_full else 8192)
+    w, h = (960, 540) if args.cache_small else (1920, 1080)
+    ntrain = 16384 if args.cache_small else 65536
+    ref_spp = args.cache_ref_spp or 8192
     extra, steps, frames, plain_max = 8, args.cache_steps, 300, 1024
     rt, ri = ptamd.gen_sphere(660, seed_mesh, room=True)
     st = ptamd.Settings(1 + extra, 3, 1.0, 1)
@@ -242,6 +250,40 @@ def cache_leg(ctx, stream, args, seed_mesh):
     return out
 
 
+KERNEL_SOURCES = ("cuda-path-tracer_amd/csrc/traverse.hpp", "cuda-path-tracer_amd/csrc/trace.hip",
+                  "cuda-path-tracer_amd/csrc/devmath.hpp")
+
+
+def kernel_source_sha():
+    """SHA-256 over the closest-hit kernel's sources: ties a committed
+    rocprofv3 traffic measurement to the code it was taken on."""
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def timed(stream, fn, reps=1):
+    """ms per call of fn() on `stream` (HIP events)."""
+    from ptamd import hip
+
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    e0.destroy()
+    e1.destroy()
+    return ms
+
+
+def same_bits(a, b):
+    return bool(np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,12 +297,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched leg")
     ap.add_argument("--no-hair", action="store_true", help="skip the 2.88M-triangle hairball leg")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache leg")
-    ap.add_argument("--cache-full", action="store_true",
-                    help="radiance-cache leg at 1920x1080 with 65,536 training samples per frame "
-                         "(default: 960x540, training scaled to 16,384)")
+    ap.add_argument("--cache-small", action="store_true",
+                    help="radiance-cache leg at 960x540 with 16,384 training paths per frame "
+                         "(default: config 4's 1920x1080 with 65,536)")
     ap.add_argument("--cache-steps", type=int, default=4, help="Adam steps per cache frame (1..64)")
     ap.add_argument("--cache-ref-spp", type=int, default=0,
-                    help="reference spp of the cache leg (default 8192, 4096 with --cache-full)")
+                    help="reference spp of the cache leg (default 8192, two halves of 4096)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported (this build runs on one GPU; got {args.gpus})",
@@ -270,8 +312,6 @@ def main():
         args.steps, args.warmup, args.no_cpu, args.no_tune, args.no_batch = 3, 2, True, True, True
         args.no_hair = True
         args.no_cache = True
-
-    import numpy as np
 
     import ptamd
     from ptamd import hip
@@ -295,10 +335,10 @@ def main():
     # Visit counts of the reference-order traversal on exactly these rays
     # (equal to the oracle's, which the parity tests check) -> algorithmic bytes.
     cnt = ctx.trace_closest_counted(d_rays.ptr, n, stream=stream.handle)
-    iters, boxes, tri_tests = (int(x) for x in cnt)
+    iters, boxes, tri_tests, max_depth = (int(x) for x in cnt)
     bytes_per_launch = 24 * n + 8 * n + 4 * iters + 24 * boxes + 36 * tri_tests
     log(f"[bench] per ray: {iters / n:.2f} node visits, {boxes / n:.2f} box tests, "
-        f"{tri_tests / n:.2f} triangle tests -> {bytes_per_launch / n:.0f} B/ray")
+        f"{tri_tests / n:.2f} triangle tests -> {bytes_per_launch / n:.0f} B/ray; deepest stack {max_depth}")
 
     def step():
         ctx.trace_closest_device(d_rays.ptr, n, d_id.ptr, d_t.ptr, 0, stream.handle)
@@ -308,7 +348,7 @@ def main():
     if not args.no_tune:
         from ptamd import autotune
 
-        best, table = autotune.tune_staged(step, stream, autotune.trace_stages(), reps=4)
+        best, table = autotune.tune_staged(step, stream, autotune.trace_stages(autotune.device_cus()), reps=4)
         tuning = {"chosen": best, "candidates": len(table),
                   "best_ms": round(min(ms for _, ms in table), 4),
                   "worst_ms": round(max(ms for _, ms in table), 4)}
@@ -340,8 +380,8 @@ def main():
         hip.synchronize()
         a.free()
         b.free()
-    hits = d_id.to_host()
-    hit_rate = float((hits != ptamd.INVALID).mean())
+    gpu_ids, gpu_t = d_id.to_host(), d_t.to_host()  # the timed launches' results
+    hit_rate = float((gpu_ids != ptamd.INVALID).mean())
     log(f"[bench] {mrays:.1f} Mrays/s ({ms_per_step:.3f} ms/launch, wall {wall:.2f}s), hit rate {hit_rate:.3f}")
     # The same launch through the host-buffer entry point (pageable copies
     # over PCIe both ways included): not the metric, recorded beside it.
@@ -353,12 +393,29 @@ def main():
         host_mrays = round(n / (time.time() - t1) / 1e6, 1)
         log(f"[bench] host-buffer entry point (PCIe copies included): {host_mrays} Mrays/s")
 
-    # LBVH build time (HIP events around the build kernels).
+    # LBVH build: kernel time of a rebuild on reserved buffers (HIP events),
+    # and the first build of a fresh context (allocation, upload and build,
+    # host wall clock), against the 320 B/triangle algorithmic bytes.
     build_ms_mesh = float(np.median([ctx.build_bvh() for _ in range(5)]))
     soup, soup_infos = ptamd.gen_soup(1 << 20, 1, emitter=False)
+    fresh = ptamd.Context(0)
+    t1 = time.time()
+    fresh.upload_scene(soup, soup_infos)
+    first_build_ms = (time.time() - t1) * 1e3
+    fresh.close()
     ctx.upload_scene(soup, soup_infos)
     build_ms_1m = float(np.median([ctx.build_bvh() for _ in range(10)]))
-    log(f"[bench] LBVH build: {build_ms_1m:.3f} ms @ 2^20 soup, {build_ms_mesh:.3f} ms @ 871,200 mesh")
+
+    def build_roofline(ntri, ms_):
+        gbs = 320.0 * ntri / (ms_ * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_triangle": 320}
+
+    lbvh = {"soup_2^20": {"ms": round(build_ms_1m, 4), "roofline": build_roofline(len(soup), build_ms_1m),
+                          "first_upload_and_build_ms": round(first_build_ms, 2)},
+            "mesh_871200": {"ms": round(build_ms_mesh, 4), "roofline": build_roofline(len(tris), build_ms_mesh)}}
+    log(f"[bench] LBVH build: {build_ms_1m:.3f} ms @ 2^20 soup ({lbvh['soup_2^20']['roofline']['frac']} of HBM; "
+        f"first build with allocation and upload {first_build_ms:.1f} ms), {build_ms_mesh:.3f} ms @ 871,200 mesh")
 
     # Progressive render legs (spp/s): config 0 (16 passes per frame) and
     # config 1 (one pass per frame), frames replayed as captured hipGraphs.
@@ -390,21 +447,16 @@ def main():
         if not args.no_tune:
             from ptamd import autotune
 
-            space = autotune.render_space()
+            space = autotune.render_space(autotune.device_cus())
             saved = {k: ptamd.tune_get(k) for k in space}
             best, table = autotune.tune(lambda: ctx.render_passes(w, h, seeds[:per_frame], dirs[:per_frame]),
                                         stream, space, reps=3)
             rtune = {"chosen": best, "candidates": len(table)}
         ctx.reset_frames()
         stream.synchronize()
-        r0, r1 = hip.Event(), hip.Event()
-        r0.record(stream)
-        for f in range(frames):
-            sl = slice(f * per_frame, (f + 1) * per_frame)
-            ctx.render_passes(w, h, seeds[sl], dirs[sl])
-        r1.record(stream)
-        r1.synchronize()
-        rms = r0.elapsed_ms(r1)
+        rms = timed(stream, lambda: [ctx.render_passes(w, h, seeds[f * per_frame:(f + 1) * per_frame],
+                                                       dirs[f * per_frame:(f + 1) * per_frame])
+                                     for f in range(frames)])
         frame_rays = ctx.last_frame_rays()  # path segments of the last frame
         render[name] = {"ms_per_pass": round(rms / passes, 3), "spp_per_s": round(passes / (rms * 1e-3), 2),
                         "Msamples_per_s": round(w * h * passes / (rms * 1e-3) / 1e6, 2), "passes": passes,
@@ -425,18 +477,20 @@ def main():
             for k, v in saved.items():
                 ptamd.tune_set(k, v)
 
-    # Config 2: 10,000-strand hairball (2,880,000 triangles, heavily
-    # overlapping boxes): build, the 16M-ray closest-hit microbench (same ray
-    # generator as config 1) with its algorithmic bytes, and a 1920x1080
-    # 6-bounce render.
-    hair = None
+    # Config 2: 10,000-strand hairball (2,880,000 triangles + a 2-triangle
+    # emitter, heavily overlapping boxes): build, the 16M-ray closest-hit
+    # microbench (same ray generator as config 1) with its algorithmic bytes,
+    # and a 1920x1080 6-bounce render.
+    hair, hair_parity = None, None
     if not args.no_hair:
         ht, hi = ptamd.gen_hairball(10000, 144, 2025, light=True)
-        ctx.upload_scene(ht, hi, settings=ptamd.Settings(6, ptamd.INVALID, 1.0, 1))
+        hst = ptamd.Settings(6, ptamd.INVALID, 1.0, 1)
+        ctx.upload_scene(ht, hi, settings=hst)
         h_build = float(np.median([ctx.build_bvh() for _ in range(3)]))
         hb0, hb1 = ptamd.scene_bounds(ht)
         hn = n
-        hrays = hip.DeviceArray.from_host(ptamd.gen_shell_rays(hn, seed_rays + 1, hb0, hb1, 1.5))
+        hrays_host = ptamd.gen_shell_rays(hn, seed_rays + 1, hb0, hb1, 1.5)
+        hrays = hip.DeviceArray.from_host(hrays_host)
         hcnt = ctx.trace_closest_counted(hrays.ptr, hn, stream=stream.handle)
         hbytes = 24 * hn + 8 * hn + 4 * int(hcnt[0]) + 24 * int(hcnt[1]) + 36 * int(hcnt[2])
 
@@ -445,13 +499,8 @@ def main():
 
         hstep()
         stream.synchronize()
-        h0, h1 = hip.Event(), hip.Event()
-        h0.record(stream)
-        for _ in range(3):
-            hstep()
-        h1.record(stream)
-        h1.synchronize()
-        h_ms = h0.elapsed_ms(h1) / 3
+        h_ms = timed(stream, hstep, 3)
+        hair_parity = (d_id.to_host(), d_t.to_host())
         hrays.free()
         w2, h2, eye2, tgt2 = 1920, 1080, (0.0, 0.3, -3.2), (0.0, 0.0, 0.0)
         ctx.set_primary_rays(ptamd.camera_corners(eye2, tgt2, (0, 1, 0), 60.0, w2 / h2), eye2)
@@ -460,23 +509,20 @@ def main():
         hseeds = np.arange(2000, 2003, dtype=np.uint64)
         ctx.render_passes(w2, h2, hseeds[:1], hdirs[:1])  # warm-up (allocation, capture)
         ctx.reset_frames()
-        h0.record(stream)
-        for p in range(1, 3):
-            ctx.render_passes(w2, h2, hseeds[p:p + 1], hdirs[p:p + 1])
-        h1.record(stream)
-        h1.synchronize()
-        hr_ms = h0.elapsed_ms(h1) / 2
+        hr_ms = timed(stream, lambda: [ctx.render_passes(w2, h2, hseeds[p:p + 1], hdirs[p:p + 1])
+                                       for p in range(1, 3)]) / 2
         hsegs = ctx.last_frame_rays()
         h_err = None
         if not args.profile:
             acc, spp = ctx.read_accumulator(w2, h2)
             h_err = render_rmse(ctx, acc / spp, w2, h2, eye2, tgt2, 128, 16, 8 * 10 ** 8) + (int(spp),)
         hair = {"triangles": int(len(ht)), "build_ms": round(h_build, 3),
+                "build_roofline": build_roofline(len(ht), h_build),
                 "closest_hit_Mrays_per_s": round(hn / (h_ms * 1e-3) / 1e6, 1), "closest_hit_ms": round(h_ms, 3),
                 "visits_per_ray": round(int(hcnt[0]) / hn, 3), "box_tests_per_ray": round(int(hcnt[1]) / hn, 3),
-                "tri_tests_per_ray": round(int(hcnt[2]) / hn, 3),
+                "tri_tests_per_ray": round(int(hcnt[2]) / hn, 3), "deepest_stack": int(hcnt[3]),
                 "algorithmic_GB_per_s": round(hbytes / (h_ms * 1e-3) / 1e9, 1),
-                "hbm_frac": round(hbytes / (h_ms * 1e-3) / 1e9 / 8000.0, 4),
+                "hbm_frac": round(hbytes / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "render_1920x1080_b6": {"ms_per_pass": round(hr_ms, 3), "spp_per_s": round(1e3 / hr_ms, 2),
                                         "segments_per_pass": int(hsegs),
                                         "Msegments_per_s": round(hsegs / (hr_ms * 1e-3) / 1e6, 1)}}
@@ -487,8 +533,9 @@ def main():
             f"Mrays/s ({hair['visits_per_ray']} visits/ray); render 1080p b6 {hr_ms:.2f} ms/pass")
 
     # Config 3: 512 soups (log-uniform 1,024..262,144 triangles), one batched
-    # build vs one build per scene; one frame tracing every scene (a
-    # 256x256-path pass each, 8 bounces, RR from 3) vs one frame per scene.
+    # build vs one build per scene (all 512); one frame tracing every scene
+    # (a 512x512-path pass each: 2^18 camera rays plus their bounces, 8
+    # bounces, RR from 3) vs one frame per scene (all 512).
     batched = None
     if not args.no_batch:
         bt, bi, boffs = ptamd.gen_batch(512, 3)
@@ -497,127 +544,144 @@ def main():
         ctx.build_bvh()
         b_ms = float(np.median([ctx.build_bvh() for _ in range(3)]))
         eye, tgt = (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)
-        bw = bh = 256
+        bw = bh = 512
         cdirs = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, 1.0)
         ctx.set_primary_rays(cdirs, eye)
         scenes = np.arange(512, dtype=np.uint32)
         seeds = np.arange(512, dtype=np.uint64)
-        ctx.render_passes(bw, bh, seeds[:1], None, scenes[:1])  # allocate queues
+        ctx.render_passes(bw, bh, seeds, None, scenes)  # warm-up (allocation, capture)
         ctx.reset_frames()
-        f0, f1 = hip.Event(), hip.Event()
-        f0.record(stream)
-        ctx.render_passes(bw, bh, seeds, None, scenes)
-        f1.record(stream)
-        f1.synchronize()
-        frame_ms = f0.elapsed_ms(f1)
+        frame_ms = timed(stream, lambda: ctx.render_passes(bw, bh, seeds, None, scenes))
         rays_b = ctx.last_frame_rays()
-        # one scene per frame (launch sequence per scene)
         ctx.reset_frames()
-        f0.record(stream)
-        for s in range(0, 512, 8):
-            ctx.render_passes(bw, bh, seeds[s:s + 1], None, scenes[s:s + 1])
-        f1.record(stream)
-        f1.synchronize()
-        single_ms = f0.elapsed_ms(f1) * 8  # every 8th scene, scaled to 512
-        # one build per scene
+        ctx.render_passes(bw, bh, seeds[:1], None, scenes[:1])  # one-pass frame: capture
+        single_ms = timed(stream, lambda: [ctx.render_passes(bw, bh, seeds[s:s + 1], None, scenes[s:s + 1])
+                                           for s in range(512)])
         sb = []
-        for s in range(0, 512, 16):
-            t = bt[boffs[s]:boffs[s + 1]]
-            ctx.upload_scene(t, bi)
+        for s in range(512):
+            ctx.upload_scene(bt[boffs[s]:boffs[s + 1]], bi)
             sb.append(ctx.build_bvh())
-        single_build_ms = float(np.sum(sb)) * 16
-        batched = {"scenes": 512, "triangles": int(len(bt)),
+        single_build_ms = float(np.sum(sb))
+        batched = {"scenes": 512, "triangles": int(len(bt)), "camera_rays_per_scene": bw * bh,
                    "build_ms": round(b_ms, 3), "scenes_per_s_built": round(512 / (b_ms * 1e-3), 1),
+                   "build_roofline": build_roofline(len(bt), b_ms),
                    "one_scene_per_launch_build_ms": round(single_build_ms, 3),
                    "frame_ms": round(frame_ms, 3), "frame_rays": rays_b,
                    "aggregate_Mrays_per_s": round(rays_b / (frame_ms * 1e-3) / 1e6, 1),
                    "one_scene_per_frame_ms": round(single_ms, 3),
-                   "one_scene_per_frame_Mrays_per_s_est": round(rays_b / (single_ms * 1e-3) / 1e6, 1)}
+                   "one_scene_per_frame_Mrays_per_s": round(rays_b / (single_ms * 1e-3) / 1e6, 1)}
+        del bt, bi
         log(f"[bench] batched: build {b_ms:.2f} ms ({512 / (b_ms * 1e-3):.0f} scenes/s) vs "
             f"{single_build_ms:.2f} ms one per launch; frame {frame_ms:.2f} ms "
             f"({batched['aggregate_Mrays_per_s']} Mrays/s) vs {single_ms:.2f} ms one scene per frame")
 
     cache = None if args.no_cache else cache_leg(ctx, stream, args, seed_mesh)
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_py as orc
 
-        o = orc.build(tris, bmin, bmax)
+        fast = orc.fast_library_usable()
+        lib = os.path.basename(orc.select_library(fast))
         threads = orc.THREADS
-        probe = rays[: 1 << 16]
+        parity = {}
+        o = orc.build(tris, bmin, bmax)
+        # One full pass over every timed ray: its results are the parity
+        # check of the timed launches, its time the first sample.
         t1 = time.time()
-        orc.trace(tris, o, probe, threads=threads)
-        rate = len(probe) / max(time.time() - t1, 1e-6)
-        # About cpu_seconds of host work: a prefix of the rays, or (when all
-        # of them take less) the whole set traced `reps` times.
-        m = int(min(n, max(len(probe), rate * args.cpu_seconds)))
-        reps = 1
-        if m == n:
-            t1 = time.time()
-            orc.trace(tris, o, rays, threads=threads)
-            reps = max(1, int(round(args.cpu_seconds / max(time.time() - t1, 1e-6))))
-        sample = rays[:m]
+        oids, ot, _ = orc.trace(tris, o, rays, threads=threads)
+        first = time.time() - t1
+        parity["c1_16M_closest_hit"] = bool(np.array_equal(oids, gpu_ids) and same_bits(ot, gpu_t))
+        reps = max(1, int(round(args.cpu_seconds / max(first, 1e-6))))
         t1 = time.time()
         for _ in range(reps):
-            orc.trace(tris, o, sample, threads=threads)
+            orc.trace(tris, o, rays, threads=threads)
         dt = time.time() - t1
-        what = f"all {n} rays, traced {reps} times" if m == n else f"first {m} of the same {n} rays"
-        cpu = {"value": round(m * reps / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"{what}, oracle reference-order traversal, OpenMP {threads} threads, {dt:.1f} s"}
-        log(f"[bench] CPU oracle: {cpu['value']} Mrays/s on {threads} threads ({what}, {dt:.1f}s)")
+        cpu = {"value": round(n * reps / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"all {n} rays traced {reps} times by the oracle's reference-order traversal "
+                         f"({lib}: g++ -O3 -march=x86-64-v3, no FP contraction), OpenMP {threads} threads "
+                         f"(the box's CPU share for one GPU), {dt:.1f} s"}
+        log(f"[bench] CPU oracle: {cpu['value']} Mrays/s on {threads} threads ({reps} x {n} rays, {dt:.1f}s); "
+            f"parity of the timed 16M hits: {parity['c1_16M_closest_hit']}")
+        if hair_parity is not None:
+            ob = orc.build(ht, hb0, hb1)
+            hids, htt, _ = orc.trace(ht, ob, hrays_host, threads=threads)
+            parity["c2_16M_closest_hit"] = bool(np.array_equal(hids, hair_parity[0]) and same_bits(htt, hair_parity[1]))
+            log(f"[bench] parity of the hairball's 16M hits: {parity['c2_16M_closest_hit']}")
 
         # The other metrics' CPU restatements on the same host (bounded runs):
-        # the LBVH build of the 2^20-triangle soup and the two render legs.
-        soup, _ = ptamd.gen_soup(1 << 20, 1, emitter=False)
+        # the LBVH build of the 2^20-triangle soup and the render legs' first
+        # passes, each also rendered on the GPU for a bit-exact parity check.
         sb0, sb1 = ptamd.scene_bounds(soup)
         t1 = time.time()
-        orc.build(soup, sb0, sb1)
+        so = orc.build(soup, sb0, sb1)
         cpu_build_ms = (time.time() - t1) * 1e3
+        ctx.upload_scene(soup, soup_infos)
+        g = ctx.get_bvh()
+        parity["lbvh_soup_2^20"] = all(same_bits(g[k], so[k]) for k in g)
         cpu_render = {}
-        for name, (rt, ri), st, (w, h), passes, eye, tgt in (
-            ("c0_soup_256x256_b4", ptamd.gen_soup(65536, 0), ptamd.Settings(4, ptamd.INVALID, 1.0, 1), (256, 256),
-             4, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)),
-            ("c1_room_1024x1024_b8_rr3", ptamd.gen_sphere(660, seed_mesh, room=True), ptamd.Settings(8, 3, 1.0, 1),
-             (1024, 1024), 1, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0))):
+        cpu_legs = [("c0_soup_256x256_b4", ptamd.gen_soup(65536, 0), ptamd.Settings(4, ptamd.INVALID, 1.0, 1),
+                     (256, 256), 4, (0.5, 0.5, -1.5), (0.5, 0.5, 0.0), 5, 1000),
+                    ("c1_room_1024x1024_b8_rr3", ptamd.gen_sphere(660, seed_mesh, room=True),
+                     ptamd.Settings(8, 3, 1.0, 1), (1024, 1024), 1, (0.0, 0.0, -1.9), (0.0, 0.0, 0.0), 5, 1000)]
+        if hair is not None:
+            cpu_legs.append(("c2_hair_1920x1080_b6", (ht, hi), hst, (1920, 1080), 1, (0.0, 0.3, -3.2),
+                             (0.0, 0.0, 0.0), 6, 2000))
+        for name, (rt, ri), st, (w, h), passes, eye, tgt, jseed, seed0 in cpu_legs:
             ob = orc.build(rt, *ptamd.scene_bounds(rt))
             acc = np.zeros((h, w, 3))
             tot = {"segments": 0, "visits": 0, "boxes": 0, "tris": 0, "hits": 0}
+            pdirs = np.stack([ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h,
+                                                   *ptamd.pass_jitter(jseed, p, w, h)) for p in range(passes)])
             t1 = time.time()
             for p in range(passes):
-                d = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h, *ptamd.pass_jitter(5, p, w, h))
-                _, sts = orc.render_pass_stats(rt, ob, ri, st, w, h, d, eye, p, 1000 + p, acc, threads=threads)
+                _, sts = orc.render_pass_stats(rt, ob, ri, st, w, h, pdirs[p], eye, p, seed0 + p, acc,
+                                               threads=threads)
                 for k in tot:
                     tot[k] += sts[k]
             dt = time.time() - t1
             cpu_render[name] = {"ms_per_pass": round(dt * 1e3 / passes, 1), "spp_per_s": round(passes / dt, 3),
                                 "Msamples_per_s": round(w * h * passes / dt / 1e6, 3), "passes": passes}
+            # The same passes on the GPU (one frame), accumulators compared.
+            ctx.upload_scene(rt, ri, settings=st)
+            ctx.set_primary_rays(pdirs[0], eye)
+            ctx.render_passes(w, h, np.arange(seed0, seed0 + passes, dtype=np.uint64), pdirs)
+            gacc, _ = ctx.read_accumulator(w, h)
+            parity[f"{name}_{passes}pass"] = bool(np.array_equal(gacc, acc))
             # Algorithmic bytes of a pass (closest-hit bytes per segment with
             # the reference-order traversal's counts, FP64 accumulate per
-            # pixel, shading fetch per hit), from the oracle's passes at the
-            # GPU legs' first seeds and jitters; roofline of the GPU pass.
+            # pixel, shading fetch per hit) from these oracle passes; roofline
+            # of the GPU legs.
             pass_bytes = (32 * tot["segments"] + 4 * tot["visits"] + 24 * tot["boxes"] + 36 * tot["tris"]
                           + 48 * w * h * passes + 24 * tot["hits"]) / passes
-            for gname in (name, name + "_16pf"):
-                g = render.get(gname)
-                if g:
-                    gbs = pass_bytes / (g["ms_per_pass"] * 1e-3) / 1e9
-                    g.update({"algorithmic_bytes_per_pass": int(pass_bytes), "algorithmic_GB_per_s": round(gbs, 1),
-                              "hbm_frac": round(gbs / 8000.0, 4),
-                              "bytes_from": f"oracle traversal counts of {passes} pass(es) at the same seeds"})
+            targets = [render.get(name), render.get(name + "_16pf")]
+            if name.startswith("c2") and hair is not None:
+                targets.append(hair["render_1920x1080_b6"])
+            for gleg in targets:
+                if gleg:
+                    gbs = pass_bytes / (gleg["ms_per_pass"] * 1e-3) / 1e9
+                    gleg.update({"algorithmic_bytes_per_pass": int(pass_bytes), "algorithmic_GB_per_s": round(gbs, 1),
+                                 "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                                 "bytes_from": f"oracle traversal counts of {passes} pass(es) at the same seeds"})
         cpu["other_metrics"] = {"lbvh_build_ms_soup_2^20": round(cpu_build_ms, 1), "render": cpu_render,
                                 "threads": threads}
         log(f"[bench] CPU oracle: LBVH build {cpu_build_ms:.0f} ms @ 2^20; render "
-            + ", ".join(f"{k} {v['ms_per_pass']} ms/pass" for k, v in cpu_render.items()))
+            + ", ".join(f"{k} {v['ms_per_pass']} ms/pass" for k, v in cpu_render.items()) + f"; parity {parity}")
 
     # HBM bytes per launch of the same kernel from the committed rocprofv3
-    # FETCH_SIZE/WRITE_SIZE runs (calibrated against a 256 MiB copy).
-    traffic = None
+    # FETCH_SIZE/WRITE_SIZE runs (calibrated against a 256 MiB copy), used
+    # only while the kernel sources are the ones it was measured on.
+    traffic, traffic_note = None, "no committed measurement for this ray count"
     tf = os.path.join(ROOT, "profiles", "closest_hit_c1_16M_hbm_bytes.json")
     if os.path.exists(tf) and n == 16 * 1024 * 1024:
         with open(tf) as f:
-            traffic = int(json.load(f)["bytes_per_launch"])
+            rec = json.load(f)
+        if rec.get("source_sha256") == kernel_source_sha():
+            traffic = int(rec["bytes_per_launch"])
+            traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE of this kernel source (profiles/)"
+        else:
+            traffic_note = "committed measurement is of other kernel sources: not used"
 
     out = {
         "metric": "closest-hit Mrays/s @871K tris",
@@ -636,18 +700,19 @@ def main():
                                "vs 660x660-band fBm UV sphere, 871,200 fp32 triangles",
                    "rays": n, "triangles": int(len(tris)), "seed_mesh": seed_mesh, "seed_rays": seed_rays},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
                      "algorithmic_bytes": int(bytes_per_launch)},
         "cpu_baseline": cpu,
+        "parity": parity,
         "tuning": tuning,
-        "lbvh_build_ms": {"soup_2^20": round(build_ms_1m, 4), "mesh_871200": round(build_ms_mesh, 4)},
+        "lbvh_build": lbvh,
         "render": render,
         "batched_scenes": batched,
         "hairball": hair,
         "radiance_cache": cache,
         "host_buffers_Mrays_per_s": host_mrays,
         "traversal": {"visits_per_ray": round(iters / n, 3), "box_tests_per_ray": round(boxes / n, 3),
-                      "tri_tests_per_ray": round(tri_tests / n, 3),
+                      "tri_tests_per_ray": round(tri_tests / n, 3), "deepest_stack": max_depth,
                       "algorithmic_bytes_per_ray": round(bytes_per_launch / n, 1), "hit_rate": round(hit_rate, 4)},
     }
     ctx.close()

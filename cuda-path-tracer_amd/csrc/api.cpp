@@ -46,6 +46,54 @@ hipStream_t pick(pt_context* ctx, void* stream) {
     return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
 }
 
+// The ray queue block of closest-hit launches on stream `st` (allocated on
+// the stream's first trace).
+uint32_t* trace_queue(pt_context* ctx, hipStream_t st) {
+    pt::DevBuf& q = ctx->trace_queues[st];
+    if (q.capacity() == 0) q.reserve(pt::trace_queue_bytes());
+    return q.as<uint32_t>();
+}
+
+// Uploads and builds run on the context stream; a trace on another stream
+// waits for the last of them.  A stream under graph capture is left alone:
+// work captured into a graph is ordered by whoever launches the graph.
+void order_after_build(pt_context* ctx, hipStream_t st) {
+    if (st == ctx->stream || !ctx->ready_ev) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    PT_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return;
+    PT_HIP(hipStreamWaitEvent(st, ctx->ready_ev, 0));
+}
+
+void mark_ready(pt_context* ctx) {
+    if (!ctx->ready_ev) PT_HIP(hipEventCreateWithFlags(&ctx->ready_ev, hipEventDisableTiming));
+    PT_HIP(hipEventRecord(ctx->ready_ev, ctx->stream));
+}
+
+// Reads and clears the scene's status word after the context stream's work.
+uint32_t take_status(pt_context* ctx) {
+    pt::DevBuf& sb = ctx->scene.status;
+    if (sb.capacity() == 0) return 0u;
+    uint32_t flags = 0;
+    PT_HIP(hipMemcpyAsync(&flags, sb.as<void>(), sizeof flags, hipMemcpyDeviceToHost, ctx->stream));
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    if (flags) {
+        PT_HIP(hipMemsetAsync(sb.as<void>(), 0, sizeof(uint32_t), ctx->stream));
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return flags;
+}
+
+// At a synchronisation point: a raised status flag becomes PT_E_INTERNAL.
+void check_status(pt_context* ctx) {
+    const uint32_t f = take_status(ctx);
+    if (f == 0u) return;
+    std::string m = "device status:";
+    if (f & PT_STATUS_STACK_OVERFLOW) m += " traversal stack deeper than 64 entries;";
+    if (f & PT_STATUS_REFIT_OVERFLOW) m += " refit level list overflow;";
+    throw pt::Error(PT_E_INTERNAL, m + " results of the affected calls are incomplete");
+}
+
 }  // namespace
 
 extern "C" {
@@ -101,7 +149,15 @@ int pt_set_stream(pt_context* ctx, void* stream) {
 
 int pt_synchronize(pt_context* ctx) {
     if (!ctx) return PT_E_INVALID_ARG;
-    return guarded(ctx, [&] { PT_HIP(hipStreamSynchronize(ctx->stream)); });
+    return guarded(ctx, [&] {
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        check_status(ctx);
+    });
+}
+
+int pt_get_status(pt_context* ctx, uint32_t* flags) {
+    if (!ctx || !flags) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] { *flags = take_status(ctx); });
 }
 
 namespace {
@@ -149,6 +205,8 @@ void upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* off
             mbox[6 * static_cast<size_t>(k) + 3 + a] = 1.0f / (b[3 + a] - b[a]);
         }
     }
+    if (s.status.capacity() == 0) s.status.reserve(256);
+    PT_HIP(hipMemsetAsync(s.status.as<void>(), 0, sizeof(uint32_t), st));
     s.off.reserve((nscenes + 1) * sizeof(uint32_t));
     s.morton_box.reserve(mbox.size() * sizeof(float));
     PT_HIP(hipMemcpyAsync(s.off.as<void>(), offs, (nscenes + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -172,7 +230,9 @@ void upload_scenes(pt_context* ctx, const pt_triangle* tris, const uint32_t* off
     }
     if (settings) ctx->renderer.settings = *settings;
     pt::build_lbvh(s, st);
+    mark_ready(ctx);
     PT_HIP(hipStreamSynchronize(st));
+    check_status(ctx);
     s.ready = true;
     // CUDAPathTracer::upload resets the frame counter (cuda_path_tracer.cu:469).
     pt::render_reset(ctx->renderer, st);
@@ -279,9 +339,11 @@ int pt_build_bvh(pt_context* ctx, float* ms_out) {
         PT_HIP(hipEventRecord(ctx->ev0, st));
         pt::build_lbvh(ctx->scene, st);
         PT_HIP(hipEventRecord(ctx->ev1, st));
+        mark_ready(ctx);
         if (ms_out) {
             PT_HIP(hipEventSynchronize(ctx->ev1));
             PT_HIP(hipEventElapsedTime(ms_out, ctx->ev0, ctx->ev1));
+            check_status(ctx);
         }
     });
 }
@@ -322,7 +384,9 @@ int pt_trace_closest_scenes_device(pt_context* ctx, const float* d_rays, const u
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         require(n == 0 || (d_rays != nullptr && d_scene != nullptr), "rays or scene ids NULL");
         require(n < (1u << 31), "at most 2^31 - 1 rays per call");
-        pt::trace_closest_scenes(ctx->scene, d_rays, d_scene, n, d_id, d_t, d_uv, pick(ctx, stream));
+        hipStream_t st = pick(ctx, stream);
+        order_after_build(ctx, st);
+        pt::trace_closest_scenes(ctx->scene, trace_queue(ctx, st), d_rays, d_scene, n, d_id, d_t, d_uv, st);
     });
 }
 
@@ -333,7 +397,9 @@ int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n, ui
         if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
         require(n == 0 || d_rays != nullptr, "rays is NULL");
         require(n < (1u << 31), "at most 2^31 - 1 rays per call");
-        pt::trace_closest(ctx->scene, d_rays, n, d_id, d_t, d_uv, pick(ctx, stream));
+        hipStream_t st = pick(ctx, stream);
+        order_after_build(ctx, st);
+        pt::trace_closest(ctx->scene, trace_queue(ctx, st), d_rays, n, d_id, d_t, d_uv, st);
     });
 }
 
@@ -352,12 +418,14 @@ int pt_trace_closest(pt_context* ctx, const float* rays, uint32_t n, uint32_t* i
         ctx->io_t.reserve(nn * 4);
         ctx->io_uv.reserve(nn * 8);
         PT_HIP(hipMemcpyAsync(ctx->io_rays.as<void>(), rays, nn * 24, hipMemcpyHostToDevice, st));
-        pt::trace_closest(ctx->scene, ctx->io_rays.as<float>(), n, id ? ctx->io_id.as<uint32_t>() : nullptr,
-                          t ? ctx->io_t.as<float>() : nullptr, uv ? ctx->io_uv.as<float>() : nullptr, st);
+        pt::trace_closest(ctx->scene, trace_queue(ctx, st), ctx->io_rays.as<float>(), n,
+                          id ? ctx->io_id.as<uint32_t>() : nullptr, t ? ctx->io_t.as<float>() : nullptr,
+                          uv ? ctx->io_uv.as<float>() : nullptr, st);
         if (id) PT_HIP(hipMemcpyAsync(id, ctx->io_id.as<void>(), nn * 4, hipMemcpyDeviceToHost, st));
         if (t) PT_HIP(hipMemcpyAsync(t, ctx->io_t.as<void>(), nn * 4, hipMemcpyDeviceToHost, st));
         if (uv) PT_HIP(hipMemcpyAsync(uv, ctx->io_uv.as<void>(), nn * 8, hipMemcpyDeviceToHost, st));
         PT_HIP(hipStreamSynchronize(st));
+        check_status(ctx);
     });
 }
 
@@ -369,14 +437,16 @@ int pt_trace_closest_counted(pt_context* ctx, const float* d_rays, uint32_t n, u
         require(n == 0 || d_rays != nullptr, "rays is NULL");
         require(n < (1u << 31), "at most 2^31 - 1 rays per call");
         hipStream_t st = pick(ctx, stream);
-        pt::DevBuf c;
-        c.reserve(3 * sizeof(unsigned long long));
-        PT_HIP(hipMemsetAsync(c.as<void>(), 0, 3 * sizeof(unsigned long long), st));
-        pt::trace_closest_counted(ctx->scene, d_rays, n, d_id, d_t, d_uv,
+        order_after_build(ctx, st);
+        pt::DevBuf& c = ctx->count_buf;
+        c.reserve(4 * sizeof(unsigned long long));
+        PT_HIP(hipMemsetAsync(c.as<void>(), 0, 4 * sizeof(unsigned long long), st));
+        pt::trace_closest_counted(ctx->scene, trace_queue(ctx, st), d_rays, n, d_id, d_t, d_uv,
                                   c.as<unsigned long long>(), st);
-        PT_HIP(hipMemcpyAsync(counters, c.as<void>(), 3 * sizeof(unsigned long long),
+        PT_HIP(hipMemcpyAsync(counters, c.as<void>(), 4 * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
         PT_HIP(hipStreamSynchronize(st));
+        check_status(ctx);
     });
 }
 
@@ -511,6 +581,7 @@ int pt_read_accumulator(pt_context* ctx, double* rgb_sum, uint32_t* spp) {
                                   hipMemcpyDeviceToHost, ctx->stream));
         }
         PT_HIP(hipStreamSynchronize(ctx->stream));
+        check_status(ctx);
         if (spp) *spp = r.spp;
     });
 }
@@ -575,6 +646,18 @@ bool tune_valid(const uint32_t* slot, uint32_t v) {
     return v != 0;
 }
 }  // namespace
+
+int pt_device_compute_units(int device, uint32_t* cus) {
+    if (!cus) return PT_E_INVALID_ARG;
+    return guarded(nullptr, [&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+            throw pt::Error(PT_E_NO_DEVICE, "no such HIP device");
+        int v = 0;
+        PT_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device));
+        *cus = static_cast<uint32_t>(v);
+    });
+}
 
 int pt_tune_set(const char* name, uint32_t value) {
     uint32_t* slot = tune_slot(name);

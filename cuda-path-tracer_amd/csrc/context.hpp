@@ -8,6 +8,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <map>
 #include <utility>
 #include <stdexcept>
 #include <string>
@@ -108,7 +109,6 @@ struct DeviceBvh {
     DevBuf nodes, tris, root_box;
     DevBuf tile_hist, digit_total;  // radix sort scratch
     DevBuf span_list, span_top, span_count;  // refit: per-level node lists
-    DevBuf ray_counter;                      // persistent traversal: ray queue block
 };
 
 // One or more scenes: scene s owns triangles [off[s], off[s+1]) and its own
@@ -129,6 +129,9 @@ struct DeviceScene {
     DevBuf texinfo;                        // uint4 per texture: first texel, w, h, 0
     uint32_t ndetail = 0, ntex = 0;
     uint32_t max_id = 0;                   // largest triangle id of the upload
+    // Status word the kernels OR PT_STATUS_* flags into (a violated bound:
+    // traversal stack, refit list); sticky until pt_get_status reads it.
+    DevBuf status;
     DeviceBvh bvh;
     bool ready = false;
     uint32_t nscenes() const { return static_cast<uint32_t>(off_host.size()) - 1; }
@@ -155,15 +158,22 @@ struct TraceTuning {
 };
 TraceTuning& trace_tuning();
 
-// Closest hit over a device ray batch (rays: 6 floats each).
-void trace_closest(const DeviceScene& scene, const float* d_rays, uint32_t n, uint32_t* d_id,
-                   float* d_t, float* d_uv, hipStream_t stream);
+// Bytes of one persistent-traversal ray queue block (slice counters and
+// used-up mask); a launch zeroes its block first, so concurrent launches
+// need distinct blocks.
+size_t trace_queue_bytes();
+// Closest hit over a device ray batch (rays: 6 floats each), rays handed
+// out through the queue block `queue`.
+void trace_closest(const DeviceScene& scene, uint32_t* queue, const float* d_rays, uint32_t n,
+                   uint32_t* d_id, float* d_t, float* d_uv, hipStream_t stream);
 // Closest hit with a scene index per ray (batched scenes).
-void trace_closest_scenes(const DeviceScene& scene, const float* d_rays, const uint32_t* d_scene,
-                          uint32_t n, uint32_t* d_id, float* d_t, float* d_uv, hipStream_t stream);
+void trace_closest_scenes(const DeviceScene& scene, uint32_t* queue, const float* d_rays,
+                          const uint32_t* d_scene, uint32_t n, uint32_t* d_id, float* d_t, float* d_uv,
+                          hipStream_t stream);
 // Same traversal, also counting inner iterations, box tests and triangle
-// tests into d_counters[3] (uint64, accumulated).
-void trace_closest_counted(const DeviceScene& scene, const float* d_rays, uint32_t n,
+// tests into d_counters[0..2] (uint64, accumulated) and the deepest
+// deferred-node stack into d_counters[3] (maximum).
+void trace_closest_counted(const DeviceScene& scene, uint32_t* queue, const float* d_rays, uint32_t n,
                            uint32_t* d_id, float* d_t, float* d_uv,
                            unsigned long long* d_counters, hipStream_t stream);
 
@@ -286,7 +296,14 @@ struct pt_context {
     pt::Renderer renderer;
     pt::RadianceCache cache;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Recorded on `stream` after every scene upload and LBVH build: a trace
+    // on another stream waits for it first.
+    hipEvent_t ready_ev = nullptr;
+    // Ray queue block of the closest-hit launches per stream (launches on one
+    // stream run in order; on different streams each has its own block).
+    std::map<hipStream_t, pt::DevBuf> trace_queues;
     pt::DevBuf io_rays, io_id, io_t, io_uv;  // staging for host-pointer traces
+    pt::DevBuf count_buf;                    // counters of pt_trace_closest_counted
     pt_context() = default;
     pt_context(const pt_context&) = delete;
     pt_context& operator=(const pt_context&) = delete;
@@ -296,6 +313,7 @@ struct pt_context {
         if (stream) (void)hipStreamSynchronize(stream);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        if (ready_ev) (void)hipEventDestroy(ready_ev);
         if (own_stream) (void)hipStreamDestroy(own_stream);
     }
 };

@@ -307,7 +307,8 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
                                                     float4* __restrict__ nodes,
                                                     uint32_t* __restrict__ span_list,
                                                     uint32_t* __restrict__ span_top,
-                                                    uint32_t* __restrict__ span_count) {
+                                                    uint32_t* __restrict__ span_count,
+                                                    uint32_t* __restrict__ status) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const uint32_t sc = scene_of[g];
@@ -370,6 +371,7 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
         if ((same >> (threadIdx.x & 63u)) & 1ull) {
             const uint32_t slot = first + static_cast<uint32_t>(__popcll(same & lt));
             if (slot < kMidCap) span_list[static_cast<size_t>(b) * kMidCap + slot] = g;
+            else raise_status(status, PT_STATUS_REFIT_OVERFLOW);  // bound argued above
         }
         pending &= ~same;
     }
@@ -636,7 +638,6 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     b.nodes.reserve(static_cast<size_t>(n) * 64);
     b.tris.reserve(static_cast<size_t>(n) * 48);
     b.root_box.reserve(static_cast<size_t>(nscenes) * 6 * sizeof(float));
-    b.ray_counter.reserve(kQueueWords * sizeof(uint32_t));
     const uint32_t num_tiles = div_up(n, kSortTile);
     b.tile_hist.reserve(static_cast<size_t>(num_tiles) * 256 * 4);
     b.digit_total.reserve(256 * 4);
@@ -695,7 +696,7 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                            s.off.as<uint32_t>(), b.scene_of.as<uint32_t>(), b.children.as<uint32_t>(),
                            b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                            b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.span_list.as<uint32_t>(),
-                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>());
+                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>(), s.status.as<uint32_t>());
         PT_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_refit_local, dim3(div_up(n, kRefitBlock)), dim3(kRefitBlock), 0, st,

@@ -46,6 +46,7 @@ struct FrameConst {
     const pt_triangle_detail* detail;  // per original id, or null: pt_triangle_info alone
     const uint32_t* texels;            // RGBA8 texels of all textures
     const uint4* texinfo;              // per texture: first texel, width, height, 0
+    uint32_t* status;                  // the scene's status word (PT_STATUS_* flags)
 };
 
 // Structure-of-arrays queue of rays (origin, direction, path index).
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
                                                             uint32_t level, uint32_t refill, uint32_t pool) {
     __shared__ uint32_t s_stack[LDS][kPersistThreads];
     const uint32_t count = counters[level];
-    TraceCount cnt{0, 0, 0};
+    TraceCount cnt{0, 0, 0, 0};
     auto load = [&](uint32_t i, V3& o, V3& d, uint32_t& single) -> uint32_t {
         o = v3(q.ox[i], q.oy[i], q.oz[i]);
         d = v3(q.dx[i], q.dy[i], q.dz[i]);
@@ -181,8 +182,8 @@ __global__ __launch_bounds__(kPersistThreads) void k_extend(const FrameConst fc,
         hq.u[i] = h.u;
         hq.v[i] = h.v;
     };
-    persistent_trace<false, LDS>(nodes, tris, count, ray_counters + level * kQueueWords, refill, pool, s_stack,
-                                 load, store, cnt);
+    persistent_trace<false, LDS>(nodes, tris, count, ray_counters + level * kQueueWords, fc.status, refill, pool,
+                                 s_stack, load, store, cnt);
 }
 
 __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]); }
@@ -192,9 +193,9 @@ __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]);
 // (closest-hit t < FLT_MAX, trace.h:45-50) shadows it.  The any-hit search
 // answers the same question and stops at its first hit.
 __device__ __forceinline__ V3 light_L(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                      uint32_t b, uint32_t e, V3 pos, const pt_dir_light& lt) {
+                                      uint32_t b, uint32_t e, V3 pos, const pt_dir_light& lt, uint32_t* status) {
     const V3 ld = ld3(lt.direction);
-    if (any_hit_in(nodes, tris, b, e, sub(pos, scale(ld, 1e-3f)), neg(ld))) return v3(0.0f, 0.0f, 0.0f);
+    if (any_hit_in(nodes, tris, b, e, sub(pos, scale(ld, 1e-3f)), neg(ld), status)) return v3(0.0f, 0.0f, 0.0f);
     return scale(ld3(lt.color), lt.intensity);
 }
 
@@ -302,7 +303,8 @@ __device__ __forceinline__ void bounce_hit(HitShade& hs, V3 d, const FrameConst&
     } else if (LIGHTS) {
         for (uint32_t k = 0; k < fc.nlights; ++k) {
             const V3 ld = ld3(lights[k].direction);
-            L = add(L, scale(light_L(nodes, tris, sb, se, hs.pos, lights[k]), gmax(dot(neg(ld), hs.nrm), 0.0f)));
+            L = add(L, scale(light_L(nodes, tris, sb, se, hs.pos, lights[k], fc.status),
+                             gmax(dot(neg(ld), hs.nrm), 0.0f)));
         }
     }
     absorbed = mul(absorbed, hs.albedo);
@@ -357,7 +359,7 @@ __device__ __forceinline__ void primary_hit(HitShade& hs, V3 d, const FrameConst
         V3 Lc = v3(0.0f, 0.0f, 0.0f);
         for (uint32_t k = 0; k < fc.nlights; ++k) {
             const V3 ld = ld3(lights[k].direction);
-            Lc = add(Lc, light_L(nodes, tris, sb, se, hs.pos, lights[k]));
+            Lc = add(Lc, light_L(nodes, tris, sb, se, hs.pos, lights[k], fc.status));
             const V3 hw = scale(sub(neg(d), ld), 0.5f);
             acc = add(acc, blinn_phong(Lc, hs.albedo, Lc, v3(0.5f, 0.5f, 0.5f), hs.nrm, neg(ld), hw, hs.spec));
         }
@@ -505,7 +507,7 @@ __global__ __launch_bounds__(kSuffixThreads) void k_train_paths(const FrameConst
         V3 acc = v3(0.0f, 0.0f, 0.0f);
         atomicAdd(counters + kTrainLevel0Slot, 1u);
         for (uint32_t b = 0;; ++b) {
-            const TraceHit h = trace_ray_in(nodes, tris, sb, se, o, d);
+            const TraceHit h = trace_ray_in(nodes, tris, sb, se, o, d, fc.status);
             if (h.id == kInvalid) break;
             HitShade hs = hit_shade<DETAIL>(fc, infos, h.id, h.t, h.u, h.v, o, d);
             if (b == 0) {
@@ -895,6 +897,7 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
     fc.intensity = r.settings.indirect_intensity;
     fc.stratified = r.settings.stratified;
     fc.nlights = scene.nlights;
+    fc.status = scene.status.as<uint32_t>();
     set_detail(fc, scene);
 
     if (!render_tuning().graphs) {
@@ -1032,6 +1035,7 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     cf.fc.intensity = r.settings.indirect_intensity;
     cf.fc.stratified = r.settings.stratified;
     cf.fc.nlights = scene.nlights;
+    cf.fc.status = scene.status.as<uint32_t>();
     set_detail(cf.fc, scene);
     cf.steps = cs.train_steps;
     cf.train = (cs.flags & PT_CACHE_TRAIN) != 0;

@@ -382,26 +382,44 @@ int pt_gen_batch_counts(uint64_t seed, uint32_t nscenes, uint32_t lo, uint32_t h
     return static_cast<int>(nscenes);
 }
 
-int pt_camera_corners(const float* eye, const float* target, const float* up, float fovy_deg,
-                      float aspect, float jx, float jy, float* dirs) {
-    if (!eye || !target || !up || !dirs) return PT_E_INVALID_ARG;
-    V3 e{eye[0], eye[1], eye[2]};
-    V3 f = normalize(sub(V3{target[0], target[1], target[2]}, e));
-    V3 r = normalize(cross(V3{up[0], up[1], up[2]}, f));
-    V3 u = cross(f, r);
-    float th = std::tan(0.5f * fovy_deg * 3.14159265358979f / 180.0f);
+int pt_camera_corners_basis(const float* right, const float* up, const float* look, float fovy,
+                            float aspect, float jx, float jy, int left_handed, float* dirs) {
+    if (!right || !up || !look || !dirs) return PT_E_INVALID_ARG;
+    // screenToRay's corner directions: the unprojected NDC corners (shifted by
+    // the jitter translation) lie on +-look + ndc.x * aspect * t * right +
+    // ndc.y * t * up, t = tan(fovy / 2) as glm::perspective computes it; the
+    // left-handed projection looks along +look, the right-handed along -look.
+    const float sgn = left_handed ? 1.0f : -1.0f;
+    const V3 r{right[0], right[1], right[2]};
+    const V3 u{up[0], up[1], up[2]};
+    const V3 b{sgn * look[0], sgn * look[1], sgn * look[2]};
+    const float th = std::tan(fovy / 2.0f);
     const float cx[4] = {-1.0f, 1.0f, 1.0f, -1.0f};
     const float cy[4] = {-1.0f, -1.0f, 1.0f, 1.0f};
     for (int k = 0; k < 4; ++k) {
-        float sx = (cx[k] - jx) * th * aspect;
-        float sy = (cy[k] - jy) * th;
-        V3 d{f.x + sx * r.x + sy * u.x, f.y + sx * r.y + sy * u.y, f.z + sx * r.z + sy * u.z};
+        const float sx = (cx[k] - jx) * th * aspect;
+        const float sy = (cy[k] - jy) * th;
+        V3 d{b.x + sx * r.x + sy * u.x, b.y + sx * r.y + sy * u.y, b.z + sx * r.z + sy * u.z};
         d = normalize(d);
         dirs[3 * k + 0] = d.x;
         dirs[3 * k + 1] = d.y;
         dirs[3 * k + 2] = d.z;
     }
     return 4;
+}
+
+int pt_camera_corners(const float* eye, const float* target, const float* up, float fovy_deg,
+                      float aspect, float jx, float jy, float* dirs) {
+    if (!eye || !target || !up || !dirs) return PT_E_INVALID_ARG;
+    // Transform::lookAt (Transform.cpp:78-87): look = normalize(target - eye),
+    // right = normalize(up x look), up' = look x right; seen through the
+    // left-handed projection, so the rays leave towards the target.
+    const V3 e{eye[0], eye[1], eye[2]};
+    const V3 f = normalize(sub(V3{target[0], target[1], target[2]}, e));
+    const V3 r = normalize(cross(V3{up[0], up[1], up[2]}, f));
+    const V3 u = cross(f, r);
+    const float rr[3] = {r.x, r.y, r.z}, uu[3] = {u.x, u.y, u.z}, ll[3] = {f.x, f.y, f.z};
+    return pt_camera_corners_basis(rr, uu, ll, fovy_deg * 3.14159265358979f / 180.0f, aspect, jx, jy, 1, dirs);
 }
 
 int pt_pass_jitter(uint64_t seed, uint32_t pass, int w, int h, float* jx, float* jy) {

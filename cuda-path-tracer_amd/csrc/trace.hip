@@ -12,37 +12,43 @@ namespace {
 
 constexpr int kTraceThreads = 256;
 
+// Wave-reduces the per-lane counts: one atomic per wave and counter
+// (sums for visits, boxes and triangles, the maximum for the stack depth).
+__device__ __forceinline__ void add_counts(const TraceCount& cnt, unsigned long long* counters) {
+    unsigned long long a = cnt.iters, b = cnt.boxes, c = cnt.tris;
+    uint32_t dmax = cnt.depth;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+        dmax = max(dmax, static_cast<uint32_t>(__shfl_down(static_cast<int>(dmax), off, 64)));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(counters + 0, a);
+        atomicAdd(counters + 1, b);
+        atomicAdd(counters + 2, c);
+        atomicMax(counters + 3, static_cast<unsigned long long>(dmax));
+    }
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(
     const float4* __restrict__ nodes, const float4* __restrict__ tris, uint32_t ntri,
     const float* __restrict__ rays, uint32_t nrays, uint32_t* __restrict__ out_id,
     float* __restrict__ out_t, float2* __restrict__ out_uv,
-    unsigned long long* __restrict__ counters) {
-    TraceCount cnt{0, 0, 0};
+    unsigned long long* __restrict__ counters, uint32_t* __restrict__ status) {
+    TraceCount cnt{0, 0, 0, 0};
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nrays; i += stride) {
         const float* r = rays + 6 * static_cast<size_t>(i);
         const V3 o = v3(r[0], r[1], r[2]);
         const V3 d = v3(r[3], r[4], r[5]);
-        const TraceHit h = trace_ray<COUNT>(nodes, tris, ntri, o, d, cnt);
+        const TraceHit h = trace_ray<COUNT>(nodes, tris, ntri, o, d, cnt, status);
         if (out_id) out_id[i] = h.id;
         if (out_t) out_t[i] = h.t;
         if (out_uv) out_uv[i] = make_float2(h.u, h.v);
     }
-    if (COUNT) {
-        // Wave-reduce the per-lane counts, one atomic per wave and counter.
-        unsigned long long a = cnt.iters, b = cnt.boxes, c = cnt.tris;
-        for (int off = 32; off > 0; off >>= 1) {
-            a += __shfl_down(a, off, 64);
-            b += __shfl_down(b, off, 64);
-            c += __shfl_down(c, off, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(counters + 0, a);
-            atomicAdd(counters + 1, b);
-            atomicAdd(counters + 2, c);
-        }
-    }
+    if (COUNT) add_counts(cnt, counters);
 }
 
 // Persistent closest-hit kernel over an AoS ray batch (ntri >= 2).
@@ -51,11 +57,11 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
     const float4* __restrict__ nodes, const float4* __restrict__ tris, const float* __restrict__ rays,
     uint32_t nrays, uint32_t* __restrict__ out_id, float* __restrict__ out_t,
     float2* __restrict__ out_uv, uint32_t* __restrict__ ray_counter, uint32_t refill, uint32_t pool,
-    unsigned long long* __restrict__ counters) {
+    unsigned long long* __restrict__ counters, uint32_t* __restrict__ status) {
     __shared__ uint32_t s_stack[LDS][kPersistThreads];
-    TraceCount cnt{0, 0, 0};
+    TraceCount cnt{0, 0, 0, 0};
     persistent_trace<COUNT, LDS>(
-        nodes, tris, nrays, ray_counter, refill, pool, s_stack,
+        nodes, tris, nrays, ray_counter, status, refill, pool, s_stack,
         [&](uint32_t i, V3& o, V3& d, uint32_t&) -> uint32_t {
             const float* rp = rays + 6 * static_cast<size_t>(i);
             o = v3(rp[0], rp[1], rp[2]);
@@ -68,19 +74,7 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
             if (out_uv) out_uv[i] = make_float2(h.u, h.v);
         },
         cnt);
-    if (COUNT) {
-        unsigned long long a = cnt.iters, b = cnt.boxes, c = cnt.tris;
-        for (int off = 32; off > 0; off >>= 1) {
-            a += __shfl_down(a, off, 64);
-            b += __shfl_down(b, off, 64);
-            c += __shfl_down(c, off, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(counters + 0, a);
-            atomicAdd(counters + 1, b);
-            atomicAdd(counters + 2, c);
-        }
-    }
+    if (COUNT) add_counts(cnt, counters);
 }
 
 // Persistent closest hit over rays tagged with a scene of a batch: the ray
@@ -90,11 +84,11 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_scenes(
     const float4* __restrict__ nodes, const float4* __restrict__ tris, const uint32_t* __restrict__ off,
     uint32_t nscenes, const float* __restrict__ rays, const uint32_t* __restrict__ scene, uint32_t nrays,
     uint32_t* __restrict__ out_id, float* __restrict__ out_t, float2* __restrict__ out_uv,
-    uint32_t* __restrict__ ray_counter, uint32_t refill, uint32_t pool) {
+    uint32_t* __restrict__ ray_counter, uint32_t refill, uint32_t pool, uint32_t* __restrict__ status) {
     __shared__ uint32_t s_stack[LDS][kPersistThreads];
-    TraceCount cnt{0, 0, 0};
+    TraceCount cnt{0, 0, 0, 0};
     persistent_trace<false, LDS>(
-        nodes, tris, nrays, ray_counter, refill, pool, s_stack,
+        nodes, tris, nrays, ray_counter, status, refill, pool, s_stack,
         [&](uint32_t i, V3& o, V3& d, uint32_t& single) -> uint32_t {
             const float* rp = rays + 6 * static_cast<size_t>(i);
             o = v3(rp[0], rp[1], rp[2]);
@@ -117,20 +111,21 @@ __global__ __launch_bounds__(kPersistThreads) void k_trace_scenes(
 __global__ __launch_bounds__(64) void k_zero_queue(uint32_t* queue) { zero_queue(queue, 1u); }
 
 template <bool COUNT>
-void launch_trace(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
+void launch_trace(const DeviceScene& s, uint32_t* queue, const float* d_rays, uint32_t n, uint32_t* d_id,
                   float* d_t, float* d_uv, unsigned long long* d_counters, hipStream_t st) {
     if (n == 0) return;
+    uint32_t* status = s.status.as<uint32_t>();
     if (s.bvh.n < 2) {
         const uint32_t blocks = static_cast<uint32_t>((static_cast<uint64_t>(n) + kTraceThreads - 1) /
                                                       kTraceThreads);
         hipLaunchKernelGGL(k_trace<COUNT>, dim3(blocks), dim3(kTraceThreads), 0, st,
                            s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), s.bvh.n, d_rays, n,
-                           d_id, d_t, reinterpret_cast<float2*>(d_uv), d_counters);
+                           d_id, d_t, reinterpret_cast<float2*>(d_uv), d_counters, status);
         PT_HIP(hipGetLastError());
         return;
     }
     const TraceTuning& tune = trace_tuning();
-    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, st, s.bvh.ray_counter.as<uint32_t>());
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, st, queue);
     const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
     const uint32_t blocks =
         static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
@@ -138,12 +133,14 @@ void launch_trace(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_
                 : (tune.lds <= 16 ? k_trace_persistent<COUNT, 16> : k_trace_persistent<COUNT, 32>);
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kPersistThreads), 0, st,
                        s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), d_rays, n, d_id, d_t,
-                       reinterpret_cast<float2*>(d_uv), s.bvh.ray_counter.as<uint32_t>(),
-                       std::min(64u, std::max(1u, tune.refill)), pool_for(tune.pool, n, blocks), d_counters);
+                       reinterpret_cast<float2*>(d_uv), queue, std::min(64u, std::max(1u, tune.refill)),
+                       pool_for(tune.pool, n, blocks), d_counters, status);
     PT_HIP(hipGetLastError());
 }
 
 }  // namespace
+
+size_t trace_queue_bytes() { return kQueueWords * sizeof(uint32_t); }
 
 TraceTuning& trace_tuning() {
     static TraceTuning t = [] {
@@ -158,11 +155,11 @@ TraceTuning& trace_tuning() {
     return t;
 }
 
-void trace_closest_scenes(const DeviceScene& s, const float* d_rays, const uint32_t* d_scene,
+void trace_closest_scenes(const DeviceScene& s, uint32_t* queue, const float* d_rays, const uint32_t* d_scene,
                           uint32_t n, uint32_t* d_id, float* d_t, float* d_uv, hipStream_t st) {
     if (n == 0) return;
     const TraceTuning& tune = trace_tuning();
-    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, st, s.bvh.ray_counter.as<uint32_t>());
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, st, queue);
     const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
     const uint32_t blocks =
         static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
@@ -170,19 +167,20 @@ void trace_closest_scenes(const DeviceScene& s, const float* d_rays, const uint3
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kPersistThreads), 0, st,
                        s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), s.off.as<uint32_t>(),
                        s.nscenes(), d_rays, d_scene, n, d_id, d_t, reinterpret_cast<float2*>(d_uv),
-                       s.bvh.ray_counter.as<uint32_t>(), std::min(64u, std::max(1u, tune.refill)), pool_for(tune.pool, n, blocks));
+                       queue, std::min(64u, std::max(1u, tune.refill)), pool_for(tune.pool, n, blocks),
+                       s.status.as<uint32_t>());
     PT_HIP(hipGetLastError());
 }
 
-void trace_closest(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
+void trace_closest(const DeviceScene& s, uint32_t* queue, const float* d_rays, uint32_t n, uint32_t* d_id,
                    float* d_t, float* d_uv, hipStream_t st) {
-    launch_trace<false>(s, d_rays, n, d_id, d_t, d_uv, nullptr, st);
+    launch_trace<false>(s, queue, d_rays, n, d_id, d_t, d_uv, nullptr, st);
 }
 
-void trace_closest_counted(const DeviceScene& s, const float* d_rays, uint32_t n, uint32_t* d_id,
-                           float* d_t, float* d_uv, unsigned long long* d_counters,
+void trace_closest_counted(const DeviceScene& s, uint32_t* queue, const float* d_rays, uint32_t n,
+                           uint32_t* d_id, float* d_t, float* d_uv, unsigned long long* d_counters,
                            hipStream_t st) {
-    launch_trace<true>(s, d_rays, n, d_id, d_t, d_uv, d_counters, st);
+    launch_trace<true>(s, queue, d_rays, n, d_id, d_t, d_uv, d_counters, st);
 }
 
 }  // namespace pt

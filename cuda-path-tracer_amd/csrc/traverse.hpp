@@ -7,7 +7,9 @@
 #pragma once
 
 #include <algorithm>
+
 #include "devmath.hpp"
+#include "pt_api.h"
 
 namespace pt {
 
@@ -17,14 +19,23 @@ struct TraceHit {
     float u, v;     // barycentrics of v1, v2
 };
 
+// Measurement counts of one lane: node visits, box tests, triangle tests and
+// the deepest deferred-node stack reached.
 struct TraceCount {
-    uint32_t iters, boxes, tris;
+    uint32_t iters, boxes, tris, depth;
 };
 
 // Stack bound: every deferred node is the sibling of a distinct ancestor of
 // the current node, and a radix tree over < 2^30 augmented 62-bit keys is at
-// most 63 internal levels deep.
+// most 63 internal levels deep.  A push past the bound is not made; it sets
+// kStatusStackOverflow in the caller's status word (which the host reports
+// as PT_E_INTERNAL), so a violated bound cannot pass silently.
 constexpr int kStackMax = 64;
+constexpr uint32_t kStatusStackOverflow = PT_STATUS_STACK_OVERFLOW;
+
+__device__ __forceinline__ void raise_status(uint32_t* status, uint32_t bit) {
+    if (status) atomicOr(status, bit);
+}
 
 template <bool COUNT, bool FINITE>
 __device__ __forceinline__ float child_box(V3 o, V3 dinv, float a, float b, float c, float d,
@@ -47,7 +58,8 @@ __device__ __forceinline__ float tri_hit(const float4* __restrict__ tris, uint32
 template <bool COUNT, bool FINITE>
 __device__ __forceinline__ TraceHit traverse_impl(const float4* __restrict__ nodes,
                                                   const float4* __restrict__ tris, V3 o, V3 d,
-                                                  V3 dinv, TraceCount& cnt, uint32_t root = 0) {
+                                                  V3 dinv, TraceCount& cnt, uint32_t* status,
+                                                  uint32_t root = 0) {
     float minT = kMinTInit;
     uint32_t best = kInvalid;
     float bu = 0.0f, bv = 0.0f;
@@ -93,7 +105,12 @@ __device__ __forceinline__ TraceHit traverse_impl(const float4* __restrict__ nod
                 node = cc;
                 if (hf <= minT) {
                     if (jf) {
-                        if (sp < kStackMax) stack[sp++] = cf;
+                        if (sp < kStackMax) {
+                            stack[sp++] = cf;
+                            if (COUNT) cnt.depth = max(cnt.depth, static_cast<uint32_t>(sp));
+                        } else {
+                            raise_status(status, kStatusStackOverflow);
+                        }
                     } else if (hf < minT || idf < best) {
                         minT = hf; best = idf; bu = uf; bv = vf;
                     }
@@ -129,7 +146,7 @@ __device__ __forceinline__ TraceHit traverse_impl(const float4* __restrict__ nod
 template <bool COUNT>
 __device__ __forceinline__ TraceHit trace_ray(const float4* __restrict__ nodes,
                                               const float4* __restrict__ tris, uint32_t ntri, V3 o,
-                                              V3 d, TraceCount& cnt) {
+                                              V3 d, TraceCount& cnt, uint32_t* status) {
     TraceHit miss{kFltMax, kInvalid, 0.0f, 0.0f};
     if (ntri == 0) return miss;
     if (ntri == 1) {
@@ -143,17 +160,17 @@ __device__ __forceinline__ TraceHit trace_ray(const float4* __restrict__ nodes,
     const V3 dinv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     const bool finite = __builtin_fabsf(dinv.x) <= kFltMax && __builtin_fabsf(dinv.y) <= kFltMax &&
                         __builtin_fabsf(dinv.z) <= kFltMax;
-    if (finite) return traverse_impl<COUNT, true>(nodes, tris, o, d, dinv, cnt);
-    return traverse_impl<COUNT, false>(nodes, tris, o, d, dinv, cnt);
+    if (finite) return traverse_impl<COUNT, true>(nodes, tris, o, d, dinv, cnt, status);
+    return traverse_impl<COUNT, false>(nodes, tris, o, d, dinv, cnt, status);
 }
 
 // One ray in the scene whose triangles (and internal nodes) occupy slots
 // [b, e) of a batched BVH; a single scene has b = 0.
 __device__ __forceinline__ TraceHit trace_ray_in(const float4* __restrict__ nodes,
                                                  const float4* __restrict__ tris, uint32_t b, uint32_t e,
-                                                 V3 o, V3 d) {
+                                                 V3 o, V3 d, uint32_t* status) {
     TraceHit miss{kFltMax, kInvalid, 0.0f, 0.0f};
-    TraceCount cnt{0, 0, 0};
+    TraceCount cnt{0, 0, 0, 0};
     if (e - b == 0) return miss;
     if (e - b == 1) {
         float u = 0.0f, v = 0.0f;
@@ -165,8 +182,8 @@ __device__ __forceinline__ TraceHit trace_ray_in(const float4* __restrict__ node
     const V3 dinv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     const bool finite = __builtin_fabsf(dinv.x) <= kFltMax && __builtin_fabsf(dinv.y) <= kFltMax &&
                         __builtin_fabsf(dinv.z) <= kFltMax;
-    if (finite) return traverse_impl<false, true>(nodes, tris, o, d, dinv, cnt, b);
-    return traverse_impl<false, false>(nodes, tris, o, d, dinv, cnt, b);
+    if (finite) return traverse_impl<false, true>(nodes, tris, o, d, dinv, cnt, status, b);
+    return traverse_impl<false, false>(nodes, tris, o, d, dinv, cnt, status, b);
 }
 
 // Whether a ray hits any triangle of the scene in [b, e) that the closest-hit
@@ -176,7 +193,7 @@ __device__ __forceinline__ TraceHit trace_ray_in(const float4* __restrict__ node
 // one), so it visits the same boxes as this search does and finds a hit
 // exactly when this one does; this one stops at the first.
 __device__ __forceinline__ bool any_hit_in(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                          uint32_t b, uint32_t e, V3 o, V3 d) {
+                                          uint32_t b, uint32_t e, V3 o, V3 d, uint32_t* status) {
     float u, v;
     uint32_t id;
     if (e - b == 0) return false;
@@ -199,6 +216,7 @@ __device__ __forceinline__ bool any_hit_in(const float4* __restrict__ nodes, con
         const bool go0 = in0 && h0 <= kMinTInit, go1 = in1 && h1 <= kMinTInit;
         if (go0 && go1) {
             if (sp < kStackMax) stack[sp++] = l + 1;
+            else raise_status(status, kStatusStackOverflow);
             node = l;
         } else if (go0 || go1) {
             node = go0 ? l : l + 1;
@@ -235,7 +253,7 @@ template <bool COUNT, int LDS, bool FINITE>
 __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
                                            const float4* __restrict__ tris, RayState& r,
                                            uint32_t (*s_stack)[kPersistThreads],
-                                           uint32_t* spill, TraceCount& cnt) {
+                                           uint32_t* spill, TraceCount& cnt, uint32_t* status) {
     const float4* rec = nodes + 4 * static_cast<size_t>(r.node);
     const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
     const uint32_t w = __float_as_uint(r3.x);
@@ -278,7 +296,10 @@ __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
             } else if (sp < kStackMax) {
                 spill[sp - LDS] = far;
                 r.sp = sp + 1;
+            } else {
+                raise_status(status, kStatusStackOverflow);
             }
+            if (COUNT) cnt.depth = max(cnt.depth, static_cast<uint32_t>(r.sp));
         }
         return true;
     }
@@ -353,7 +374,7 @@ inline uint32_t pool_for(uint32_t pool, uint32_t nmax, uint32_t blocks) {
 template <bool COUNT, int LDS, class Load, class Store>
 __device__ __forceinline__ void persistent_trace(const float4* __restrict__ nodes,
                                                  const float4* __restrict__ tris, uint32_t nrays,
-                                                 uint32_t* __restrict__ queue,
+                                                 uint32_t* __restrict__ queue, uint32_t* status,
                                                  uint32_t refill, uint32_t pool_large,
                                                  uint32_t (*s_stack)[kPersistThreads], Load load,
                                                  Store store, TraceCount& cnt) {
@@ -470,8 +491,8 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
         // With no busy lane the wave stops once no slice has rays left; until
         // then the next pass probes the slice just chosen.
         if (__ballot(active) == 0ull && drained) break;
-        if (active && !(r.finite ? visit_step<COUNT, LDS, true>(nodes, tris, r, s_stack, spill, cnt)
-                                 : visit_step<COUNT, LDS, false>(nodes, tris, r, s_stack, spill, cnt))) {
+        if (active && !(r.finite ? visit_step<COUNT, LDS, true>(nodes, tris, r, s_stack, spill, cnt, status)
+                                 : visit_step<COUNT, LDS, false>(nodes, tris, r, s_stack, spill, cnt, status))) {
             TraceHit h;
             h.id = r.best;
             h.t = r.best == kInvalid ? kFltMax : r.minT;

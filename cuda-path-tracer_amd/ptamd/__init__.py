@@ -32,6 +32,8 @@ PT_E_UNSUPPORTED = -5
 PT_E_OOM = -6
 PT_E_INTERNAL = -7
 INVALID = 0xFFFFFFFF
+PT_STATUS_STACK_OVERFLOW = 1
+PT_STATUS_REFIT_OVERFLOW = 2
 FLT_MAX = float(np.finfo(np.float32).max)
 
 # pt_triangle (40 B) and pt_triangle_info (64 B).
@@ -102,6 +104,7 @@ _SIGS = {
     "pt_last_error": [_vp, ctypes.c_char_p, ctypes.c_size_t],
     "pt_set_stream": [_vp, _vp],
     "pt_synchronize": [_vp],
+    "pt_get_status": [_vp, _vp],
     "pt_upload_scene": [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _vp, _vp],
     "pt_upload_scenes": [_vp, _vp, _vp, _u32, _vp, _u32, _vp, _vp],
     "pt_get_bvh_scene": [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -135,6 +138,7 @@ _SIGS = {
     "pt_gen_shell_rays": [_u64, _vp, _vp, _f, _u32, _vp],
     "pt_gen_batch_counts": [_u64, _u32, _u32, _u32, _vp],
     "pt_camera_corners": [_vp, _vp, _vp, _f, _f, _f, _f, _vp],
+    "pt_camera_corners_basis": [_vp, _vp, _vp, _f, _f, _f, _f, _i, _vp],
     "pt_pass_jitter": [_u64, _u32, _i, _i, _vp, _vp],
     "pt_cache_init": [_vp, _u64, _vp, _vp],
     "pt_cache_set_params": [_vp, _vp],
@@ -155,6 +159,7 @@ _SIGS = {
     "pt_load_tga": [ctypes.c_char_p, _vp, _vp, _vp],
     "pt_asset_error": [ctypes.c_char_p, ctypes.c_size_t],
     "pt_tune_set": [ctypes.c_char_p, _u32],
+    "pt_device_compute_units": [_i, _vp],
     "pt_tune_get": [ctypes.c_char_p, _vp],
 }
 for _name, _args in _SIGS.items():
@@ -260,6 +265,16 @@ def camera_corners(eye, target, up=(0.0, 1.0, 0.0), fovy_deg=60.0, aspect=1.0, j
     return d
 
 
+def camera_corners_basis(right, up, look, fovy, aspect=1.0, jx=0.0, jy=0.0, left_handed=True):
+    """screenToRay's four corner directions for a camera rotation with
+    columns right, up, look and glm::perspective's fovy (radians)."""
+    d = np.zeros(12, np.float32)
+    right, up, look = _f32(right, 3), _f32(up, 3), _f32(look, 3)
+    _check(_lib.pt_camera_corners_basis(_ptr(right), _ptr(up), _ptr(look), fovy, aspect, jx, jy,
+                                        1 if left_handed else 0, _ptr(d)))
+    return d
+
+
 def pass_jitter(seed, pass_index, w, h):
     jx = np.zeros(1, np.float32)
     jy = np.zeros(1, np.float32)
@@ -360,6 +375,12 @@ def write_png(path, rgba, srgb=False, top_down=False):
 
 def tune_set(name, value):
     _check(_lib.pt_tune_set(name.encode(), int(value)))
+
+
+def device_compute_units(device=0):
+    v = np.zeros(1, np.uint32)
+    _check(_lib.pt_device_compute_units(device, _ptr(v)))
+    return int(v[0])
 
 
 def tune_get(name):
@@ -491,7 +512,9 @@ class Context:
                                              d_uv or None, stream or None))
 
     def trace_closest_counted(self, d_rays, n, d_id=0, d_t=0, d_uv=0, stream=0):
-        c = np.zeros(3, np.uint64)
+        """Node visits, box tests, triangle tests (totals) and the deepest
+        deferred-node stack of any ray."""
+        c = np.zeros(4, np.uint64)
         self._c(_lib.pt_trace_closest_counted(self._h, d_rays, n, d_id or None, d_t or None,
                                               d_uv or None, _ptr(c), stream or None))
         return c
@@ -569,6 +592,12 @@ class Context:
 
     def synchronize(self):
         self._c(_lib.pt_synchronize(self._h))
+
+    def get_status(self):
+        """PT_STATUS_* flags raised since the last read (cleared by reading)."""
+        v = np.zeros(1, np.uint32)
+        self._c(_lib.pt_get_status(self._h, _ptr(v)))
+        return int(v[0])
 
     def set_stream(self, stream):
         self._c(_lib.pt_set_stream(self._h, stream or None))

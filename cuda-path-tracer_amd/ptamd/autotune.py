@@ -10,7 +10,12 @@ from __future__ import annotations
 
 import itertools
 
-from . import hip, tune_get, tune_set
+from . import device_compute_units, hip, tune_get, tune_set
+
+
+def device_cus(device=0):
+    """Compute units of the device (the launch-shape search scales with it)."""
+    return device_compute_units(device)
 
 
 def _time(fn, stream, reps):

@@ -88,6 +88,10 @@ class DeviceArray:
     def zero(self):
         _ok(_hip.hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
+    def fill_bytes(self, value):
+        """Every byte set to `value` (0..255)."""
+        _ok(_hip.hipMemset(self.ptr, int(value), self.nbytes), "hipMemset")
+
     def free(self):
         if self.ptr:
             _hip.hipFree(self.ptr)

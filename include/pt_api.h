@@ -40,6 +40,16 @@ extern "C" {
 
 #define PT_INVALID_INDEX 0xFFFFFFFFu
 
+/* Device status flags (pt_get_status).  Each marks a bound that the
+ * library's arguments show cannot be exceeded; a kernel that finds one
+ * exceeded skips the step it cannot store and raises the flag, and the next
+ * synchronising entry point of the context (pt_synchronize, host-pointer
+ * traces, pt_trace_closest_counted, timed pt_build_bvh, uploads,
+ * pt_read_accumulator) returns PT_E_INTERNAL and clears it. */
+#define PT_STATUS_STACK_OVERFLOW 1u /* deferred-node stack deeper than 64 (reference: */
+                                    /* MAX_TRAVERSAL_STACK_SIZE 64, cuda_defs.h:21, unchecked) */
+#define PT_STATUS_REFIT_OVERFLOW 2u /* refit level list over its capacity */
+
 /* Triangle as the caller hands it over: three fp32 vertices plus the caller's
  * triangle id (reference: CUDA::Triangle, geometry.h:20-30 -- v0, v1, v2,
  * infoIdx; 40 bytes).  `id` indexes the shading records and is the "original
@@ -129,6 +139,9 @@ int pt_last_error(const pt_context* ctx, char* buf, size_t len);
  * context's own stream). */
 int pt_set_stream(pt_context* ctx, void* stream);
 int pt_synchronize(pt_context* ctx);
+/* Waits for the context's stream, then returns and clears the device status
+ * flags (PT_STATUS_*) raised since the last read; 0 = no bound exceeded. */
+int pt_get_status(pt_context* ctx, uint32_t* flags);
 
 /* ---- scene + LBVH --------------------------------------------------------- */
 /* Reference: CUDAPathTracer::upload -> CUDA::Scene::upload -> BVHBuilder::build
@@ -191,7 +204,12 @@ int pt_get_scene_info(pt_context* ctx, uint32_t* ntris, float* bmin, float* bmax
 /* Batch form of CUDA::trace (trace.h:73-169): rays[i] = origin.xyz, dir.xyz.
  * Outputs (any may be NULL): id = original triangle id of the closest hit
  * (PT_INVALID_INDEX on miss, equal distances -> lowest id), t (FLT_MAX on
- * miss), uv = barycentrics of v1, v2 (math_util.h:19-25). */
+ * miss), uv = barycentrics of v1, v2 (math_util.h:19-25).
+ * The `_device` forms on a stream other than the context's first wait for
+ * the context's last upload or build (unless the stream is being captured
+ * into a graph: then the caller orders it).  Each stream has its own ray
+ * queue, allocated on its first trace, so traces on different streams may
+ * overlap. */
 int pt_trace_closest(pt_context* ctx, const float* rays, uint32_t n,
                      uint32_t* id, float* t, float* uv);
 int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n,
@@ -202,7 +220,8 @@ int pt_trace_closest_scenes_device(pt_context* ctx, const float* d_rays, const u
                                    void* stream);
 /* Measurement variant (no reference counterpart): the same traversal, also
  * counting inner-loop iterations, box tests and triangle tests into
- * counters[3] (host, uint64).  Host-synchronous. */
+ * counters[0..2] (host, uint64, totals) and the deepest deferred-node stack
+ * any ray reached into counters[3].  Host-synchronous. */
 int pt_trace_closest_counted(pt_context* ctx, const float* d_rays, uint32_t n,
                              uint32_t* d_id, float* d_t, float* d_uv,
                              uint64_t* counters, void* stream);
@@ -314,6 +333,10 @@ int pt_cache_last_batch(pt_context* ctx, float* raw, float* target, uint32_t n);
  * cache's predictions (3 per pixel) and whether the pixel's path reached x2
  * (valid[w*h]); any pointer may be NULL.  For inspecting the cache. */
 int pt_cache_last_queries(pt_context* ctx, float* raw, float* pred, uint32_t* valid);
+
+/* Compute units of HIP device `device` (no reference counterpart; launch
+ * shapes and the autotuner's search space scale with it). */
+int pt_device_compute_units(int device, uint32_t* cus);
 
 /* ---- launch tuning (no reference counterpart) -------------------------------
  * Process-wide launch-shape knobs read by the kernels' host launchers; results

@@ -55,11 +55,28 @@ int pt_gen_shell_rays(uint64_t seed, const float* bmin, const float* bmax, float
 int pt_gen_batch_counts(uint64_t seed, uint32_t nscenes, uint32_t lo, uint32_t hi,
                         uint32_t* counts);
 
-/* Pinhole camera corner directions (normalised) in the order bottom-left,
- * bottom-right, top-right, top-left, through NDC corners shifted by
- * (-jx, -jy) as the reference's jitter translation does
- * (PathTracingApp.cpp:225-242).  up must not be parallel to target-eye.
- * Returns the number of corners written (4). */
+/* The four corner directions PathTracingApp::screenToRay returns
+ * (PathTracingApp.cpp:217-242: corners (0,0), (w,0), (w,h), (0,h) at the near
+ * plane -- bottom-left, bottom-right, top-right, top-left), normalised, for a
+ * camera whose rotation has columns right, up, look (CameraComponent::
+ * updateViewMatrix, CameraComponent.cpp:97-135).  NDC x runs along right,
+ * NDC y along up; with left_handed = 1 (glm's perspectiveLH, the projection
+ * the app's scene presets, its left-handed asset import and Transform::lookAt
+ * are set up for, GLM_FORCE_LEFT_HANDED at CMakeLists.txt:109) the rays leave
+ * along +look, with 0 (glm 0.9.8's default perspectiveRH, what the shipped
+ * build compiles to) along -look.  fovy is the value handed to
+ * glm::perspective, in radians (the app's setPerspective(45.0f),
+ * PathTracingApp.cpp:441, is 45 radians: tan(22.5) = 0.557, a 58.3 degree
+ * vertical field of view).  jx, jy is the NDC jitter translation; the corners
+ * move by (-jx, -jy).  Returns the number of corners written (4). */
+int pt_camera_corners_basis(const float* right, const float* up, const float* look, float fovy,
+                            float aspect, float jx, float jy, int left_handed, float* dirs);
+
+/* Pinhole camera at `eye` viewing `target`: pt_camera_corners_basis of the
+ * camera that Transform::lookAt(target, up) orients (Transform.cpp:78-87:
+ * look = normalize(target - eye), right = normalize(up x look),
+ * up' = look x right), left-handed projection, vertical field of view
+ * fovy_deg in degrees.  up must not be parallel to target - eye.  Returns 4. */
 int pt_camera_corners(const float* eye, const float* target, const float* up,
                       float fovy_deg, float aspect, float jx, float jy, float* dirs);
 

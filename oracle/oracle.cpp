@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -108,6 +109,7 @@ struct Hit {
 
 struct Counters {
     uint64_t iters = 0, boxes = 0, tris = 0;
+    uint64_t depth = 0;  // deepest deferred-node stack
 };
 
 struct Bvh {
@@ -185,7 +187,10 @@ Hit trace_one(const Bvh& bvh, V3 o, V3 d, Counters* cnt) {
                 node = ci[cl];
                 if (h[fa] <= minT) {
                     if (internal[fa]) {
+                        // The depth bound (< 64 for < 2^30 leaves) keeps sp < 128.
+                        if (sp >= 128) std::abort();
                         stack[sp++] = ci[fa];
+                        if (cnt && static_cast<uint64_t>(sp) > cnt->depth) cnt->depth = static_cast<uint64_t>(sp);
                     } else {
                         update(ci[fa], h[fa], hu[fa], hv[fa]);
                     }
@@ -631,9 +636,9 @@ void or_trace(const pt_triangle* tris, const uint32_t* perm, const uint32_t* chi
               const float* boxes, uint32_t n, const float* rays, uint32_t nrays, uint32_t* id,
               float* t, float* uv, uint64_t* counters, int threads) {
     Bvh bvh{tris, perm, children, boxes, n};
-    uint64_t it = 0, bx = 0, tr = 0;
+    uint64_t it = 0, bx = 0, tr = 0, dp = 0;
     if (threads <= 0) threads = 1;
-#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) reduction(+ : it, bx, tr)
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads) reduction(+ : it, bx, tr) reduction(max : dp)
     for (int64_t i = 0; i < static_cast<int64_t>(nrays); ++i) {
         const float* r = rays + 6 * i;
         Counters c;
@@ -647,11 +652,13 @@ void or_trace(const pt_triangle* tris, const uint32_t* perm, const uint32_t* chi
         it += c.iters;
         bx += c.boxes;
         tr += c.tris;
+        dp = std::max(dp, c.depth);
     }
     if (counters) {
         counters[0] = it;
         counters[1] = bx;
         counters[2] = tr;
+        counters[3] = dp;
     }
 }
 
@@ -1037,6 +1044,83 @@ void or_shade_detail(const pt_triangle_info* info, const pt_triangle_detail* det
     albedo[0] = si.albedo.x;
     albedo[1] = si.albedo.y;
     albedo[2] = si.albedo.z;
+}
+
+// a14: PathTracingApp::screenToRay (PathTracingApp.cpp:217-242) for the main
+// camera component.  View matrix rows right, up, look with the position
+// folded in (CameraComponent::updateViewMatrix, CameraComponent.cpp:97-135);
+// glm::perspective with -1..1 depth, tanHalfFovy computed in float from the
+// value as passed (third_party/glm/glm/gtc/matrix_transform.inl:248-300):
+// perspectiveRH (left_handed = 0, glm's default, w = -z) or perspectiveLH
+// (left_handed = 1, GLM_FORCE_LEFT_HANDED, w = +z); the jitter translate
+// times viewProj, inverted; screenToNDC (CameraComponent.cpp:149-155); start
+// and end points unprojected and divided by w; direction normalize(end -
+// start).  The matrix work is in double with a general 4x4 inverse
+// (Gauss-Jordan, partial pivoting), the reference's is in float: the
+// directions agree to a few float ulps, not bit for bit.
+void or_screen_to_ray(const float* rot, const float* pos, float fovy, float screen_w, float screen_h,
+                      float znear, float zfar, float jx, float jy, const float* p, int left_handed,
+                      float* dir) {
+    double V[4][4] = {}, P[4][4] = {}, T[4][4] = {}, M[4][4] = {}, TP[4][4] = {};
+    for (int r = 0; r < 3; ++r) {
+        double d = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            V[r][c] = rot[3 * r + c];  // row r = rotation column r (right, up, look)
+            d += static_cast<double>(pos[c]) * rot[3 * r + c];
+        }
+        V[r][3] = -d;
+    }
+    V[3][3] = 1.0;
+    const float tan_half = std::tan(fovy / 2.0f);
+    const double aspect = static_cast<double>(screen_w) / screen_h;
+    const double n = znear, f = zfar;
+    P[0][0] = 1.0 / (aspect * tan_half);
+    P[1][1] = 1.0 / tan_half;
+    P[2][2] = (left_handed ? 1.0 : -1.0) * (f + n) / (f - n);
+    P[2][3] = -(2.0 * f * n) / (f - n);
+    P[3][2] = left_handed ? 1.0 : -1.0;
+    for (int k = 0; k < 4; ++k) T[k][k] = 1.0;
+    T[0][3] = jx;
+    T[1][3] = jy;
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c)
+            for (int k = 0; k < 4; ++k) TP[r][c] += T[r][k] * P[k][c];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c)
+            for (int k = 0; k < 4; ++k) M[r][c] += TP[r][k] * V[k][c];
+    // inverse(M): Gauss-Jordan on [M | I].
+    double A[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 8; ++c) A[r][c] = c < 4 ? M[r][c] : (c - 4 == r ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+        for (int k = 0; k < 8; ++k) std::swap(A[c][k], A[piv][k]);
+        const double s = 1.0 / A[c][c];
+        for (int k = 0; k < 8; ++k) A[c][k] *= s;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double m = A[r][c];
+            for (int k = 0; k < 8; ++k) A[r][k] -= m * A[c][k];
+        }
+    }
+    const double ndc[3] = {p[0] / static_cast<double>(screen_w) * 2.0 - 1.0,
+                           p[1] / static_cast<double>(screen_h) * 2.0 - 1.0,
+                           (p[2] - n) / (f - n) * 2.0 - 1.0};
+    auto unproject = [&](double z, double* out) {
+        const double v[4] = {ndc[0], ndc[1], z, 1.0};
+        double w[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int r = 0; r < 4; ++r)
+            for (int k = 0; k < 4; ++k) w[r] += A[r][4 + k] * v[k];
+        for (int r = 0; r < 3; ++r) out[r] = w[r] / w[3];
+    };
+    double s[3], e[3];
+    unproject(ndc[2], s);
+    unproject(1.0, e);
+    const double d[3] = {e[0] - s[0], e[1] - s[1], e[2] - s[2]};
+    const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int k = 0; k < 3; ++k) dir[k] = static_cast<float>(d[k] / len);
 }
 
 }  // extern "C"

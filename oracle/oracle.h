@@ -60,7 +60,8 @@ int or_ray_aabb(const float* o, const float* d, const float* bmin, const float* 
 
 /* a7: closest hit in the reference's traversal order (trace.h:73-169) with the
  * lowest-id tie rule.  Outputs may be NULL.  counters (may be NULL) receives
- * 3 uint64 totals: inner-loop iterations, box tests, triangle tests. */
+ * 4 uint64: totals of inner-loop iterations, box tests, triangle tests, and
+ * the deepest deferred-node stack any ray reached. */
 void or_trace(const pt_triangle* tris, const uint32_t* perm, const uint32_t* children,
               const float* boxes, uint32_t n, const float* rays, uint32_t nrays,
               uint32_t* id, float* t, float* uv, uint64_t* counters, int threads);
@@ -152,6 +153,16 @@ void or_render_cache_pass_detail(const pt_triangle* tris, const uint32_t* perm, 
                                  const float* cbmin, const float* cbmax, uint32_t ntrain,
                                  uint32_t extra_bounces, double* accum, float* train_raw,
                                  float* train_target, int threads);
+
+/* a14: direction of PathTracingApp::screenToRay (PathTracingApp.cpp:217-242)
+ * for screen point p[3] (x, y in pixels, y = 0 at the bottom; z a depth
+ * between the clip planes).  rot[9] = the camera rotation's columns right,
+ * up, look (rot[0..2] = right); pos = camera position; fovy as passed to
+ * glm::perspective (radians); jx, jy = the NDC jitter translation;
+ * left_handed: glm's perspectiveLH (1) or its default perspectiveRH (0). */
+void or_screen_to_ray(const float* rot, const float* pos, float fovy, float screen_w, float screen_h,
+                      float znear, float zfar, float jx, float jy, const float* p, int left_handed,
+                      float* dir);
 
 /* Sampling helpers shared by the render, exposed for tests. */
 void or_pcg4d(const uint32_t* in, uint32_t* out);

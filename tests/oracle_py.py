@@ -48,11 +48,42 @@ _SIGS = {
     "or_pcg4d": ([_vp, _vp], None),
     "or_sincos2pi": ([_f, _vp, _vp], None),
     "or_to_hemisphere": ([_vp, _vp, _vp], None),
+    "or_screen_to_ray": ([_vp, _vp, _f, _f, _f, _f, _f, _f, _f, _vp, _i, _vp], None),
 }
-for _n, (_a, _r) in _SIGS.items():
-    _fn = getattr(_lib, _n)
-    _fn.argtypes = _a
-    _fn.restype = _r
+FAST_LIB_PATH = os.path.join(ROOT, "oracle", "liboracle_fast.so")
+
+
+def _bind(lib):
+    for _n, (_a, _r) in _SIGS.items():
+        _fn = getattr(lib, _n)
+        _fn.argtypes = _a
+        _fn.restype = _r
+    return lib
+
+
+_bind(_lib)
+
+
+def fast_library_usable():
+    """Whether liboracle_fast.so (AVX2 code) exists and this host reports
+    the instruction sets it was compiled for."""
+    if not os.path.exists(FAST_LIB_PATH):
+        return False
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = set(next(line for line in f if line.startswith("flags")).split(":", 1)[1].split())
+    except (OSError, StopIteration):
+        return False
+    return {"avx2", "fma", "bmi2", "movbe"} <= flags
+
+
+def select_library(fast):
+    """Bind every function of this module to liboracle_fast.so (fast=True,
+    for CPU baseline timings) or liboracle.so.  Returns the path in use."""
+    global _lib
+    path = FAST_LIB_PATH if fast else LIB_PATH
+    _lib = _bind(ctypes.CDLL(path))
+    return path
 
 THREADS = max(1, min(16, os.cpu_count() or 1))
 
@@ -134,7 +165,7 @@ def trace(tris, bvh, rays, counters=False, threads=THREADS):
     ids = np.zeros(n, np.uint32)
     t = np.zeros(n, np.float32)
     uv = np.zeros((n, 2), np.float32)
-    c = np.zeros(3, np.uint64) if counters else None
+    c = np.zeros(4, np.uint64) if counters else None
     _lib.or_trace(_p(tris), _p(bvh["perm"]), _p(bvh["children"]), _p(bvh["bboxes"]), len(tris),
                   _p(rays), n, _p(ids), _p(t), _p(uv), _p(c), threads)
     return (ids, t, uv, c) if counters else (ids, t, uv)
@@ -318,3 +349,14 @@ def render_cache_pass(tris, bvh, infos, settings, w, h, dirs, eye, pass_index, s
                                      _p(raw), _p(tgt), threads)
     del keep
     return accum, raw, tgt
+
+
+def screen_to_ray(rot_cols, pos, fovy, w, h, znear, zfar, jx, jy, p, left_handed=True):
+    """screenToRay's direction for screen point p (x, y, z); rot_cols = the
+    camera rotation's columns (right, up, look) as a (3, 3) array of rows;
+    left_handed: glm's perspectiveLH, else its default perspectiveRH."""
+    rot = _f32(np.asarray(rot_cols, np.float32).reshape(9))
+    out = np.zeros(3, np.float32)
+    _lib.or_screen_to_ray(_p(rot), _p(_f32(pos)), float(fovy), float(w), float(h), float(znear), float(zfar),
+                          float(jx), float(jy), _p(_f32(p)), 1 if left_handed else 0, _p(out))
+    return out

@@ -223,3 +223,41 @@ def test_cache_frame_with_detail_matches_oracle(ctx, room):
     acc_g, _ = ctx.read_accumulator(W, H)
     assert max(rel_err(acc_g, acc_o)) <= 1e-3
     ctx.upload_detail(np.zeros(0, ptamd.DETAIL_DTYPE))
+
+
+def test_cache_frame_full_room_mesh(ctx):
+    """A cache frame on config 4's scene (the 871,200-triangle mesh in the
+    room) at 96x54 with 2,048 training paths of 8 extra bounces: training
+    batch bit-identical to the oracle's, image within 1e-3 of it."""
+    tris, infos = ptamd.gen_sphere(660, 2024, room=True)
+    st = ptamd.Settings(9, 3, 1.0, 1)
+    w, h = 96, 54
+    ctx.upload_scene(tris, infos, settings=st)
+    d = ptamd.camera_corners(EYE, TGT, (0, 1, 0), 60.0, w / h)
+    ctx.set_primary_rays(d, EYE)
+    ctx.reset_frames()
+    bmin, bmax = ptamd.scene_bounds(tris)
+    ctx.cache_init(5, bmin, bmax)
+    p = random_params(23, bias=0.2)
+    ctx.cache_set_params(p)
+    ntrain, extra = 2048, 8
+    cs = ptamd.CacheSettings(ntrain, 1, extra, train=False)
+    bvh = orc.build(tris, bmin, bmax)
+    acc_o = np.zeros((h, w, 3))
+    for f in range(2):
+        seed = 4000 + f
+        say(f"[cache frame] full room mesh {w}x{h}, frame {f}")
+        ctx.render_cache_frame(w, h, seed, cs)
+        raw, tgt = ctx.cache_last_batch(ntrain)
+        acc_o, oraw, otgt = orc.render_cache_pass(tris, bvh, infos, st, w, h, d, EYE, f, seed,
+                                                  fp16_weights(p), bmin, bmax, ntrain, extra, acc_o)
+        valid = ~np.isnan(otgt[:, 0])
+        assert valid.sum() > ntrain // 2
+        np.testing.assert_array_equal(np.isnan(tgt[:, 0]), ~valid)
+        np.testing.assert_array_equal(raw[valid], oraw[valid])
+        np.testing.assert_array_equal(tgt[valid], otgt[valid])
+    acc_g, spp = ctx.read_accumulator(w, h)
+    assert spp == 2
+    err = rel_err(acc_g, acc_o)
+    say(f"  image mean relative per-channel error {err}")
+    assert acc_o.sum() > 0 and max(err) <= 1e-3

@@ -362,3 +362,138 @@ def test_render_parity_full_room_mesh(ctx):
             orc.render_pass(tris, b, infos, st, w, h, d0, eye, p, int(seeds[p]), o)
         assert spp == npass and o.sum() > 0
         assert np.array_equal(g, o)
+
+
+def test_deep_stack_trace(ctx):
+    """Trees 42 internal levels deep (tests/deep_scene.py): deferred-node
+    stacks of 40+ entries, deeper than every LDS stack setting, so pushes
+    spill to scratch.  Traced bit-exact against the oracle at trace.lds_stack
+    8, 16 and 32; the GPU's visit counters and deepest-stack counter equal the
+    oracle's; no status flag is raised."""
+    from deep_scene import BOX_MAX, BOX_MIN, deep_rays, deep_scene
+    from ptamd import hip
+
+    tris, infos = deep_scene()
+    o = check_build(ctx, tris, infos, BOX_MIN, BOX_MAX, tag="deep")
+    rays = deep_rays(8192)
+    oids, ot, ouv, oc = orc.trace(tris, o, rays, counters=True)
+    assert int(oc[3]) >= 40
+    saved = ptamd.tune_get("trace.lds_stack")
+    d_rays = hip.DeviceArray.from_host(rays)
+    try:
+        for lds in (8, 16, 32):
+            say(f"[deep stack] trace.lds_stack={lds}, oracle depth {int(oc[3])}")
+            ptamd.tune_set("trace.lds_stack", lds)
+            ids, t, uv = ctx.trace_closest(rays)
+            assert np.array_equal(ids, oids)
+            assert np.array_equal(t.view(np.uint32), ot.view(np.uint32))
+            assert np.array_equal(uv.view(np.uint32), ouv.view(np.uint32))
+            c = ctx.trace_closest_counted(d_rays.ptr, len(rays))
+            assert np.array_equal(c, oc), (c, oc)
+            assert int(c[3]) > lds
+    finally:
+        ptamd.tune_set("trace.lds_stack", saved)
+        d_rays.free()
+    assert ctx.get_status() == 0
+
+
+def test_trace_parity_c2_full_hairball(ctx):
+    """Config 2 at its full size: the 2,880,002-triangle hairball the bench
+    times, 300,000 shell rays of the bench's generator plus 100,000 interior
+    rays, bit-exact with equal visit counters."""
+    tris, infos = ptamd.gen_hairball(10000, 144, 2025, light=True)
+    assert len(tris) == 2_880_002
+    bmin, bmax = ptamd.scene_bounds(tris)
+    rays = np.concatenate([ptamd.gen_shell_rays(300000, 78, bmin, bmax, 1.5),
+                           interior_rays(100000, 79, bmin, bmax)])
+    rate = check_trace(ctx, tris, infos, rays, "c2-hair-full")
+    assert rate > 0.05
+
+
+def test_batched_config3_full_size(ctx):
+    """Config 3 as the bench runs it: 512 soups with log-uniform sizes in
+    [1,024, 262,144] (24.0M triangles) built by one launch sequence; 40
+    scenes (the smallest, the largest and every 13th) checked against their
+    own oracle builds, and traced through per-ray scene ids bit-exact; three
+    scenes rendered through the batch (8 bounces, RR from 3) equal to their
+    single-scene oracle renders."""
+    tris, infos, offs = ptamd.gen_batch(512, 3)
+    say(f"[batched c3] 512 scenes n={len(tris)}")
+    counts = np.diff(offs)
+    assert counts.min() >= 1024 and counts.max() <= 262144
+    st = ptamd.Settings(8, 3, 1.0, 1)
+    ctx.upload_scenes(tris, offs, infos, settings=st)
+    picks = sorted(set([int(np.argmin(counts)), int(np.argmax(counts))] + list(range(0, 512, 13))))
+    rays_all, sc_all, expect, builds = [], [], [], {}
+    for s in picks:
+        t = tris[offs[s]:offs[s + 1]]
+        bmin, bmax = ptamd.scene_bounds(t)
+        o = orc.build(t, bmin, bmax)
+        builds[s] = o
+        g = ctx.get_bvh_scene(s)
+        for k in ("morton", "perm", "children", "parents_leaf", "parents_internal"):
+            assert np.array_equal(g[k], o[k]), f"scene {s}: {k}"
+        assert np.array_equal(g["bboxes"].view(np.uint32), o["bboxes"].view(np.uint32)), f"scene {s}"
+        r = np.concatenate([ptamd.gen_shell_rays(2000, s, bmin, bmax, 1.3), interior_rays(2000, s, bmin, bmax)])
+        rays_all.append(r)
+        sc_all.append(np.full(len(r), s, np.uint32))
+        expect.append(orc.trace(t, o, r))
+    say(f"[batched c3] {len(picks)} scenes built bit-exact; tracing")
+    perm = np.random.default_rng(5).permutation(sum(len(r) for r in rays_all))
+    ids, t, uv = ctx.trace_closest_scenes(np.concatenate(rays_all)[perm], np.concatenate(sc_all)[perm])
+    assert np.array_equal(ids, np.concatenate([e[0] for e in expect])[perm])
+    assert np.array_equal(t.view(np.uint32), np.concatenate([e[1] for e in expect])[perm].view(np.uint32))
+    assert np.array_equal(uv.view(np.uint32), np.concatenate([e[2] for e in expect])[perm].view(np.uint32))
+    # three scenes of the batch rendered in one frame (one pass each)
+    w, h = 48, 40
+    eye, tgt = (0.5, 0.5, -1.5), (0.5, 0.5, 0.0)
+    d0 = ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h)
+    ctx.set_primary_rays(d0, eye)
+    for s in picks[:3]:
+        say(f"[batched c3] render scene {s}")
+        ctx.reset_frames()
+        ctx.render_passes(w, h, np.array([900 + s], np.uint64), None, np.array([s], np.uint32))
+        g, spp = ctx.read_accumulator(w, h)
+        o = np.zeros((h, w, 3))
+        orc.render_pass(tris[offs[s]:offs[s + 1]], builds[s], infos, st, w, h, d0, eye, 0, 900 + s, o)
+        assert spp == 1 and np.array_equal(g, o)
+
+
+def test_traces_on_user_streams(ctx):
+    """Device traces on two caller streams right after an asynchronous
+    rebuild on the context stream: each stream has its own ray queue, so the
+    two launches overlap without sharing hand-out counters, and both equal the
+    oracle (outputs pre-filled with a sentinel, so a skipped ray shows)."""
+    from ptamd import hip
+
+    tris, infos = ptamd.gen_soup(200000, 21)
+    ctx.upload_scene(tris, infos)
+    bmin, bmax = ptamd.scene_bounds(tris)
+    b = orc.build(tris, bmin, bmax)
+    r1 = ptamd.gen_shell_rays(300000, 22, bmin, bmax, 1.5)
+    r2 = interior_rays(300000, 23, bmin, bmax)
+    o1, o2 = orc.trace(tris, b, r1), orc.trace(tris, b, r2)
+    s1, s2 = hip.Stream(), hip.Stream()
+    d1, d2 = hip.DeviceArray.from_host(r1), hip.DeviceArray.from_host(r2)
+    i1, i2 = hip.DeviceArray(len(r1), np.uint32), hip.DeviceArray(len(r2), np.uint32)
+    t1, t2 = hip.DeviceArray(len(r1), np.float32), hip.DeviceArray(len(r2), np.float32)
+    try:
+        for rep in range(3):
+            say(f"[streams] rebuild + two concurrent traces, round {rep}")
+            for a in (i1, i2, t1, t2):
+                a.fill_bytes(0xAB)
+            ctx.build_bvh(timed=False)
+            ctx.trace_closest_device(d1.ptr, len(r1), i1.ptr, t1.ptr, 0, s1.handle)
+            ctx.trace_closest_device(d2.ptr, len(r2), i2.ptr, t2.ptr, 0, s2.handle)
+            s1.synchronize()
+            s2.synchronize()
+            assert np.array_equal(i1.to_host(), o1[0]) and np.array_equal(t1.to_host().view(np.uint32),
+                                                                          o1[1].view(np.uint32))
+            assert np.array_equal(i2.to_host(), o2[0]) and np.array_equal(t2.to_host().view(np.uint32),
+                                                                          o2[1].view(np.uint32))
+    finally:
+        for a in (d1, d2, i1, i2, t1, t2):
+            a.free()
+        s1.destroy()
+        s2.destroy()
+    assert ctx.get_status() == 0

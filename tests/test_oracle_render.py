@@ -162,5 +162,5 @@ def test_render_stats_equal_trace_counts():
     rays = np.concatenate([np.broadcast_to(eye, d.shape), d], axis=1).astype(np.float32)
     ids, _, _, c = orc.trace(tris, b, rays, counters=True)
     assert sts["segments"] == w * h
-    assert (sts["visits"], sts["boxes"], sts["tris"]) == tuple(int(v) for v in c)
+    assert (sts["visits"], sts["boxes"], sts["tris"]) == tuple(int(v) for v in c[:3])
     assert sts["hits"] == int((ids != ptamd.INVALID).sum()) > 0

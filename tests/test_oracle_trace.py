@@ -153,8 +153,9 @@ def test_counters_consistent():
     b = orc.build(tris, bmin, bmax)
     rays = ptamd.gen_shell_rays(2000, 9, bmin, bmax, 1.5)
     _, _, _, c = orc.trace(tris, b, rays, counters=True)
-    iters, boxes, tri_tests = (int(x) for x in c)
+    iters, boxes, tri_tests, depth = (int(x) for x in c)
     assert iters >= len(rays)  # the root is always visited
+    assert 1 <= depth <= 63
     assert boxes + tri_tests == 2 * iters
     # threads do not change the totals
     _, _, _, c1 = orc.trace(tris, b, rays, counters=True, threads=1)
