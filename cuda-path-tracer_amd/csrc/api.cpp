@@ -108,6 +108,7 @@ int pt_create(pt_context** out, int device) {
         if (e != hipSuccess || count <= 0) throw pt::Error(PT_E_NO_DEVICE, "no HIP device visible");
         if (device < 0 || device >= count) throw pt::Error(PT_E_NO_DEVICE, "device index out of range");
         PT_HIP(hipSetDevice(device));
+        pt::trace_scratch_setup(device);
         ctx = new pt_context();
         ctx->device = device;
         PT_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
@@ -400,6 +401,21 @@ int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n, ui
         hipStream_t st = pick(ctx, stream);
         order_after_build(ctx, st);
         pt::trace_closest(ctx->scene, trace_queue(ctx, st), d_rays, n, d_id, d_t, d_uv, st);
+    });
+}
+
+int pt_trace_closest_device_noalloc(pt_context* ctx, const float* d_rays, uint32_t n, uint32_t* d_id,
+                                    float* d_t, float* d_uv, void* stream) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        if (!ctx->scene.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        require(n == 0 || d_rays != nullptr, "rays is NULL");
+        require(n <= PT_NOALLOC_MAX_RAYS, "at most 2^31 - 1 rays per call (32-bit hand-out counters)");
+        uint32_t* queue = pt::trace_scratch(ctx->device);
+        if (!queue) throw pt::Error(PT_E_INTERNAL, "the device's scratch block was not set up");
+        hipStream_t st = pick(ctx, stream);
+        order_after_build(ctx, st);
+        pt::trace_closest(ctx->scene, queue, d_rays, n, d_id, d_t, d_uv, st);
     });
 }
 

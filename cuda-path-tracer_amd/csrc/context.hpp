@@ -162,6 +162,13 @@ TraceTuning& trace_tuning();
 // used-up mask); a launch zeroes its block first, so concurrent launches
 // need distinct blocks.
 size_t trace_queue_bytes();
+// The allocation-free entry point's queue block: trace_scratch_setup (with
+// `device` current, once per device, at context creation) resolves the
+// device's copy of the library's static scratch array; trace_scratch returns
+// it (null before set-up).  Calls that use it overlap on one GPU at their
+// peril: they share the one block.
+void trace_scratch_setup(int device);
+uint32_t* trace_scratch(int device);
 // Closest hit over a device ray batch (rays: 6 floats each), rays handed
 // out through the queue block `queue`.
 void trace_closest(const DeviceScene& scene, uint32_t* queue, const float* d_rays, uint32_t n,

@@ -3,9 +3,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "context.hpp"
 #include "traverse.hpp"
+
+// Scratch of the allocation-free trace entry point: one ray queue block
+// (slice counters and used-up mask, 16,640 bytes).  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code this name
+// is only a handle for hipGetSymbolAddress: trace_scratch_setup resolves each
+// device's copy once, and kernels receive that address.
+static __device__ __align__(256) uint32_t scratch[pt::kQueueWords];
 
 namespace pt {
 namespace {
@@ -141,6 +151,29 @@ void launch_trace(const DeviceScene& s, uint32_t* queue, const float* d_rays, ui
 }  // namespace
 
 size_t trace_queue_bytes() { return kQueueWords * sizeof(uint32_t); }
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<uint32_t*> g_scratch;  // per device: address of its `scratch`, or null
+}  // namespace
+
+void trace_scratch_setup(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) throw Error(PT_E_INVALID_ARG, "negative device index");
+    if (g_scratch.size() <= static_cast<size_t>(device)) g_scratch.resize(static_cast<size_t>(device) + 1, nullptr);
+    if (g_scratch[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+        throw Error(PT_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<uint32_t*>(p);
+}
+
+uint32_t* trace_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || static_cast<size_t>(device) >= g_scratch.size()) return nullptr;
+    return g_scratch[device];
+}
 
 TraceTuning& trace_tuning() {
     static TraceTuning t = [] {

@@ -115,6 +115,7 @@ _SIGS = {
     "pt_get_scene_info": [_vp, _vp, _vp, _vp],
     "pt_trace_closest": [_vp, _vp, _u32, _vp, _vp, _vp],
     "pt_trace_closest_device": [_vp, _vp, _u32, _vp, _vp, _vp, _vp],
+    "pt_trace_closest_device_noalloc": [_vp, _vp, _u32, _vp, _vp, _vp, _vp],
     "pt_trace_closest_counted": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp],
     "pt_set_primary_rays": [_vp, _vp, _vp],
     "pt_render_pass": [_vp, _i, _i, _u64],
@@ -510,6 +511,12 @@ class Context:
     def trace_closest_device(self, d_rays, n, d_id, d_t, d_uv=0, stream=0):
         self._c(_lib.pt_trace_closest_device(self._h, d_rays, n, d_id or None, d_t or None,
                                              d_uv or None, stream or None))
+
+    def trace_closest_device_noalloc(self, d_rays, n, d_id, d_t, d_uv=0, stream=0):
+        """trace_closest_device through the library's static scratch block
+        (no allocation; calls overlapping on one GPU race)."""
+        self._c(_lib.pt_trace_closest_device_noalloc(self._h, d_rays, n, d_id or None, d_t or None,
+                                                     d_uv or None, stream or None))
 
     def trace_closest_counted(self, d_rays, n, d_id=0, d_t=0, d_uv=0, stream=0):
         """Node visits, box tests, triangle tests (totals) and the deepest

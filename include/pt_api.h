@@ -214,6 +214,18 @@ int pt_trace_closest(pt_context* ctx, const float* rays, uint32_t n,
                      uint32_t* id, float* t, float* uv);
 int pt_trace_closest_device(pt_context* ctx, const float* d_rays, uint32_t n,
                             uint32_t* d_id, float* d_t, float* d_uv, void* stream);
+/* pt_trace_closest_device without any allocation (no reference
+ * counterpart; for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The ray
+ * queue block lives in a static device array of the library (16,640 bytes
+ * per GPU, held until the process ends) instead of a per-stream buffer
+ * allocated on first use.  n <= PT_NOALLOC_MAX_RAYS; larger batches are
+ * rejected with PT_E_INVALID_ARG.  All calls on one GPU share the block:
+ * calls that overlap on one GPU (different streams, or different contexts
+ * on the same device) race and must be serialised by the caller. */
+#define PT_NOALLOC_MAX_RAYS 0x7fffffffu
+int pt_trace_closest_device_noalloc(pt_context* ctx, const float* d_rays, uint32_t n,
+                                    uint32_t* d_id, float* d_t, float* d_uv, void* stream);
 /* Closest hit in batched scenes: ray i is traced in scene d_scene[i]. */
 int pt_trace_closest_scenes_device(pt_context* ctx, const float* d_rays, const uint32_t* d_scene,
                                    uint32_t n, uint32_t* d_id, float* d_t, float* d_uv,
