@@ -18,7 +18,8 @@ Prints one JSON line.  Options: --gpus N (only 1 is supported), --steps K,
 --warmup W, --rays R, --no-cpu (skip the CPU baseline and the parity
 checks), --no-batch, --no-hair, --no-cache, --cache-small (960x540 with
 16,384 training paths), --profile (short run for rocprofv3: 2 warmup + 3
-steps, no CPU baseline, no hairball, batched or cache legs).
+steps, no CPU baseline, no hairball, batched or cache legs), --profile-cache
+(short eager radiance-cache workload for rocprofv3).
 """
 import argparse
 import hashlib
@@ -250,6 +251,51 @@ def cache_leg(ctx, stream, args, seed_mesh):
     return out
 
 
+def profile_cache():
+    """Short radiance-cache workload for rocprofv3 (--profile-cache): the
+    1920x1080 room scene, 6 cache frames with 65,536 training paths and 4
+    Adam steps each, launched eagerly (render.graphs 0) so the profiler sees
+    each kernel launch, then 5 inferences over 2,073,600 queries and 5 Adam
+    steps over 16,384 samples."""
+    import ptamd
+    from ptamd import hip
+
+    w, h, ntrain = 1920, 1080, 65536
+    rt, ri = ptamd.gen_sphere(660, 2024, room=True)
+    ctx = ptamd.Context(0)
+    stream = hip.Stream()
+    ctx.set_stream(stream.handle)
+    ctx.upload_scene(rt, ri, settings=ptamd.Settings(9, 3, 1.0, 1))
+    bmin, bmax = ptamd.scene_bounds(rt)
+    eye, tgt = (0.0, 0.0, -1.9), (0.0, 0.0, 0.0)
+    ptamd.tune_set("render.graphs", 0)
+    ctx.cache_init(7, bmin, bmax)
+    cs = ptamd.CacheSettings(ntrain, 4, 8, train=True)
+    for f in range(6):
+        ctx.set_primary_rays(ptamd.camera_corners(eye, tgt, (0, 1, 0), 60.0, w / h,
+                                                  *ptamd.pass_jitter(43, f, w, h)), eye)
+        ctx.render_cache_frame(w, h, 3 * 10 ** 8 + f, cs)
+        stream.synchronize()
+        log(f"[profile-cache] cache frame {f}")
+    rng = np.random.default_rng(0)
+    npix = w * h
+    raw = np.zeros((npix, 12), np.float32)
+    raw[:, :3] = rng.uniform(bmin, bmax, (npix, 3))
+    raw[:, 3:9] = rng.normal(size=(npix, 6))
+    raw[:, 9:] = 0.5
+    d_raw = hip.DeviceArray.from_host(raw)
+    d_out = hip.DeviceArray((npix, 3), np.float32)
+    chunk = ntrain // 4
+    d_tgt = hip.DeviceArray.from_host(rng.uniform(0, 1, (chunk, 3)).astype(np.float32))
+    for _ in range(5):
+        ctx.cache_infer_device(d_raw.ptr, npix, d_out.ptr, stream.handle)
+    for _ in range(5):
+        ctx.cache_train_device(d_raw.ptr, d_tgt.ptr, chunk, 0, stream.handle)
+    stream.synchronize()
+    log("[profile-cache] 5 inferences, 5 Adam steps")
+    print(json.dumps({"profile_cache": "done", "frames": 6, "resolution": [w, h], "train_samples": ntrain}))
+
+
 KERNEL_SOURCES = ("cuda-path-tracer_amd/csrc/traverse.hpp", "cuda-path-tracer_amd/csrc/trace.hip",
                   "cuda-path-tracer_amd/csrc/devmath.hpp")
 
@@ -292,6 +338,8 @@ def main():
     ap.add_argument("--rays", type=int, default=16 * 1024 * 1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true")
+    ap.add_argument("--profile-cache", action="store_true",
+                    help="short eager radiance-cache workload for rocprofv3, then exit")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched leg")
@@ -308,6 +356,9 @@ def main():
         print(f"bench.py: only --gpus 1 is supported (this build runs on one GPU; got {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    if args.profile_cache:
+        profile_cache()
+        return
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.no_tune, args.no_batch = 3, 2, True, True, True
         args.no_hair = True
@@ -447,10 +498,11 @@ def main():
         if not args.no_tune:
             from ptamd import autotune
 
-            space = autotune.render_space(autotune.device_cus())
-            saved = {k: ptamd.tune_get(k) for k in space}
-            best, table = autotune.tune(lambda: ctx.render_passes(w, h, seeds[:per_frame], dirs[:per_frame]),
-                                        stream, space, reps=3)
+            stages = autotune.render_stages(autotune.device_cus())
+            saved = {k: ptamd.tune_get(k) for st in stages for k in st}
+            best, table = autotune.tune_staged(lambda: ctx.render_passes(w, h, seeds[:per_frame],
+                                                                         dirs[:per_frame]),
+                                               stream, stages, reps=3)
             rtune = {"chosen": best, "candidates": len(table)}
         ctx.reset_frames()
         stream.synchronize()

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -378,6 +379,41 @@ int pt_get_scene_info(pt_context* ctx, uint32_t* ntris, float* bmin, float* bmax
     });
 }
 
+int pt_verify_bvh(pt_context* ctx, uint32_t* bad_scenes) {
+    if (!ctx) return PT_E_INVALID_ARG;
+    return guarded(ctx, [&] {
+        const pt::DeviceScene& s = ctx->scene;
+        if (!s.ready) throw pt::Error(PT_E_NO_SCENE, "no scene uploaded");
+        const uint32_t ns = s.nscenes();
+        std::vector<float> root(static_cast<size_t>(ns) * 6, 0.0f);
+        if (s.n > 0) {
+            PT_HIP(hipMemcpyAsync(root.data(), s.bvh.root_box.as<void>(), root.size() * sizeof(float),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        }
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        check_status(ctx);
+        uint32_t bad = 0, first = 0;
+        for (uint32_t k = 0; k < ns; ++k) {
+            if (s.off_host[k + 1] == s.off_host[k]) continue;  // an empty scene has no root
+            if (std::memcmp(&root[6 * static_cast<size_t>(k)], &s.box_host[6 * static_cast<size_t>(k)],
+                            6 * sizeof(float)) != 0) {
+                if (bad++ == 0) first = k;
+            }
+        }
+        if (bad_scenes) *bad_scenes = bad;
+        if (bad) {
+            const float* r = &root[6 * static_cast<size_t>(first)];
+            const float* e = &s.box_host[6 * static_cast<size_t>(first)];
+            char msg[320];
+            std::snprintf(msg, sizeof msg,
+                          "%u scene(s) whose root box differs from the enclosing box; first: scene %u, root "
+                          "(%g %g %g)-(%g %g %g), enclosing (%g %g %g)-(%g %g %g)",
+                          bad, first, r[0], r[1], r[2], r[3], r[4], r[5], e[0], e[1], e[2], e[3], e[4], e[5]);
+            throw pt::Error(PT_E_INTERNAL, msg);
+        }
+    });
+}
+
 int pt_trace_closest_scenes_device(pt_context* ctx, const float* d_rays, const uint32_t* d_scene,
                                    uint32_t n, uint32_t* d_id, float* d_t, float* d_uv, void* stream) {
     if (!ctx) return PT_E_INVALID_ARG;
@@ -651,12 +687,20 @@ uint32_t* tune_slot(const char* name) {
     if (k == "trace.refill") return &pt::trace_tuning().refill;
     if (k == "trace.lds_stack") return &pt::trace_tuning().lds;
     if (k == "trace.pool") return &pt::trace_tuning().pool;
+    if (k == "trace.filter") return &pt::trace_tuning().filter;
     if (k == "render.graphs") return &pt::render_tuning().graphs;
+    if (k == "render.chains") return &pt::render_tuning().chains;
+    if (k == "render.paths") return &pt::render_tuning().paths;
+    if (k == "render.shade_min") return &pt::render_tuning().shade_min;
     return nullptr;
 }
 
 bool tune_valid(const uint32_t* slot, uint32_t v) {
     if (slot == &pt::render_tuning().graphs) return true;
+    if (slot == &pt::trace_tuning().filter) return v <= 1;
+    if (slot == &pt::render_tuning().chains) return v >= 1 && v <= 8;
+    if (slot == &pt::render_tuning().paths) return v <= 1;
+    if (slot == &pt::render_tuning().shade_min) return v >= 1 && v <= 64;
     if (slot == &pt::trace_tuning().lds) return v == 8 || v == 16 || v == 32;
     if (slot == &pt::trace_tuning().pool) return v >= 64 && v <= 4096;
     return v != 0;

@@ -155,6 +155,7 @@ struct TraceTuning {
     uint32_t refill = 32;    // refill idle lanes when fewer than this many are busy
     uint32_t lds = 16;       // traversal stack entries kept in LDS: 8, 16 or 32
     uint32_t pool = 256;     // rays a wave takes per hand-out on long queues (64..4096)
+    uint32_t filter = 1;     // closest-hit batches: root filter before the persistent loop
 };
 TraceTuning& trace_tuning();
 
@@ -211,12 +212,19 @@ struct Renderer {
     bool ring_used[kParamRing] = {};
     int ring_next = 0;
     void* graph_exec = nullptr;   // hipGraphExec_t of the last captured frame
-    alignas(8) char graph_key[192] = {};
+    alignas(8) char graph_key[256] = {};
     bool timed = false;   // a pass has been bracketed by the context events
     uint32_t last_levels = 0;  // queue levels of the last frame
+    uint32_t last_chains = 1;  // level chains of the last frame
+    // A plain frame's chains 1.. run on these streams (fork and join events;
+    // created before the first frame with that many chains, default priority).
+    hipStream_t chain_st[7] = {};
+    hipEvent_t chain_fork = nullptr;
+    hipEvent_t chain_join[7] = {};
     DevBuf cache_io;           // cache-frame queries and training batch
-    // A cache frame traces its training paths' last bounces on `side` while
-    // the pixel paths are resolved on the frame's stream (fork and join
+    // A cache frame traces its training paths, from their primary rays to
+    // their ends, on `side` while the pixel paths' two levels, inference,
+    // resolve and accumulation run on the frame's stream (fork and join
     // events); created on the first cache frame, default priority.
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
@@ -237,6 +245,9 @@ struct Renderer {
 
 struct RenderTuning {
     uint32_t graphs = 1;  // replay each frame's kernels as one captured hipGraph
+    uint32_t chains = 4;  // level chains of a plain frame (1..8), one stream each
+    uint32_t paths = 1;   // plain frames path-persistent (one launch, no bounce levels)
+    uint32_t shade_min = 32;  // path-persistent frames: lanes that shade together (1..64)
 };
 RenderTuning& render_tuning();
 
@@ -268,8 +279,9 @@ void render_last_queries(Renderer& r, float* raw, float* pred, uint32_t* valid, 
 // and training scratch (csrc/mlp.hip).
 struct RadianceCache {
     DevBuf master, m, v;   // fp32 parameters and Adam moments
-    DevBuf fwd, bwd;       // fp16 weight fragments (forward / backward)
-    DevBuf acts, deltas, part, loss;  // training scratch
+    DevBuf fwd, bwd;       // fp16 weight fragments (forward / backward, 16x16x16)
+    DevBuf f32;            // fp16 forward fragments for the 16x16x32 inference
+    DevBuf part, loss;     // training scratch: per-workgroup gradient partials, per-sample loss
     DevBuf dstep;          // uint32 Adam step count (device side)
     float box_min[3] = {0, 0, 0}, box_inv[3] = {1, 1, 1};
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;

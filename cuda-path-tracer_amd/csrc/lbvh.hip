@@ -8,9 +8,10 @@
 //
 //   k_morton        30-bit Morton key (the scene's box) + index + scene id
 //   k_sort_hist     \
-//   k_sort_rowscan   } LSD radix sort, 8-bit digits, stable: 4 passes over
-//   k_sort_scatter  /  the key, then 1-2 over the scene id when batched
-//                      (wave64 ballot match + LDS staging per 4096-key tile)
+//   k_sort_rowscan   } LSD radix sort, 10-bit digits, stable: 3 passes over
+//   k_sort_scatter  /  the 30-bit key, then 1-2 over the scene id when
+//                      batched (wave64 ballot match + LDS staging per
+//                      4096-key tile)
 //   k_radix_tree    Karras split per internal node, parents, counter reset,
 //                   list of nodes whose leaf range crosses a refit block
 //   k_refit_local   sorted triangle records + bottom-up AABBs inside each
@@ -33,6 +34,9 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kSortKeysPerThread = 16;
 constexpr int kSortTile = kSortThreads * kSortKeysPerThread;  // 4096 keys
+constexpr int kDigitBits = 10;                                 // 3 passes over a 30-bit key
+constexpr uint32_t kRadix = 1u << kDigitBits;                  // 1024 digits
+constexpr int kDigitsPerThread = kRadix / kSortThreads;        // 4
 // Refit levels.  A node's level is the smallest k whose block of sorted
 // leaves contains the node's whole leaf range:
 //   level 0: inside a 1024-leaf block    -> k_refit_local (LDS, one workgroup per block)
@@ -134,12 +138,12 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_
     return pre + x - v;
 }
 
-// Digit of an element: bits [shift, shift+8) of its key, or -- in the
+// Digit of an element: bits [shift, shift+10) of its key, or -- in the
 // batched build's last passes -- of the scene id of its original slot.
 template <bool BY_SCENE>
 __device__ __forceinline__ uint32_t sort_digit(uint32_t k, uint32_t v,
                                                const uint32_t* __restrict__ scene_of, int shift) {
-    return ((BY_SCENE ? scene_of[v] : k) >> shift) & 255u;
+    return ((BY_SCENE ? scene_of[v] : k) >> shift) & (kRadix - 1u);
 }
 
 // Per-tile digit histogram: tile_hist[digit * num_tiles + tile].
@@ -150,9 +154,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __re
                                                             uint32_t n, int shift,
                                                             uint32_t num_tiles,
                                                             uint32_t* __restrict__ tile_hist) {
-    __shared__ uint32_t h[256];
+    __shared__ uint32_t h[kRadix];
     const int tid = threadIdx.x;
-    h[tid] = 0;
+#pragma unroll
+    for (int j = 0; j < kDigitsPerThread; ++j) h[tid + j * kSortThreads] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
 #pragma unroll 4
@@ -161,7 +166,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __re
         if (i < n) atomicAdd(&h[sort_digit<BY_SCENE>(keys[i], BY_SCENE ? vals[i] : 0u, scene_of, shift)], 1u);
     }
     __syncthreads();
-    tile_hist[static_cast<size_t>(tid) * num_tiles + blockIdx.x] = h[tid];
+#pragma unroll
+    for (int j = 0; j < kDigitsPerThread; ++j) {
+        const int d = tid + j * kSortThreads;
+        tile_hist[static_cast<size_t>(d) * num_tiles + blockIdx.x] = h[d];
+    }
 }
 
 // One block per digit: exclusive scan of that digit's row of tile counts (in
@@ -185,9 +194,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_rowscan(uint32_t* __restr
 
 // Stable scatter of one 4096-key tile.  Each wave ranks 64 keys at a time
 // against the keys of its chunk so far: lanes holding the same digit are
-// found with eight ballots, their order is the popcount of lower peer lanes,
+// found with ten ballots, their order is the popcount of lower peer lanes,
 // and a per-wave LDS counter per digit carries the count between rounds.
 // Keys are then staged in LDS in digit order and written out in runs.
+// Thread t owns digits 4t .. 4t+3 in the per-digit scans.
 template <bool BY_SCENE>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
@@ -195,9 +205,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t* __restrict__ scene_of, uint32_t n, int shift,
     uint32_t num_tiles, const uint32_t* __restrict__ tile_hist,
     const uint32_t* __restrict__ digit_total) {
-    __shared__ uint32_t s_wc[4][256];
-    __shared__ uint32_t s_dstart[256];
-    __shared__ uint32_t s_gbase[256];
+    __shared__ uint32_t s_wc[4][kRadix];
+    __shared__ uint32_t s_dstart[kRadix];
+    __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
     __shared__ uint32_t s_w[4];
@@ -206,7 +216,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const int lane = tid & 63;
     const int wave = tid >> 6;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) s_wc[k][tid] = 0;
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < kDigitsPerThread; ++j) s_wc[k][tid + j * kSortThreads] = 0;
     __syncthreads();
 
     const uint32_t tile = blockIdx.x;
@@ -225,7 +237,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
         const uint32_t d = valid ? sort_digit<BY_SCENE>(k, v, scene_of, shift) : 0u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < kDigitBits; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -243,18 +255,33 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     }
     __syncthreads();
 
-    // Digit `tid`: offsets of each wave's run, tile-local start, global base.
-    const uint32_t c0 = s_wc[0][tid], c1 = s_wc[1][tid], c2 = s_wc[2][tid], c3 = s_wc[3][tid];
-    __syncthreads();
-    s_wc[0][tid] = 0;
-    s_wc[1][tid] = c0;
-    s_wc[2][tid] = c0 + c1;
-    s_wc[3][tid] = c0 + c1 + c2;
+    // Digits 4t..4t+3: offsets of each wave's run, tile-local start, global base.
+    uint32_t tot[kDigitsPerThread], gt[kDigitsPerThread];
+    uint32_t tsum = 0, gsum = 0;
+#pragma unroll
+    for (int j = 0; j < kDigitsPerThread; ++j) {
+        const int d = kDigitsPerThread * tid + j;
+        const uint32_t c0 = s_wc[0][d], c1 = s_wc[1][d], c2 = s_wc[2][d], c3 = s_wc[3][d];
+        tot[j] = c0 + c1 + c2 + c3;
+        gt[j] = digit_total[d];
+        tsum += tot[j];
+        gsum += gt[j];
+        s_wc[0][d] = 0;  // each digit's words are read and rewritten by its own thread only
+        s_wc[1][d] = c0;
+        s_wc[2][d] = c0 + c1;
+        s_wc[3][d] = c0 + c1 + c2;
+    }
     uint32_t tmp;
-    const uint32_t dstart = block_excl_scan_256(c0 + c1 + c2 + c3, s_w, tmp);
-    const uint32_t gpre = block_excl_scan_256(digit_total[tid], s_w, tmp);
-    s_dstart[tid] = dstart;
-    s_gbase[tid] = gpre + tile_hist[static_cast<size_t>(tid) * num_tiles + tile];
+    uint32_t dstart = block_excl_scan_256(tsum, s_w, tmp);
+    uint32_t gpre = block_excl_scan_256(gsum, s_w, tmp);
+#pragma unroll
+    for (int j = 0; j < kDigitsPerThread; ++j) {
+        const int d = kDigitsPerThread * tid + j;
+        s_dstart[d] = dstart;
+        s_gbase[d] = gpre + tile_hist[static_cast<size_t>(d) * num_tiles + tile];
+        dstart += tot[j];
+        gpre += gt[j];
+    }
     __syncthreads();
 
 #pragma unroll
@@ -639,8 +666,8 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     b.tris.reserve(static_cast<size_t>(n) * 48);
     b.root_box.reserve(static_cast<size_t>(nscenes) * 6 * sizeof(float));
     const uint32_t num_tiles = div_up(n, kSortTile);
-    b.tile_hist.reserve(static_cast<size_t>(num_tiles) * 256 * 4);
-    b.digit_total.reserve(256 * 4);
+    b.tile_hist.reserve(static_cast<size_t>(num_tiles) * kRadix * 4);
+    b.digit_total.reserve(kRadix * 4);
     const uint32_t mid_blocks = div_up(n, kMidBlock);
     b.span_list.reserve(static_cast<size_t>(mid_blocks) * kMidCap * 4);
     b.span_top.reserve(n4);
@@ -657,14 +684,15 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     if (n >= 2) {
         uint32_t* kout = b.keys_alt.as<uint32_t>();
         uint32_t* vout = b.vals_alt.as<uint32_t>();
-        // 4 passes over the 30-bit keys, then the scene id's bytes (stable:
-        // each scene keeps its key order, ties in input order).
+        // 3 passes over the 30-bit keys, then the scene id's 10-bit digits
+        // (stable: each scene keeps its key order, ties in input order).
+        constexpr int kKeyPasses = (30 + kDigitBits - 1) / kDigitBits;
         int scene_passes = 0;
-        for (uint32_t v = nscenes - 1; v != 0; v >>= 8) ++scene_passes;
-        const int passes = 4 + scene_passes;
+        for (uint32_t v = nscenes - 1; v != 0; v >>= kDigitBits) ++scene_passes;
+        const int passes = kKeyPasses + scene_passes;
         for (int pass = 0; pass < passes; ++pass) {
-            const bool by_scene = pass >= 4;
-            const int shift = 8 * (by_scene ? pass - 4 : pass);
+            const bool by_scene = pass >= kKeyPasses;
+            const int shift = kDigitBits * (by_scene ? pass - kKeyPasses : pass);
             if (by_scene) {
                 hipLaunchKernelGGL(k_sort_hist<true>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
                                    b.scene_of.as<uint32_t>(), n, shift, num_tiles, b.tile_hist.as<uint32_t>());
@@ -672,7 +700,7 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                 hipLaunchKernelGGL(k_sort_hist<false>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
                                    b.scene_of.as<uint32_t>(), n, shift, num_tiles, b.tile_hist.as<uint32_t>());
             }
-            hipLaunchKernelGGL(k_sort_rowscan, dim3(256), dim3(kSortThreads), 0, st,
+            hipLaunchKernelGGL(k_sort_rowscan, dim3(kRadix), dim3(kSortThreads), 0, st,
                                b.tile_hist.as<uint32_t>(), num_tiles, b.digit_total.as<uint32_t>());
             if (by_scene) {
                 hipLaunchKernelGGL(k_sort_scatter<true>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,

@@ -11,13 +11,19 @@
 // Weights are pre-packed in A-fragment order (8 bytes per lane per 16x16
 // block) and staged in LDS.
 //
-// Training step (deterministic: no floating-point atomics):
-//   k_mlp_fwd_bwd   per 16-sample block: forward, relative-L2 loss, backward
-//                   deltas; stores layer inputs and deltas, feature-major fp16
-//   k_mlp_wgrad     dW = act^T delta and db = sum delta as split-K MFMA GEMMs,
-//                   one partial per sample slice
-//   k_mlp_adam      sums the partials in slice order, Adam on the masters
-//   k_mlp_pack      masters -> fp16 fragments for the next step
+// Inference uses gfx950's doubled-K v_mfma_f32_16x16x32_f16 with its own
+// fragment packing (below, kperm) and four 16-sample blocks per wave.
+//
+// Training step (deterministic: no floating-point atomics), two launches:
+//   k_mlp_train      per workgroup, 64-sample rounds (one 16-sample block per
+//                    wave): forward, relative-L2 loss, backward deltas, then
+//                    the layer inputs and deltas go to LDS feature-major and
+//                    the four waves accumulate dW = act^T delta (MFMA, K =
+//                    samples) and db = sum delta over the workgroup's
+//                    samples; one fp32 partial per workgroup
+//   k_mlp_adam_pack  sums the partials in workgroup order, Adam on the fp32
+//                    masters, and writes each updated weight into its fp16
+//                    fragments (forward, backward and inference layouts)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -31,6 +37,7 @@ namespace pt {
 namespace {
 
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = 96, kHid = 64, kOut = 3, kLayers = 5;
@@ -53,24 +60,66 @@ __host__ __device__ constexpr int bwd_base(int l) { return l <= 1 ? 0 : bwd_base
 constexpr int kBwdFrags = bwd_base(kLayers);  // 52
 constexpr int kFragHalves = 64 * 4;
 
+// Inference fragments for gfx950's v_mfma_f32_16x16x32_f16 (A: lane L holds
+// row L%16, k-elements 8(L/16)+j).  Two 16x16 D fragments of one layer --
+// output rows 32q..32q+15 and 32q+16..32q+31 -- concatenated lane by lane
+// (lane L: rows 4(L/16)+e of each) are the next layer's B operand for k-block
+// q when the k index is permuted: element j of lane group g is feature
+// kperm(q, g, j).  The weights are packed with the same permutation, so
+// activations still never leave the registers.
+__host__ __device__ constexpr int kperm(int q, int g, int j) { return 32 * q + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
+__host__ __device__ constexpr int kb32(int l) { return (dim(l) + 31) / 32; }
+__host__ __device__ constexpr int f32_base(int l) { return l == 0 ? 0 : f32_base(l - 1) + rb(l - 1) * kb32(l - 1); }
+constexpr int kF32Frags = f32_base(kLayers);  // 12 + 3 * 8 + 2 = 38
+static_assert(kF32Frags == 38, "16x16x32 fragment count");
+constexpr int kF32Halves = 64 * 8;
+
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ floatx4 mfma(half4 a, half4 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
+__device__ __forceinline__ floatx4 mfma32(half8 a, half8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// Masters -> 16x16x32 forward fragment (l, r, q), lane L, element j:
+// W_l[kperm(q, L/16, j)][16r + L%16], zero outside the layer's shape.
+__device__ __forceinline__ void pack32_frag(const float* __restrict__ master, _Float16* __restrict__ f32,
+                                            int frag, int lane) {
+    int l = 0;
+    while (l + 1 < kLayers && frag >= f32_base(l + 1)) ++l;
+    const int k = frag - f32_base(l);
+    const int r = k / kb32(l), q = k % kb32(l);
+    const int ni = dim(l), no = dim(l + 1);
+    const float* W = master + w_off(l);
+    const int o = 16 * r + lane % 16;
+    _Float16* out = f32 + static_cast<size_t>(frag) * kF32Halves + lane * 8;
+    for (int j = 0; j < 8; ++j) {
+        const int i = kperm(q, lane / 16, j);
+        out[j] = (i < ni && o < no) ? static_cast<_Float16>(W[i * no + o]) : static_cast<_Float16>(0.0f);
+    }
+}
+
 // Masters -> fp16 fragments.  Forward fragment (l, r, f), lane L, element e:
 // W_l[16f + 4(L/16) + e][16r + L%16]; backward fragment (l, r, f):
-// W_l[16r + L%16][16f + 4(L/16) + e]; zero outside the layer's shape.
-// Also advances the device-side Adam step counter by `advance` (one thread).
+// W_l[16r + L%16][16f + 4(L/16) + e]; zero outside the layer's shape; then
+// the 16x16x32 inference fragments (pack32_frag).  Also advances the
+// device-side Adam step counter by `advance` (one thread).
 __global__ __launch_bounds__(kThreads) void k_mlp_pack(const float* __restrict__ master,
                                                        _Float16* __restrict__ fwd,
                                                        _Float16* __restrict__ bwd,
+                                                       _Float16* __restrict__ f32,
                                                        uint32_t* __restrict__ step, uint32_t advance) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0 && advance) step[0] += advance;
     const int frag = t / 64, lane = t % 64;
-    if (frag >= kFwdFrags + kBwdFrags) return;
+    if (frag >= kFwdFrags + kBwdFrags + kF32Frags) return;
+    if (frag >= kFwdFrags + kBwdFrags) {
+        pack32_frag(master, f32, frag - kFwdFrags - kBwdFrags, lane);
+        return;
+    }
     int l = 0, r = 0, f = 0;
     bool forward = frag < kFwdFrags;
     if (forward) {
@@ -210,42 +259,191 @@ __device__ __forceinline__ void stage_frags(uint2* dst, const uint2* __restrict_
     __syncthreads();
 }
 
-// Inference: out[3s + c] = exp(z_c).
-__global__ __launch_bounds__(kThreads) void k_mlp_infer(const uint2* __restrict__ fwd,
+// Input features kperm(q, g, 0..7) of one sample for the 16x16x32 B operand
+// (lane group g).  Frequencies use the hardware sine and cosine, whose
+// argument is in revolutions: sin/cos(2^k pi p) = sin/cos(2 pi frac(2^(k-1) p)).
+__device__ __forceinline__ half8 encode32(int q, int g, const float* p01, const float* raw) {
+    half8 x;
+#pragma unroll
+    for (int j = 0; j < 8; j += 2) {
+        const int f = kperm(q, g, j);  // even: (sin, cos) of one argument, or inputs past 72
+        if (f < 72) {
+            const int a = f / 24, k = (f % 24) / 2;
+            const float rev = ldexpf(p01[a], k - 1);
+            const float fr = rev - floorf(rev);
+            x[j] = static_cast<_Float16>(__builtin_amdgcn_sinf(fr));
+            x[j + 1] = static_cast<_Float16>(__builtin_amdgcn_cosf(fr));
+        } else {
+            x[j] = static_cast<_Float16>(f < 81 ? raw[3 + f - 72] : 0.0f);
+            x[j + 1] = static_cast<_Float16>(f + 1 < 81 ? raw[3 + f + 1 - 72] : 0.0f);
+        }
+    }
+    return x;
+}
+
+__device__ __forceinline__ half8 lds_frag32(const uint4* __restrict__ frags, int idx) {
+    const uint4 u = frags[idx * 64 + (threadIdx.x & 63)];
+    half8 h;
+    __builtin_memcpy(&h, &u, sizeof h);
+    return h;
+}
+
+// Next layer's B operand for k-block q from the D fragments of row blocks
+// 2q and 2q + 1 (ReLU, fp16).
+__device__ __forceinline__ half8 join_relu(floatx4 a, floatx4 b) {
+    half8 h;
+    h[0] = static_cast<_Float16>(a.x > 0.0f ? a.x : 0.0f);
+    h[1] = static_cast<_Float16>(a.y > 0.0f ? a.y : 0.0f);
+    h[2] = static_cast<_Float16>(a.z > 0.0f ? a.z : 0.0f);
+    h[3] = static_cast<_Float16>(a.w > 0.0f ? a.w : 0.0f);
+    h[4] = static_cast<_Float16>(b.x > 0.0f ? b.x : 0.0f);
+    h[5] = static_cast<_Float16>(b.y > 0.0f ? b.y : 0.0f);
+    h[6] = static_cast<_Float16>(b.z > 0.0f ? b.z : 0.0f);
+    h[7] = static_cast<_Float16>(b.w > 0.0f ? b.w : 0.0f);
+    return h;
+}
+
+// Inference: out[3s + c] = exp(z_c).  A wave runs kInferBlocks 16-sample
+// blocks at a time so each weight fragment read from LDS feeds kInferBlocks
+// MFMAs.
+constexpr int kInferBlocks = 4;
+
+__global__ __launch_bounds__(kThreads) void k_mlp_infer(const uint4* __restrict__ f32,
                                                         const float* __restrict__ master,
                                                         const float* __restrict__ raw, uint32_t n,
                                                         SceneBox box, float* __restrict__ out) {
-    __shared__ uint2 s_frag[kFwdFrags * 64];
-    stage_frags(s_frag, fwd, kFwdFrags);
-    const int lane = threadIdx.x & 63;
+    __shared__ uint4 s_frag[kF32Frags * 64];
+    __shared__ float s_bias[kLayers][kHid];
+    for (int k = threadIdx.x; k < kF32Frags * 64; k += blockDim.x) s_frag[k] = f32[k];
+    for (int k = threadIdx.x; k < kLayers * kHid; k += blockDim.x) {
+        const int l = k / kHid, o = k % kHid;
+        s_bias[l][o] = o < dim(l + 1) ? master[b_off(l) + o] : 0.0f;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, g = lane / 16;
     const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
-    for (uint32_t blk = wave; blk * 16 < n; blk += nwaves) {
-        const uint32_t s = blk * 16 + lane % 16;
-        half4 X[fb(0)];
-        half4 H[5][4];
-        encode_block(raw, n, s, box, X);
-        const floatx4 z = forward_block(s_frag, master, X, H);
-        if (lane < 16 && s < n) {
-            out[3 * static_cast<size_t>(s) + 0] = expf(z.x);
-            out[3 * static_cast<size_t>(s) + 1] = expf(z.y);
-            out[3 * static_cast<size_t>(s) + 2] = expf(z.z);
+    auto bias = [&](int l, int r) {
+        const int o = 16 * r + 4 * g;
+        floatx4 v;
+        v.x = s_bias[l][o + 0];
+        v.y = s_bias[l][o + 1];
+        v.z = s_bias[l][o + 2];
+        v.w = s_bias[l][o + 3];
+        return v;
+    };
+    constexpr uint32_t kSpan = 16 * kInferBlocks;
+    const bool aligned = (reinterpret_cast<uintptr_t>(raw) & 15u) == 0u;  // 48-byte samples, 16-byte loads
+    for (uint32_t base = wave * kSpan; base < n; base += nwaves * kSpan) {
+        // Weight fragments are read from LDS where used: kept from one
+        // iteration to the next they would take 152 VGPRs (one wave per SIMD).
+        asm volatile("" ::: "memory");
+        half8 X[kInferBlocks][3];
+#pragma unroll
+        for (int b = 0; b < kInferBlocks; ++b) {
+            const uint32_t s = base + 16 * b + lane % 16;
+            float rw[12];
+            if (s < n && aligned) {
+                const float4* rp = reinterpret_cast<const float4*>(raw + 12 * static_cast<size_t>(s));
+                const float4 r0 = rp[0], r1 = rp[1], r2 = rp[2];
+                rw[0] = r0.x; rw[1] = r0.y; rw[2] = r0.z; rw[3] = r0.w;
+                rw[4] = r1.x; rw[5] = r1.y; rw[6] = r1.z; rw[7] = r1.w;
+                rw[8] = r2.x; rw[9] = r2.y; rw[10] = r2.z; rw[11] = r2.w;
+            } else if (s < n) {
+                for (int k = 0; k < 12; ++k) rw[k] = raw[12 * static_cast<size_t>(s) + k];
+            } else {
+                for (int k = 0; k < 12; ++k) rw[k] = 0.0f;
+            }
+            float p01[3];
+            for (int a = 0; a < 3; ++a) p01[a] = (rw[a] - box.mn[a]) * box.inv_ext[a];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) X[b][q] = encode32(q, g, p01, rw);
+        }
+        // Layer 0: 96 -> 64.
+        half8 H[kInferBlocks][2];
+#pragma unroll
+        for (int rp = 0; rp < 2; ++rp) {
+            floatx4 acc[kInferBlocks][2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const floatx4 bv = bias(0, 2 * rp + h);
+#pragma unroll
+                for (int b = 0; b < kInferBlocks; ++b) acc[b][h] = bv;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const half8 a = lds_frag32(s_frag, f32_base(0) + (2 * rp + h) * 3 + q);
+#pragma unroll
+                    for (int b = 0; b < kInferBlocks; ++b) acc[b][h] = mfma32(a, X[b][q], acc[b][h]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kInferBlocks; ++b) H[b][rp] = join_relu(acc[b][0], acc[b][1]);
+        }
+        // Layers 1..3: 64 -> 64.
+#pragma unroll
+        for (int l = 1; l < 4; ++l) {
+            half8 Hn[kInferBlocks][2];
+#pragma unroll
+            for (int rp = 0; rp < 2; ++rp) {
+                floatx4 acc[kInferBlocks][2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const floatx4 bv = bias(l, 2 * rp + h);
+#pragma unroll
+                    for (int b = 0; b < kInferBlocks; ++b) acc[b][h] = bv;
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const half8 a = lds_frag32(s_frag, f32_base(l) + (2 * rp + h) * 2 + q);
+#pragma unroll
+                        for (int b = 0; b < kInferBlocks; ++b) acc[b][h] = mfma32(a, H[b][q], acc[b][h]);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < kInferBlocks; ++b) Hn[b][rp] = join_relu(acc[b][0], acc[b][1]);
+            }
+#pragma unroll
+            for (int b = 0; b < kInferBlocks; ++b) {
+                H[b][0] = Hn[b][0];
+                H[b][1] = Hn[b][1];
+            }
+        }
+        // Layer 4: 64 -> 3 (rows 0..2 of the first row block), exp.
+        floatx4 z[kInferBlocks];
+        const floatx4 b4 = bias(4, 0);
+#pragma unroll
+        for (int b = 0; b < kInferBlocks; ++b) z[b] = b4;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const half8 a = lds_frag32(s_frag, f32_base(4) + q);
+#pragma unroll
+            for (int b = 0; b < kInferBlocks; ++b) z[b] = mfma32(a, H[b][q], z[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < kInferBlocks; ++b) {
+            const uint32_t s = base + 16 * b + lane % 16;
+            if (lane < 16 && s < n) {
+                out[3 * static_cast<size_t>(s) + 0] = expf(z[b].x);
+                out[3 * static_cast<size_t>(s) + 1] = expf(z[b].y);
+                out[3 * static_cast<size_t>(s) + 2] = expf(z[b].z);
+            }
         }
     }
 }
 
-// Store a 16x16 fragment (lane: rows 4(l/16)+e, column = sample l%16) into a
-// feature-major fp16 array [feature][ld] at feature row0, sample s0.
-__device__ __forceinline__ void store_fm(_Float16* __restrict__ a, size_t ld, int row0, uint32_t s,
-                                         uint32_t n, half4 v, int nrows) {
-    const int lane = threadIdx.x & 63;
-    if (s >= n) return;
-    const int r = row0 + 4 * (lane / 16);
-    if (r + 0 < nrows) a[static_cast<size_t>(r + 0) * ld + s] = v.x;
-    if (r + 1 < nrows) a[static_cast<size_t>(r + 1) * ld + s] = v.y;
-    if (r + 2 < nrows) a[static_cast<size_t>(r + 2) * ld + s] = v.z;
-    if (r + 3 < nrows) a[static_cast<size_t>(r + 3) * ld + s] = v.w;
-}
+// Training rounds: 64 samples per workgroup round, one 16-sample block per
+// wave.  Layer inputs act_l (act0: 96 rows, act1..4: 64) and deltas
+// delta_l = dL/dz_l scaled by n (delta0..3: 64 rows, delta4: 16 rows with
+// rows 3..15 zero) go to LDS feature-major, [row][sample] with a padded row.
+constexpr int kTrainChunk = 64;
+constexpr int kActRows = kIn + 4 * kHid;   // 352
+constexpr int kDelRows = 4 * kHid + 16;    // 272
+constexpr int kRowLd = kTrainChunk + 4;    // 68 halves: rows 34 banks apart
+__host__ __device__ constexpr int act_row(int l) { return l == 0 ? 0 : kIn + kHid * (l - 1); }
+__host__ __device__ constexpr int del_row(int l) { return kHid * l; }
+constexpr int kWTiles = fwd_base(kLayers);  // 76 weight tiles (16 x 16), 19 per wave
+constexpr int kTilesPerWave = kWTiles / 4;
+static_assert(kWTiles % 4 == 0, "weight tiles split evenly over four waves");
+constexpr uint32_t kMaxTrainBlocks = 128;  // workgroups (gradient partials) of one step
 
 // Deltas are gradients of n * loss (a static loss scale of n keeps them well
 // inside fp16's normal range); they are clamped to fp16's finite range so an
@@ -264,38 +462,60 @@ __device__ __forceinline__ half4 mask4(floatx4 d, half4 h) {
     return to_half4(m);
 }
 
-// Forward + loss + backward per 16-sample block.  Stores, feature-major with
-// leading dimension ld (>= n, multiple of 16): act_l = input of layer l
-// (act0: 96 rows, act1..4: 64) and delta_l = dL/dz_l scaled by n
-// (delta0..3: 64 rows, delta4: 16 rows with rows 3..15 zero).  loss[s] is the
-// sample's sum over channels.  A sample whose first target channel is NaN is
-// masked: zero loss, zero deltas (it still counts in the mean's n).
-__global__ __launch_bounds__(kThreads) void k_mlp_fwd_bwd(
+// A 16x16 D-layout fragment (lane: rows 4(l/16)+e, column = sample l%16) into
+// LDS rows row0 .. row0+15, columns col0 .. col0+15.
+__device__ __forceinline__ void lds_put(_Float16 (*rows)[kRowLd], int row0, int col0, half4 v) {
+    const int lane = threadIdx.x & 63;
+    const int r = row0 + 4 * (lane / 16), c = col0 + lane % 16;
+    rows[r + 0][c] = v.x;
+    rows[r + 1][c] = v.y;
+    rows[r + 2][c] = v.z;
+    rows[r + 3][c] = v.w;
+}
+
+// MFMA operand from LDS: lane l reads row row0 + l%16, samples k0 + 4(l/16) .. +3.
+__device__ __forceinline__ half4 lds_get(const _Float16 (*rows)[kRowLd], int row0, int k0) {
+    const int lane = threadIdx.x & 63;
+    half4 h;
+    __builtin_memcpy(&h, &rows[row0 + lane % 16][k0 + 4 * (lane / 16)], sizeof h);
+    return h;
+}
+
+// One training step's gradient partials.  Workgroup b handles rounds b,
+// b + gridDim.x, ... and writes part[b][param] (weights and biases, fp32).
+// loss[s] = the sample's relative-L2 loss summed over channels.  A sample
+// whose first target channel is NaN is masked: zero loss, zero deltas (it
+// still counts in the mean's n).  Thread 0 of workgroup 0 advances the Adam
+// step counter by `advance` (read by k_mlp_adam_pack after this launch).
+__global__ __launch_bounds__(kThreads) void k_mlp_train(
     const uint2* __restrict__ fwd, const uint2* __restrict__ bwd, const float* __restrict__ master,
-    const float* __restrict__ raw, const float* __restrict__ target, uint32_t n, size_t ld,
-    SceneBox box, _Float16* __restrict__ acts, _Float16* __restrict__ deltas,
-    float* __restrict__ loss) {
+    const float* __restrict__ raw, const float* __restrict__ target, uint32_t n, SceneBox box,
+    float* __restrict__ part, float* __restrict__ loss, uint32_t* __restrict__ step, uint32_t advance) {
     __shared__ uint2 s_fwd[kFwdFrags * 64];
     __shared__ uint2 s_bwd[kBwdFrags * 64];
+    __shared__ _Float16 s_act[kActRows][kRowLd];
+    __shared__ _Float16 s_del[kDelRows][kRowLd];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && advance) step[0] += advance;
     for (int k = threadIdx.x; k < kBwdFrags * 64; k += blockDim.x) s_bwd[k] = bwd[k];
     stage_frags(s_fwd, fwd, kFwdFrags);
     const int lane = threadIdx.x & 63;
-    const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
-    _Float16* act[5];
-    act[0] = acts;
-    for (int l = 1; l < 5; ++l) act[l] = acts + (kIn + 64 * (l - 1)) * ld;
-    _Float16* dl[5];
-    for (int l = 0; l < 5; ++l) dl[l] = deltas + 64 * l * ld;
-    for (uint32_t blk = wave; blk * 16 < n; blk += nwaves) {
-        const uint32_t s = blk * 16 + lane % 16;
+    const int wave = threadIdx.x / 64;
+    floatx4 acc[kTilesPerWave];
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    // Bias sums: thread t owns delta row t (layers 0..3) and, for t < 3,
+    // output bias t (delta4 row t).
+    float bsum = 0.0f, bsum4 = 0.0f;
+    for (uint32_t round = blockIdx.x; round * kTrainChunk < n; round += gridDim.x) {
+        const uint32_t s = round * kTrainChunk + 16 * wave + lane % 16;
+        const int col0 = 16 * wave;
         half4 X[fb(0)];
         half4 H[5][4];
         encode_block(raw, n, s, box, X);
         const floatx4 z = forward_block(s_fwd, master, X, H);
-        for (int f = 0; f < fb(0); ++f) store_fm(act[0], ld, 16 * f, s, n, X[f], kIn);
+        for (int f = 0; f < fb(0); ++f) lds_put(s_act, act_row(0) + 16 * f, col0, X[f]);
         for (int l = 1; l < 5; ++l)
-            for (int r = 0; r < 4; ++r) store_fm(act[l], ld, 16 * r, s, n, H[l][r], kHid);
+            for (int r = 0; r < 4; ++r) lds_put(s_act, act_row(l) + 16 * r, col0, H[l][r]);
         // Output delta: d/dz of (e^z - t)^2 / (sg(e^z)^2 + eps), per sample.
         floatx4 d4 = {0.0f, 0.0f, 0.0f, 0.0f};
         if (lane < 16 && s < n && target[3 * static_cast<size_t>(s)] == target[3 * static_cast<size_t>(s)]) {
@@ -313,142 +533,134 @@ __global__ __launch_bounds__(kThreads) void k_mlp_fwd_bwd(
             d4.x = dd[0];
             d4.y = dd[1];
             d4.z = dd[2];
-            loss[s] = lsum;
-        } else if (lane < 16 && s < n) {
+            if (loss) loss[s] = lsum;
+        } else if (lane < 16 && s < n && loss) {
             loss[s] = 0.0f;
         }
-        half4 D[4];
-        D[0] = to_half4(d4);
-        store_fm(dl[4], ld, 0, s, n, D[0], 16);
+        const half4 D4 = to_half4(d4);
+        lds_put(s_del, del_row(4), col0, D4);
         // delta_3 = W4 delta_4 (masked by ReLU of layer 3's output = H[4]).
         half4 Dn[4];
         for (int r = 0; r < 4; ++r) {
-            floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma(lds_frag(s_bwd, bwd_base(4) + r), D[0], acc);
-            Dn[r] = mask4(acc, H[4][r]);
-            store_fm(dl[3], ld, 16 * r, s, n, Dn[r], kHid);
+            floatx4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+            a = mfma(lds_frag(s_bwd, bwd_base(4) + r), D4, a);
+            Dn[r] = mask4(a, H[4][r]);
+            lds_put(s_del, del_row(3) + 16 * r, col0, Dn[r]);
         }
         for (int l = 3; l >= 1; --l) {
             half4 Dp[4];
             for (int r = 0; r < 4; ++r) {
-                floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                floatx4 a = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-                for (int f = 0; f < 4; ++f) acc = mfma(lds_frag(s_bwd, bwd_base(l) + r * 4 + f), Dn[f], acc);
-                Dp[r] = mask4(acc, H[l][r]);
-                store_fm(dl[l - 1], ld, 16 * r, s, n, Dp[r], kHid);
+                for (int f = 0; f < 4; ++f) a = mfma(lds_frag(s_bwd, bwd_base(l) + r * 4 + f), Dn[f], a);
+                Dp[r] = mask4(a, H[l][r]);
+                lds_put(s_del, del_row(l - 1) + 16 * r, col0, Dp[r]);
             }
             for (int r = 0; r < 4; ++r) Dn[r] = Dp[r];
         }
+        __syncthreads();
+        // dW tiles of this wave over the round's 64 samples (K = 4 x 16).
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+#pragma unroll
+            for (int t = 0; t < kTilesPerWave; ++t) {
+                const int task = wave + 4 * t;
+                int l = 0;
+                while (l + 1 < kLayers && task >= fwd_base(l + 1)) ++l;
+                const int k = task - fwd_base(l);
+                const int ri = k % fb(l), co = k / fb(l);
+                const half4 a = lds_get(s_act, act_row(l) + 16 * ri, 16 * kb);
+                const half4 b = lds_get(s_del, (l == 4 ? kHid * 4 : del_row(l)) + 16 * co, 16 * kb);
+                acc[t] = mfma(a, b, acc[t]);
+            }
+        }
+        {
+            const int t = threadIdx.x;
+            float a = 0.0f, a4 = 0.0f;
+            for (int j = 0; j < kTrainChunk; ++j) {
+                a += static_cast<float>(s_del[t][j]);
+                if (t < kOut) a4 += static_cast<float>(s_del[kHid * 4 + t][j]);
+            }
+            bsum += a;
+            bsum4 += a4;
+        }
+        __syncthreads();  // the next round rewrites the LDS rows
     }
-}
-
-// Split-K weight and bias gradients.  Task (layer l, row block r, column
-// block c, slice) computes dW_l[16r..][16c..] over samples
-// [slice * slice_len, ...) with one MFMA per 16 samples; bias tasks sum the
-// deltas.  Each parameter is written once per slice into part[slice][param].
-struct GradTask {
-    int layer, r, c;  // c = -1: bias block r of the layer
-};
-
-__global__ __launch_bounds__(kThreads) void k_mlp_wgrad(const _Float16* __restrict__ acts,
-                                                        const _Float16* __restrict__ deltas,
-                                                        uint32_t n, size_t ld, uint32_t slice_len,
-                                                        uint32_t nslices, float* __restrict__ part) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    // Task list: weight blocks of every layer, then bias blocks.
-    constexpr int kWTasks = fwd_base(kLayers);  // fb(l) * rb(l) per layer
-    constexpr int kBTasks = 4 * 4 + 1;
-    const uint32_t ntasks = (kWTasks + kBTasks) * nslices;
-    if (wave >= ntasks) return;
-    const uint32_t slice = wave % nslices;
-    int task = static_cast<int>(wave / nslices);
-    const uint32_t s_begin = slice * slice_len;
-    const uint32_t s_end = min(n, s_begin + slice_len);
-    float* out = part + static_cast<size_t>(slice) * kParams;
-    const _Float16* act[5];
-    act[0] = acts;
-    for (int l = 1; l < 5; ++l) act[l] = acts + (kIn + 64 * (l - 1)) * ld;
-    const _Float16* dl[5];
-    for (int l = 0; l < 5; ++l) dl[l] = deltas + 64 * l * ld;
-    if (task < kWTasks) {
+    float* out = part + static_cast<size_t>(blockIdx.x) * kParams;
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) {
+        const int task = wave + 4 * t;
         int l = 0;
         while (l + 1 < kLayers && task >= fwd_base(l + 1)) ++l;
         const int k = task - fwd_base(l);
-        const int r = k % fb(l);  // rows: input features
-        const int c = k / fb(l);  // columns: output features
+        const int ri = k % fb(l), co = k / fb(l);
         const int ni = dim(l), no = dim(l + 1);
-        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        const _Float16* arow = act[l] + static_cast<size_t>(16 * r + lane % 16) * ld;
-        const _Float16* drow = dl[l] + static_cast<size_t>(16 * c + lane % 16) * ld;
-        const bool arow_ok = 16 * r + lane % 16 < ni;
-        const bool drow_ok = 16 * c + lane % 16 < (l == 4 ? 16 : no);
-        for (uint32_t s0 = s_begin; s0 < s_end; s0 += 16) {
-            const uint32_t s = s0 + 4 * (lane / 16);
-            half4 a = {0, 0, 0, 0}, b = {0, 0, 0, 0};
-            if (arow_ok) {
-                for (int e = 0; e < 4; ++e)
-                    if (s + e < s_end) a[e] = arow[s + e];
-            }
-            if (drow_ok) {
-                for (int e = 0; e < 4; ++e)
-                    if (s + e < s_end) b[e] = drow[s + e];
-            }
-            acc = mfma(a, b, acc);
-        }
-        // D lane: column (output) lane % 16, rows (inputs) 4(lane/16) + e.
-        const int o = 16 * c + lane % 16;
+        // D lane: column (output) 16co + lane%16, rows (inputs) 16ri + 4(lane/16) + e.
+        const int o = 16 * co + lane % 16;
         for (int e = 0; e < 4; ++e) {
-            const int i = 16 * r + 4 * (lane / 16) + e;
-            if (i < ni && o < no) out[w_off(l) + i * no + o] = acc[e];
+            const int i = 16 * ri + 4 * (lane / 16) + e;
+            if (i < ni && o < no) out[w_off(l) + i * no + o] = acc[t][e];
         }
-        return;
     }
-    task -= kWTasks;
-    const int l = task / 4, r = task % 4;
-    const int no = dim(l + 1);
-    const int o = 16 * r + lane % 16;
-    float sum = 0.0f;
-    if (o < no) {
-        const _Float16* drow = dl[l] + static_cast<size_t>(o) * ld;
-        for (uint32_t s = s_begin + lane / 16; s < s_end; s += 4) sum += static_cast<float>(drow[s]);
+    {
+        const int t = threadIdx.x;
+        out[b_off(t / kHid) + t % kHid] = bsum;
+        if (t < kOut) out[b_off(4) + t] = bsum4;
     }
-    sum += __shfl_xor(sum, 16, 64);
-    sum += __shfl_xor(sum, 32, 64);
-    if (lane < 16 && o < no) out[b_off(l) + o] = sum;
 }
 
-// Adam (Kingma & Ba 2015) on the fp32 masters; the gradient is the sum of the
-// slice partials in slice order divided by n (the loss is a mean over n).
-// The bias corrections use step t = step[0] + 1, read on the device so a
-// captured frame replays with the right t (k_mlp_pack advances it).
-__global__ __launch_bounds__(kThreads) void k_mlp_adam(float* __restrict__ master, float* __restrict__ m,
-                                                       float* __restrict__ v, const float* __restrict__ part,
-                                                       uint32_t nslices, float inv_n, float lr, float b1,
-                                                       float b2, float eps, const uint32_t* __restrict__ step,
-                                                       float* __restrict__ grad_out) {
+// Adam (Kingma & Ba 2015) on the fp32 masters, one thread per parameter; the
+// gradient is the sum of the workgroup partials in workgroup order divided
+// by n (the loss is a mean over n).  The bias corrections use step t =
+// step[0] (k_mlp_train advanced it), read on the device so a captured frame
+// replays with the right t.  The updated weight is written into its fp16
+// fragments: forward (16x16x16, layer input as k), backward (16x16x16, layer
+// output as k) and inference (16x16x32, permuted k).  grad_out: only the
+// gradient, no update.
+__global__ __launch_bounds__(kThreads) void k_mlp_adam_pack(float* __restrict__ master, float* __restrict__ m,
+                                                            float* __restrict__ v, const float* __restrict__ part,
+                                                            uint32_t nparts, float inv_n, float lr, float b1,
+                                                            float b2, float eps, const uint32_t* __restrict__ step,
+                                                            float* __restrict__ grad_out, _Float16* __restrict__ fwd,
+                                                            _Float16* __restrict__ bwd, _Float16* __restrict__ f32) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kParams) return;
     float g = 0.0f;
-    for (uint32_t s = 0; s < nslices; ++s) g += part[static_cast<size_t>(s) * kParams + p];
+    for (uint32_t s = 0; s < nparts; ++s) g += part[static_cast<size_t>(s) * kParams + p];
     g *= inv_n;
     if (grad_out) {
         grad_out[p] = g;
         return;
     }
-    const float t = static_cast<float>(step[0] + 1u);
+    const float t = static_cast<float>(step[0]);
     const float c1 = 1.0f - powf(b1, t);
     const float c2 = 1.0f - powf(b2, t);
     const float mk = b1 * m[p] + (1.0f - b1) * g;
     const float vk = b2 * v[p] + (1.0f - b2) * g * g;
     m[p] = mk;
     v[p] = vk;
-    master[p] = master[p] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    const float w = master[p] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    master[p] = w;
+    int l = 0;
+    while (l + 1 < kLayers && p >= w_off(l + 1)) ++l;
+    if (p >= b_off(l)) return;  // a bias: no fragment
+    const int no = dim(l + 1);
+    const int i = (p - w_off(l)) / no, o = (p - w_off(l)) % no;
+    const _Float16 h = static_cast<_Float16>(w);
+    // Forward fragment (l, o/16, i/16): lane 16((i%16)/4) + o%16, element i%4.
+    fwd[(fwd_base(l) + (o / 16) * fb(l) + i / 16) * kFragHalves + (16 * ((i % 16) / 4) + o % 16) * 4 + i % 4] = h;
+    // Backward fragment (l, i/16, o/16), l >= 1: lane 16((o%16)/4) + i%16, element o%4.
+    if (l >= 1)
+        bwd[(bwd_base(l) + (i / 16) * rb(l) + o / 16) * kFragHalves + (16 * ((o % 16) / 4) + i % 16) * 4 + o % 4] = h;
+    // Inference fragment (l, o/16, i/32): k index i - 32q = kperm's position.
+    const int q = i / 32, ip = i - 32 * q;
+    const int g4 = ip < 16 ? ip / 4 : (ip - 16) / 4;
+    const int j = ip < 16 ? ip % 4 : 4 + (ip - 16) % 4;
+    f32[(f32_base(l) + (o / 16) * kb32(l) + q) * kF32Halves + (16 * g4 + o % 16) * 8 + j] = h;
 }
 
 uint32_t blocks_for(uint64_t work, uint32_t per) { return static_cast<uint32_t>((work + per - 1) / per); }
 
-constexpr uint32_t kSliceLen = 1024;  // samples per weight-gradient partial
 
 }  // namespace
 
@@ -486,6 +698,7 @@ void RadianceCache::set_params(const float* host, hipStream_t st) {
     m.reserve(kParams * sizeof(float));
     v.reserve(kParams * sizeof(float));
     fwd.reserve(static_cast<size_t>(kFwdFrags) * kFragHalves * 2);
+    f32.reserve(static_cast<size_t>(kF32Frags) * kF32Halves * 2);
     bwd.reserve(static_cast<size_t>(kBwdFrags) * kFragHalves * 2);
     dstep.reserve(sizeof(uint32_t));
     PT_HIP(hipMemcpyAsync(master.as<void>(), host, kParams * sizeof(float), hipMemcpyHostToDevice, st));
@@ -504,9 +717,9 @@ void RadianceCache::get_params(float* host, hipStream_t st) {
 }
 
 void RadianceCache::pack(hipStream_t st, uint32_t advance) {
-    hipLaunchKernelGGL(k_mlp_pack, dim3(blocks_for((kFwdFrags + kBwdFrags) * 64, kThreads)), dim3(kThreads), 0,
-                       st, master.as<float>(), fwd.as<_Float16>(), bwd.as<_Float16>(), dstep.as<uint32_t>(),
-                       advance);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(blocks_for((kFwdFrags + kBwdFrags + kF32Frags) * 64, kThreads)),
+                       dim3(kThreads), 0, st, master.as<float>(), fwd.as<_Float16>(), bwd.as<_Float16>(),
+                       f32.as<_Float16>(), dstep.as<uint32_t>(), advance);
     PT_HIP(hipGetLastError());
 }
 
@@ -518,40 +731,37 @@ void RadianceCache::infer(const float* d_raw, uint32_t n, float* d_out, hipStrea
         b.mn[k] = box_min[k];
         b.inv_ext[k] = box_inv[k];
     }
-    const uint32_t blocks = std::min<uint32_t>(blocks_for(n, 64), 4096);
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(kThreads), 0, st, fwd.as<uint2>(), master.as<float>(),
+    const uint32_t blocks = std::min<uint32_t>(blocks_for(n, 4 * 16 * kInferBlocks), 2048);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(kThreads), 0, st, f32.as<uint4>(), master.as<float>(),
                        d_raw, n, b, d_out);
     PT_HIP(hipGetLastError());
 }
 
+// Workgroups (gradient partials) of a training step over n samples.
+uint32_t train_blocks(uint32_t n) {
+    return std::max<uint32_t>(1u, std::min<uint32_t>(kMaxTrainBlocks, blocks_for(n, kTrainChunk)));
+}
+
 void RadianceCache::prepare_train(uint32_t n) {
-    const size_t ld = (static_cast<size_t>(n) + 15) & ~size_t(15);
-    acts.reserve(ld * (kIn + 4 * kHid) * 2);
-    deltas.reserve(ld * (5 * kHid) * 2);
-    part.reserve(static_cast<size_t>(blocks_for(n, kSliceLen)) * kParams * sizeof(float));
-    loss.reserve(ld * sizeof(float));
+    part.reserve(static_cast<size_t>(train_blocks(n)) * kParams * sizeof(float));
+    loss.reserve((static_cast<size_t>(n) + 16) * sizeof(float));
 }
 
 void RadianceCache::launch_train(const float* d_raw, const float* d_target, uint32_t n, float* d_loss,
                                  float* d_grad_out, hipStream_t st) {
-    const size_t ld = (static_cast<size_t>(n) + 15) & ~size_t(15);
-    const uint32_t nslices = blocks_for(n, kSliceLen);
+    const uint32_t nb = train_blocks(n);
     SceneBox b;
     for (int k = 0; k < 3; ++k) {
         b.mn[k] = box_min[k];
         b.inv_ext[k] = box_inv[k];
     }
-    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(std::min<uint32_t>(blocks_for(n, 64), 2048)), dim3(kThreads), 0, st,
-                       fwd.as<uint2>(), bwd.as<uint2>(), master.as<float>(), d_raw, d_target, n, ld, b,
-                       acts.as<_Float16>(), deltas.as<_Float16>(), d_loss ? d_loss : loss.as<float>());
-    constexpr int kTasks = fwd_base(kLayers) + 4 * 4 + 1;
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(blocks_for(static_cast<uint64_t>(kTasks) * nslices, kThreads / 64)),
-                       dim3(kThreads), 0, st, acts.as<_Float16>(), deltas.as<_Float16>(), n, ld, kSliceLen,
-                       nslices, part.as<float>());
-    hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(kParams, kThreads)), dim3(kThreads), 0, st, master.as<float>(),
-                       m.as<float>(), v.as<float>(), part.as<float>(), nslices, 1.0f / static_cast<float>(n), lr,
-                       beta1, beta2, eps, dstep.as<uint32_t>(), d_grad_out);
-    if (!d_grad_out) pack(st, 1);
+    hipLaunchKernelGGL(k_mlp_train, dim3(nb), dim3(kThreads), 0, st, fwd.as<uint2>(), bwd.as<uint2>(),
+                       master.as<float>(), d_raw, d_target, n, b, part.as<float>(),
+                       d_loss ? d_loss : loss.as<float>(), dstep.as<uint32_t>(), d_grad_out ? 0u : 1u);
+    hipLaunchKernelGGL(k_mlp_adam_pack, dim3(blocks_for(kParams, kThreads)), dim3(kThreads), 0, st,
+                       master.as<float>(), m.as<float>(), v.as<float>(), part.as<float>(), nb,
+                       1.0f / static_cast<float>(n), lr, beta1, beta2, eps, dstep.as<uint32_t>(), d_grad_out,
+                       fwd.as<_Float16>(), bwd.as<_Float16>(), f32.as<_Float16>());
     PT_HIP(hipGetLastError());
 }
 

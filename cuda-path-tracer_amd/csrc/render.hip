@@ -91,6 +91,22 @@ constexpr int kShadeThreads = 1024;
 // levels 0 and 1 count pixel paths only, and run at the same time.
 constexpr uint32_t kTrainLevel0Slot = 62, kTrainLevel1Slot = 63;
 constexpr uint32_t kMaxLevels = 64;  // ray queue blocks: at most 61 path levels (60 bounces)
+// A plain frame's paths may be split into up to kMaxChains chains: contiguous
+// ranges of the first level's slots, each traced level by level on a stream
+// of its own, with its own queue lengths (words 0..63 of the chain's counter
+// block) and ray queue blocks (the kMaxLevels blocks after them).  Every level
+// ends with its longest rays on a few lanes; with several chains one chain's
+// tail overlaps the other chains' work.  A chain keeps its paths in its own
+// slot range at every level (paths only drop out), so the queues are shared.
+constexpr uint32_t kMaxChains = 8;
+static_assert(sizeof(Renderer::chain_st) / sizeof(hipStream_t) == kMaxChains - 1, "chain streams");
+constexpr size_t kChainWords = 64 + static_cast<size_t>(kMaxLevels) * kQueueWords;
+constexpr uint32_t kMinChainPaths = 16384;  // smaller chains are merged
+
+// First level-0 slot of chain c of n (chain c holds [chain_begin(c), chain_begin(c + 1))).
+__host__ __device__ __forceinline__ uint32_t chain_begin(uint32_t npaths, uint32_t c, uint32_t n) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(npaths) * c) / n);
+}
 
 // Queue slot -> pixel for primary rays: 8x8 tiles in bands of 8 rows (the
 // last band and the last tile of a band may be smaller), so the 64 lanes of
@@ -107,21 +123,25 @@ __device__ __forceinline__ uint32_t tiled_pixel(uint32_t i, uint32_t w, uint32_t
     return (y0 + c / tw) * w + 8u * tx + c % tw;
 }
 
+// counters: the first of `nchains` chain counter blocks (kChainWords words
+// apart; the grid has at least nchains workgroups).
 __global__ __launch_bounds__(kThreads) void k_generate(const FrameConst fc,
                                                        const PassParams* __restrict__ pp,
                                                        RayQueue q, PathState ps,
                                                        uint32_t* __restrict__ counters,
-                                                       uint32_t* __restrict__ ray_counters,
-                                                       uint32_t nlevels) {
+                                                       uint32_t nlevels, uint32_t nchains) {
     const uint32_t npaths = fc.npix * fc.npasses;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
-        counters[0] = npaths;
-        for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
+        for (uint32_t c = 0; c < nchains; ++c) {
+            uint32_t* cc = counters + c * kChainWords;
+            cc[0] = chain_begin(npaths, c + 1, nchains) - chain_begin(npaths, c, nchains);
+            for (uint32_t k = 1; k <= nlevels; ++k) cc[k] = 0u;
+        }
         counters[kTrainLevel0Slot] = 0u;
         counters[kTrainLevel1Slot] = 0u;
     }
-    if (blockIdx.x == 0) zero_queue(ray_counters, nlevels);
+    if (blockIdx.x < nchains) zero_queue(counters + blockIdx.x * kChainWords + 64, nlevels);
     for (uint32_t slot = gid; slot < npaths; slot += gridDim.x * blockDim.x) {
         const uint32_t pass = slot / fc.npix;
         const PassParams& P = pp[pass];
@@ -476,6 +496,229 @@ __global__ __launch_bounds__(kShadeThreads) void k_shade(const FrameConst fc,
     }
 }
 
+// Hand-out of an index range in pools (persistent_trace's protocol).
+//
+// Wave-uniform state over [0, n): pools of `pool` indices from slice
+// counters of a queue block (see persistent_trace).  next_pool() leaves a
+// non-empty pool in [next, end) or sets `drained` once every slice is used
+// up; each call either takes a pool or adds a used-up slice to `seen`, so
+// it returns after at most nq + 1 atomics.
+struct Handout {
+    uint32_t* queue;
+    uint32_t nq, per, extra, pool, q;
+    uint64_t seen;
+    uint32_t next, end;
+    bool drained;
+
+    __device__ __forceinline__ void init(uint32_t* qb, uint32_t n, uint32_t pool_size, uint32_t wave_id) {
+        queue = qb;
+        nq = queue_slices(n);
+        per = n / nq;
+        extra = n % nq;
+        pool = pool_size;
+        q = wave_id % nq;
+        const uint64_t all = nq == 64u ? ~0ull : (1ull << nq) - 1ull;
+        seen = ~all;
+        next = end = 0;
+        drained = n == 0;
+    }
+
+    __device__ __forceinline__ void next_pool() {
+        const int lane = threadIdx.x & 63;
+        uint64_t* mask = reinterpret_cast<uint64_t*>(queue + kQueueMask);
+        while (!drained && next >= end) {
+            const uint32_t beg = q * per + min(q, extra);
+            const uint32_t len = per + (q < extra ? 1u : 0u);
+            uint32_t base = 0;
+            if (lane == 0) {
+                base = atomicAdd(queue + q * kQueueStride, pool);
+                if (base < len && len - base <= pool) atomicOr(mask, 1ull << q);
+            }
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base < len) {
+                next = beg + base;
+                end = beg + base + min(pool, len - base);
+                return;
+            }
+            seen |= 1ull << q;
+            uint32_t lo = 0, hi = 0;
+            if (lane == 0) {
+                const uint64_t m = __hip_atomic_load(mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                lo = static_cast<uint32_t>(m);
+                hi = static_cast<uint32_t>(m >> 32);
+            }
+            lo = __builtin_amdgcn_readfirstlane(lo);
+            hi = __builtin_amdgcn_readfirstlane(hi);
+            const uint64_t open = ~(seen | (static_cast<uint64_t>(hi) << 32) | lo);
+            if (open == 0ull) {
+                drained = true;
+            } else {
+                const uint64_t above = q < 63u ? open & (~0ull << (q + 1u)) : 0ull;
+                q = static_cast<uint32_t>(__builtin_ctzll(above ? above : open));
+            }
+        }
+    }
+};
+
+// Path-persistent frame (render.paths): one launch traces every path of a
+// frame from its primary ray to its end, one lane per path at a time.  A
+// lane advances its path's current segment by one node visit per loop
+// iteration (persistent_trace's visit); a lane whose segment has ended waits
+// until `shade_min` lanes of its wave wait (or none is traversing), then those
+// lanes shade together -- primary_hit / bounce_hit / next_bounce, the
+// wavefront's functions, so every path computes the same bits -- and either
+// start their next segment or end the path (radiance into the path state for
+// k_accumulate).  Lanes without a path take new ones (pools of path slots,
+// persistent_trace's hand-out) when fewer than `refill` are busy.  There are
+// no bounce levels, so no frame-wide wait for each level's longest ray: a
+// frame waits once, for its longest path.  Segments per bounce are counted
+// per workgroup in LDS and added to counters[b] at the end.
+enum : uint32_t { kLaneIdle = 0u, kLaneTrace = 1u, kLaneShade = 2u };
+
+__global__ __launch_bounds__(64) void k_paths_setup(uint32_t* __restrict__ counters, uint32_t* __restrict__ queue,
+                                                    uint32_t nlevels) {
+    for (uint32_t k = threadIdx.x; k <= nlevels && k < kTrainLevel0Slot; k += blockDim.x) counters[k] = 0u;
+    if (threadIdx.x == 0) {
+        counters[kTrainLevel0Slot] = 0u;
+        counters[kTrainLevel1Slot] = 0u;
+    }
+    zero_queue(queue, 1u);
+}
+
+template <int LDS, bool LIGHTS, bool DETAIL>
+__global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, const PassParams* __restrict__ pp,
+                                                           const pt_triangle_info* __restrict__ infos,
+                                                           const float4* __restrict__ nodes,
+                                                           const float4* __restrict__ tris,
+                                                           const uint32_t* __restrict__ off,
+                                                           const pt_dir_light* __restrict__ lights, PathState ps,
+                                                           uint32_t* __restrict__ counters,
+                                                           uint32_t* __restrict__ queue, uint32_t refill,
+                                                           uint32_t pool_large, uint32_t shade_min) {
+    __shared__ uint32_t s_stack[LDS][kPersistThreads];
+    __shared__ uint32_t s_level[kMaxLevels];
+    for (uint32_t k = threadIdx.x; k < kMaxLevels; k += blockDim.x) s_level[k] = 0u;
+    __syncthreads();
+    const uint32_t npaths = fc.npix * fc.npasses;
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u);
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    // Waves past one per kRayPoolSmall paths skip the loop (but reach the
+    // final barrier).
+    if (wave_id == 0u || static_cast<uint64_t>(wave_id) * kRayPoolSmall < npaths) {
+        uint32_t spill[kStackMax - LDS];
+        TraceCount cnt{0, 0, 0, 0};
+        Handout ho;
+        ho.init(queue, npaths, npaths / 4u >= gridDim.x * blockDim.x ? pool_large : kRayPoolSmall, wave_id);
+        RayState r;
+        uint32_t state = kLaneIdle;
+        uint32_t path = 0, pix = 0, pass = 0, bounce = 0;
+        uint32_t root = kInvalid, single = kInvalid;
+        V3 absorbed = v3(1.0f, 1.0f, 1.0f), acc = v3(0.0f, 0.0f, 0.0f);
+        const uint32_t w = static_cast<uint32_t>(fc.w), h = static_cast<uint32_t>(fc.h);
+        for (;;) {
+            // New paths for idle lanes.
+            uint64_t busy = __ballot(state != kLaneIdle);
+            if (!ho.drained && static_cast<uint32_t>(__popcll(busy)) < refill) {
+                uint64_t idle = ~busy;
+                while (idle != 0ull) {
+                    ho.next_pool();
+                    if (ho.drained) break;
+                    const uint32_t avail = ho.end - ho.next;
+                    const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
+                    const bool take = ((idle >> lane) & 1ull) && rank < avail;
+                    const uint32_t took = min(static_cast<uint32_t>(__popcll(idle)), avail);
+                    if (take) {
+                        const uint32_t slot = ho.next + rank;
+                        pass = slot / fc.npix;
+                        pix = tiled_pixel(slot - pass * fc.npix, w, h);
+                        path = pass * fc.npix + pix;
+                        const PassParams& P = pp[pass];
+                        const float tx = static_cast<float>(pix % w) / static_cast<float>(fc.w);
+                        const float ty = static_cast<float>(pix / w) / static_cast<float>(fc.h);
+                        const V3 d00 = v3(P.dirs[0], P.dirs[1], P.dirs[2]);
+                        const V3 d10 = v3(P.dirs[3], P.dirs[4], P.dirs[5]);
+                        const V3 d11 = v3(P.dirs[6], P.dirs[7], P.dirs[8]);
+                        const V3 d01 = v3(P.dirs[9], P.dirs[10], P.dirs[11]);
+                        r.o = v3(P.eye[0], P.eye[1], P.eye[2]);
+                        r.d = mix(mix(d00, d10, tx), mix(d01, d11, tx), ty);
+                        const uint32_t sb = off[P.scene], se = off[P.scene + 1];
+                        root = se - sb >= 2 ? sb : kInvalid;
+                        single = se - sb == 1 ? sb : kInvalid;
+                        bounce = 0;
+                        absorbed = v3(1.0f, 1.0f, 1.0f);
+                        acc = v3(0.0f, 0.0f, 0.0f);
+                        init_ray(r, root);
+                        state = kLaneTrace;
+                    }
+                    ho.next += took;
+                    idle = ~__ballot(state != kLaneIdle);
+                }
+            }
+            busy = __ballot(state != kLaneIdle);
+            if (busy == 0ull && ho.drained) break;
+            // Count the segments that start this iteration (bounce of each).
+            // (Counted when a lane enters kLaneTrace: at take and at continue.)
+            // One node visit for every traversing lane; a scene with fewer
+            // than two triangles answers at once.
+            if (state == kLaneTrace) {
+                if (root == kInvalid) {
+                    r.minT = kMinTInit;
+                    r.best = kInvalid;
+                    if (single != kInvalid) {
+                        float u = 0.0f, v = 0.0f;
+                        uint32_t id;
+                        const float t = tri_hit(tris, single, r.o, r.d, u, v, id);
+                        if (t <= kMinTInit) {
+                            r.minT = t; r.best = id; r.bu = u; r.bv = v;
+                        }
+                    }
+                    state = kLaneShade;
+                } else if (!(r.finite ? visit_step<false, LDS, true>(nodes, tris, r, s_stack, spill, cnt, fc.status)
+                                      : visit_step<false, LDS, false>(nodes, tris, r, s_stack, spill, cnt, fc.status))) {
+                    state = kLaneShade;
+                }
+            }
+            const uint64_t shading = __ballot(state == kLaneShade);
+            const uint64_t tracing = __ballot(state == kLaneTrace);
+            if (shading != 0ull && (static_cast<uint32_t>(__popcll(shading)) >= shade_min || tracing == 0ull)) {
+                if (state == kLaneShade) {
+                    atomicAdd(&s_level[bounce], 1u);  // this segment, now complete
+                    bool cont = false;
+                    if (r.best != kInvalid) {
+                        const PassParams& P = pp[pass];
+                        HitShade hs = hit_shade<DETAIL>(fc, infos, r.best, r.minT, r.bu, r.bv, r.o, r.d);
+                        const uint32_t sb = off[P.scene], se = off[P.scene + 1];
+                        if (bounce == 0) {
+                            primary_hit<LIGHTS>(hs, r.d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+                        } else {
+                            bounce_hit<LIGHTS>(hs, r.d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+                        }
+                        V3 no{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
+                        if (bounce < fc.bounces) cont = next_bounce(pix, P.pass, bounce, P.seedw, fc, hs, absorbed, no, nd);
+                        if (cont) {
+                            ++bounce;
+                            r.o = no;
+                            r.d = nd;
+                            init_ray(r, root);
+                            state = kLaneTrace;
+                        }
+                    }
+                    if (!cont) {
+                        ps.cr[path] = acc.x;
+                        ps.cg[path] = acc.y;
+                        ps.cb[path] = acc.z;
+                        state = kLaneIdle;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kMaxLevels && k < kTrainLevel0Slot; k += blockDim.x)
+        if (s_level[k]) atomicAdd(counters + k, s_level[k]);
+}
+
 // Cache frames: the training paths, one lane per path from its primary ray
 // to its end, on a stream of their own beside the pixel paths' two levels,
 // the inference and the resolve (the per-level launches of a few thousand
@@ -723,34 +966,95 @@ struct GraphKey {
     const void* accum;
     const void* counters;
     const void* lights;
-    uint32_t blocks, refill, lds, pool;
+    uint32_t blocks, refill, lds, pool, chains;
     hipStream_t stream;
 };
 
-void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, hipStream_t st) {
+RayQueue shifted(const RayQueue& q, uint32_t k) {
+    return RayQueue{q.ox + k, q.oy + k, q.oz + k, q.dx + k, q.dy + k, q.dz + k, q.path + k};
+}
+HitQueue shifted(const HitQueue& h, uint32_t k) { return HitQueue{h.t + k, h.id + k, h.u + k, h.v + k}; }
+
+// Chains of a plain frame of npaths paths (render.chains, at most one per
+// kMinChainPaths paths, at least one); 0: a path-persistent frame
+// (render.paths).
+uint32_t frame_chains(size_t npaths) {
+    if (render_tuning().paths) return 0u;
+    const uint32_t want = std::min(kMaxChains, std::max(1u, render_tuning().chains));
+    const size_t fit = std::max<size_t>(1, npaths / kMinChainPaths);
+    return static_cast<uint32_t>(std::min<size_t>(want, fit));
+}
+
+// Chain streams and their fork/join events (default priority), created
+// outside any capture before the first frame that uses them.
+void ensure_chain_streams(Renderer& r, uint32_t nchains) {
+    if (nchains > 1 && !r.chain_fork) PT_HIP(hipEventCreateWithFlags(&r.chain_fork, hipEventDisableTiming));
+    for (uint32_t c = 1; c < nchains; ++c) {
+        if (!r.chain_st[c - 1]) PT_HIP(hipStreamCreateWithFlags(&r.chain_st[c - 1], hipStreamNonBlocking));
+        if (!r.chain_join[c - 1]) PT_HIP(hipEventCreateWithFlags(&r.chain_join[c - 1], hipEventDisableTiming));
+    }
+}
+
+template <int LDS>
+auto paths_kernel(const FrameConst& fc) {
+    return fc.detail ? (fc.nlights ? k_paths<LDS, true, true> : k_paths<LDS, false, true>)
+                     : (fc.nlights ? k_paths<LDS, true, false> : k_paths<LDS, false, false>);
+}
+
+void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, uint32_t nchains, hipStream_t st) {
     const size_t npaths = static_cast<size_t>(fc.npix) * fc.npasses;
     Layout L = carve(r.queues.as<char>(), npaths);
     uint32_t* cnt = r.counters.as<uint32_t>();
-    uint32_t* rcnt = cnt + 64;
     const PassParams* pp = r.params.as<PassParams>();
     const TraceTuning& tune = trace_tuning();
     const uint32_t nlev = fc.bounces + 1;
     const uint32_t grid = grid_for(npaths);
-    const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
     const uint32_t refill = std::min(64u, std::max(1u, tune.refill));
-    const uint32_t pool = pool_for(tune.pool, static_cast<uint32_t>(std::min<size_t>(npaths, 0xffffffffu)), pgrid);
+    if (nchains == 0) {  // path-persistent frame
+        hipLaunchKernelGGL(k_paths_setup, dim3(1), dim3(64), 0, st, cnt, cnt + 64, nlev);
+        const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid);
+        const uint32_t pool = pool_for(tune.pool, static_cast<uint32_t>(npaths), pgrid);
+        auto kern = tune.lds <= 8 ? paths_kernel<8>(fc) : paths_kernel<16>(fc);
+        hipLaunchKernelGGL(kern, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
+                           scene.infos.as<pt_triangle_info>(), scene.bvh.nodes.as<float4>(),
+                           scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>(), scene.lights.as<pt_dir_light>(),
+                           L.ps, cnt, cnt + 64, refill, pool,
+                           std::min(64u, std::max(1u, render_tuning().shade_min)));
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
+                           r.accum.as<double>());
+        return;
+    }
     auto extend = tune.lds <= 8 ? k_extend<8> : (tune.lds <= 16 ? k_extend<16> : k_extend<32>);
     auto shade = fc.detail ? (fc.nlights ? k_shade<false, true, true> : k_shade<false, false, true>)
                            : (fc.nlights ? k_shade<false, true, false> : k_shade<false, false, false>);
-    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt, nlev);
-    for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
-        hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
-                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
-                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill, pool);
-        hipLaunchKernelGGL(shade, dim3(shade_grid_for(npaths)), dim3(kShadeThreads), 0, st, fc, pp,
-                           scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
-                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, CacheIO{}, scene.lights.as<pt_dir_light>(),
-                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
+    hipLaunchKernelGGL(k_generate, dim3(std::max(grid, nchains)), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt,
+                       nlev, nchains);
+    if (nchains > 1) {
+        PT_HIP(hipEventRecord(r.chain_fork, st));
+        for (uint32_t c = 1; c < nchains; ++c) PT_HIP(hipStreamWaitEvent(r.chain_st[c - 1], r.chain_fork, 0));
+    }
+    const uint32_t np32 = static_cast<uint32_t>(npaths);
+    for (uint32_t c = 0; c < nchains; ++c) {
+        hipStream_t cs = c == 0 ? st : r.chain_st[c - 1];
+        const uint32_t b = chain_begin(np32, c, nchains), len = chain_begin(np32, c + 1, nchains) - b;
+        const RayQueue q[2] = {shifted(L.q[0], b), shifted(L.q[1], b)};
+        const HitQueue hq = shifted(L.hq, b);
+        uint32_t* cc = cnt + c * kChainWords;
+        const uint32_t pgrid = std::min(std::max(1u, tune.blocks), grid_for(len));
+        const uint32_t pool = pool_for(tune.pool, len, pgrid);
+        for (uint32_t lvl = 0; lvl < nlev; ++lvl) {
+            hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, cs, fc, pp,
+                               scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
+                               scene.off.as<uint32_t>(), q[lvl & 1], hq, cc, cc + 64, lvl, refill, pool);
+            hipLaunchKernelGGL(shade, dim3(shade_grid_for(len)), dim3(kShadeThreads), 0, cs, fc, pp,
+                               scene.infos.as<pt_triangle_info>(), q[lvl & 1], hq, q[(lvl + 1) & 1], L.ps, cc, lvl,
+                               CacheIO{}, scene.lights.as<pt_dir_light>(), scene.bvh.nodes.as<float4>(),
+                               scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
+        }
+        if (c > 0) {
+            PT_HIP(hipEventRecord(r.chain_join[c - 1], cs));
+            PT_HIP(hipStreamWaitEvent(st, r.chain_join[c - 1], 0));
+        }
     }
     hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
                        r.accum.as<double>());
@@ -764,6 +1068,11 @@ Renderer::~Renderer() {
     for (auto& e : ring_ev)
         if (e) (void)hipEventDestroy(e);
     if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (chain_fork) (void)hipEventDestroy(chain_fork);
+    for (auto& e : chain_join)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& c : chain_st)
+        if (c) (void)hipStreamDestroy(c);
     if (join_ev) (void)hipEventDestroy(join_ev);
     if (side) (void)hipStreamDestroy(side);
 }
@@ -777,8 +1086,8 @@ void render_resize(Renderer& r, int w, int h, hipStream_t st) {
     r.w = w;
     r.h = h;
     r.accum.reserve(np * 3 * sizeof(double));
-    // Queue lengths (64 words), then one ray queue block per level.
-    r.counters.reserve((64 + static_cast<size_t>(kMaxLevels) * kQueueWords) * sizeof(uint32_t));
+    // Per chain: queue lengths (64 words), then one ray queue block per level.
+    r.counters.reserve(kMaxChains * kChainWords * sizeof(uint32_t));
     render_reset(r, st);
 }
 
@@ -857,11 +1166,23 @@ void replay(void*& exec, char (&key_store)[Cap], const Key& key, hipStream_t st,
         }
         hipGraph_t g = nullptr;
         PT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        launch();
+        try {
+            launch();
+        } catch (...) {
+            // End the capture so the stream is usable again; drop the partial graph.
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(st, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            (void)hipGetLastError();
+            throw;
+        }
         const hipError_t le = hipGetLastError();
         const hipError_t ce = hipStreamEndCapture(st, &g);
-        if (le != hipSuccess) throw Error(PT_E_HIP, std::string("frame capture: ") + hipGetErrorString(le));
-        if (ce != hipSuccess) throw Error(PT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (le != hipSuccess) throw Error(PT_E_HIP, std::string("frame capture: ") + hipGetErrorString(le));
+            throw Error(PT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        }
         hipGraphExec_t ex = nullptr;
         const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -900,8 +1221,10 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
     fc.status = scene.status.as<uint32_t>();
     set_detail(fc, scene);
 
+    const uint32_t nchains = frame_chains(npix * npasses);
+    ensure_chain_streams(r, std::max(1u, nchains));
     if (!render_tuning().graphs) {
-        launch_frame(r, scene, fc, st);
+        launch_frame(r, scene, fc, nchains, st);
         PT_HIP(hipGetLastError());
     } else {
         GraphKey key;
@@ -920,12 +1243,14 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         key.refill = trace_tuning().refill;
         key.lds = trace_tuning().lds;
         key.pool = trace_tuning().pool;
+        key.chains = nchains;
         key.stream = st;
         replay(r.graph_exec, r.graph_key, key, st,
-               [&] { launch_frame(r, scene, fc, st); });
+               [&] { launch_frame(r, scene, fc, nchains, st); });
     }
     r.spp += npasses;
     r.last_levels = fc.bounces + 1;
+    r.last_chains = std::max(1u, nchains);
 }
 
 namespace {
@@ -1084,7 +1409,7 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
         const void* ptrs[13] = {scene.bvh.nodes.as<void>(), scene.bvh.tris.as<void>(), scene.infos.as<void>(),
                                 scene.off.as<void>(),       r.queues.as<void>(),     r.params.as<void>(),
                                 r.accum.as<void>(),         r.counters.as<void>(),   cache.master.as<void>(),
-                                cache.acts.as<void>(),      cache.part.as<void>(),   cache.loss.as<void>(),
+                                cache.f32.as<void>(),       cache.part.as<void>(),   cache.loss.as<void>(),
                                 scene.lights.as<void>()};
         std::memcpy(key.ptrs, ptrs, sizeof ptrs);
         for (int k = 0; k < 3; ++k) {
@@ -1101,6 +1426,7 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     }
     r.spp += 1;
     r.last_levels = ntrain ? std::max(2u, cf.io.train_levels) : 2u;
+    r.last_chains = 1;
     r.last_train = ntrain;
 }
 
@@ -1133,12 +1459,17 @@ uint64_t render_last_rays(Renderer& r, hipStream_t st) {
     if (r.last_levels == 0) return 0;
     // Levels 0 .. last_levels-1 plus the cache frame's training paths at
     // their first two levels (slots 62 and 63, zeroed by every frame).
-    uint32_t h[64];
-    PT_HIP(hipMemcpyAsync(h, r.counters.as<void>(), 64 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    // A plain frame's chains each count their own levels.
+    uint32_t h[kMaxChains][64];
+    const uint32_t nch = std::max(1u, std::min(kMaxChains, r.last_chains));
+    for (uint32_t c = 0; c < nch; ++c)
+        PT_HIP(hipMemcpyAsync(h[c], r.counters.as<uint32_t>() + c * kChainWords, 64 * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, st));
     PT_HIP(hipStreamSynchronize(st));
     uint64_t total = 0;
-    for (uint32_t k = 0; k < std::min(r.last_levels, kTrainLevel0Slot); ++k) total += h[k];
-    return total + h[kTrainLevel0Slot] + h[kTrainLevel1Slot];
+    for (uint32_t c = 0; c < nch; ++c)
+        for (uint32_t k = 0; k < std::min(r.last_levels, kTrainLevel0Slot); ++k) total += h[c][k];
+    return total + h[0][kTrainLevel0Slot] + h[0][kTrainLevel1Slot];
 }
 
 RenderTuning& render_tuning() {

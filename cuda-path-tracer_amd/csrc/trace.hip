@@ -61,30 +61,37 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
     if (COUNT) add_counts(cnt, counters);
 }
 
-// Persistent closest-hit kernel over an AoS ray batch (ntri >= 2).
-template <bool COUNT, int LDS>
+// Persistent closest-hit kernel over an AoS ray batch (ntri >= 2); FILTER:
+// the root filter (persistent_trace_filtered), without counting.
+template <bool COUNT, int LDS, bool FILTER>
 __global__ __launch_bounds__(kPersistThreads) void k_trace_persistent(
     const float4* __restrict__ nodes, const float4* __restrict__ tris, const float* __restrict__ rays,
     uint32_t nrays, uint32_t* __restrict__ out_id, float* __restrict__ out_t,
     float2* __restrict__ out_uv, uint32_t* __restrict__ ray_counter, uint32_t refill, uint32_t pool,
     unsigned long long* __restrict__ counters, uint32_t* __restrict__ status) {
+    static_assert(!(COUNT && FILTER), "the filter's repeated root visit would change the visit counts");
     __shared__ uint32_t s_stack[LDS][kPersistThreads];
-    TraceCount cnt{0, 0, 0, 0};
-    persistent_trace<COUNT, LDS>(
-        nodes, tris, nrays, ray_counter, status, refill, pool, s_stack,
-        [&](uint32_t i, V3& o, V3& d, uint32_t&) -> uint32_t {
-            const float* rp = rays + 6 * static_cast<size_t>(i);
-            o = v3(rp[0], rp[1], rp[2]);
-            d = v3(rp[3], rp[4], rp[5]);
-            return 0u;
-        },
-        [&](uint32_t i, const TraceHit& h) {
-            if (out_id) out_id[i] = h.id;
-            if (out_t) out_t[i] = h.t;
-            if (out_uv) out_uv[i] = make_float2(h.u, h.v);
-        },
-        cnt);
-    if (COUNT) add_counts(cnt, counters);
+    auto load = [&](uint32_t i, V3& o, V3& d, uint32_t&) -> uint32_t {
+        const float* rp = rays + 6 * static_cast<size_t>(i);
+        o = v3(rp[0], rp[1], rp[2]);
+        d = v3(rp[3], rp[4], rp[5]);
+        return 0u;
+    };
+    auto store = [&](uint32_t i, const TraceHit& h) {
+        if (out_id) out_id[i] = h.id;
+        if (out_t) out_t[i] = h.t;
+        if (out_uv) out_uv[i] = make_float2(h.u, h.v);
+    };
+    if constexpr (FILTER) {
+        __shared__ uint32_t s_surv[kPersistThreads / 64][kSurvCap];
+        persistent_trace_filtered<LDS>(nodes, tris, nrays, ray_counter, status, refill, pool, s_stack,
+                                       s_surv[threadIdx.x / 64], load, store);
+    } else {
+        TraceCount cnt{0, 0, 0, 0};
+        persistent_trace<COUNT, LDS>(nodes, tris, nrays, ray_counter, status, refill, pool, s_stack, load, store,
+                                     cnt);
+        if (COUNT) add_counts(cnt, counters);
+    }
 }
 
 // Persistent closest hit over rays tagged with a scene of a batch: the ray
@@ -139,8 +146,13 @@ void launch_trace(const DeviceScene& s, uint32_t* queue, const float* d_rays, ui
     const uint64_t want = (static_cast<uint64_t>(n) + kPersistThreads - 1) / kPersistThreads;
     const uint32_t blocks =
         static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, tune.blocks)));
-    auto kern = tune.lds <= 8 ? k_trace_persistent<COUNT, 8>
-                : (tune.lds <= 16 ? k_trace_persistent<COUNT, 16> : k_trace_persistent<COUNT, 32>);
+    const bool filter = !COUNT && tune.filter != 0;
+    auto kern = filter ? (tune.lds <= 8 ? k_trace_persistent<false, 8, true>
+                                        : (tune.lds <= 16 ? k_trace_persistent<false, 16, true>
+                                                          : k_trace_persistent<false, 32, true>))
+                       : (tune.lds <= 8 ? k_trace_persistent<COUNT, 8, false>
+                                        : (tune.lds <= 16 ? k_trace_persistent<COUNT, 16, false>
+                                                          : k_trace_persistent<COUNT, 32, false>));
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(kPersistThreads), 0, st,
                        s.bvh.nodes.as<float4>(), s.bvh.tris.as<float4>(), d_rays, n, d_id, d_t,
                        reinterpret_cast<float2*>(d_uv), queue, std::min(64u, std::max(1u, tune.refill)),

@@ -504,4 +504,186 @@ __device__ __forceinline__ void persistent_trace(const float4* __restrict__ node
     }
 }
 
+// ---- root filter ---------------------------------------------------------
+//
+// A batch of rays that start outside the scene's box (the closest-hit
+// microbench's shell rays) mostly leaves after the root visit.  In the
+// persistent loop above such a ray takes a lane through a refill and one
+// visit while its neighbours walk deep paths, so most of the wave's issue
+// slots go to lanes that do nothing.  The filtered variant first runs the
+// root visit of 64 fresh rays at once on every lane of the wave (busy lanes
+// keep their own ray's state in other registers), answers the rays that end
+// there, and queues the rest ("survivors") in a per-wave LDS ring; idle
+// lanes take survivors from the ring and trace them from the root as above.
+// Results are the same bits: a ray that ends at the root visit ends exactly
+// as the full traversal would, and a survivor is traced in full.
+
+constexpr uint32_t kSurvCap = 128;  // survivor ring entries per wave
+
+// The root visit alone (visit_step's decisions without the stack): leaf
+// children fold into the best hit; returns whether an internal child is to
+// be visited.
+template <bool FINITE>
+__device__ __forceinline__ bool root_visit(const float4* __restrict__ nodes, const float4* __restrict__ tris,
+                                           RayState& r) {
+    const float4* rec = nodes + 4 * static_cast<size_t>(r.node);
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    const uint32_t w = __float_as_uint(r3.x);
+    const uint32_t l = w >> 2;
+    const bool in0 = (w & 1u) != 0, in1 = (w & 2u) != 0;
+    const float h0 = child_box<false, FINITE>(r.o, r.dinv, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y);
+    const float h1 = child_box<false, FINITE>(r.o, r.dinv, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w);
+    const uint32_t nleaf = static_cast<uint32_t>(!in0) + static_cast<uint32_t>(!in1);
+    for (uint32_t j = 0; j < nleaf; ++j) {
+        float u, v;
+        uint32_t id;
+        const float t = tri_hit(tris, (j == 0 && !in0) ? l : l + 1, r.o, r.d, u, v, id);
+        if (t < r.minT || (t == r.minT && id < r.best)) {
+            r.minT = t; r.best = id; r.bu = u; r.bv = v;
+        }
+    }
+    return (in0 && h0 <= r.minT) || (in1 && h1 <= r.minT);
+}
+
+__device__ __forceinline__ void init_ray(RayState& r, uint32_t root) {
+    r.dinv = v3(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    r.finite = __builtin_fabsf(r.dinv.x) <= kFltMax && __builtin_fabsf(r.dinv.y) <= kFltMax &&
+               __builtin_fabsf(r.dinv.z) <= kFltMax;
+    r.minT = kMinTInit;
+    r.best = kInvalid;
+    r.bu = 0.0f;
+    r.bv = 0.0f;
+    r.node = root;
+    r.sp = 0;
+}
+
+__device__ __forceinline__ TraceHit final_hit(const RayState& r) {
+    TraceHit h;
+    h.id = r.best;
+    h.t = r.best == kInvalid ? kFltMax : r.minT;
+    h.u = r.best == kInvalid ? 0.0f : r.bu;
+    h.v = r.best == kInvalid ? 0.0f : r.bv;
+    return h;
+}
+
+// persistent_trace with the root filter (no counting).  s_surv: this wave's
+// kSurvCap-entry ring in LDS.  The hand-out protocol (pools from slice
+// counters, used-up mask, `seen`) is persistent_trace's; a wave stops once
+// every slice is used up, its ring is empty and no lane is busy.
+template <int LDS, class Load, class Store>
+__device__ __forceinline__ void persistent_trace_filtered(const float4* __restrict__ nodes,
+                                                          const float4* __restrict__ tris, uint32_t nrays,
+                                                          uint32_t* __restrict__ queue, uint32_t* status,
+                                                          uint32_t refill, uint32_t pool_large,
+                                                          uint32_t (*s_stack)[kPersistThreads],
+                                                          uint32_t* __restrict__ s_surv, Load load, Store store) {
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u);
+    if (wave_id > 0u && static_cast<uint64_t>(wave_id) * kRayPoolSmall >= nrays) return;
+    uint32_t spill[kStackMax - LDS];
+    TraceCount cnt{0, 0, 0, 0};
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    RayState r;
+    bool active = false;
+    uint32_t ray = 0;
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform
+    bool drained = false;                  // wave-uniform: no rays left to hand out
+    uint32_t qhead = 0, qtail = 0;         // wave-uniform: survivor ring positions
+    const uint32_t pool =
+        nrays / 4u >= gridDim.x * blockDim.x ? pool_large : kRayPoolSmall;
+    const uint32_t nq = queue_slices(nrays);
+    const uint32_t per = nrays / nq, extra = nrays % nq;
+    const uint64_t all = nq == 64u ? ~0ull : (1ull << nq) - 1ull;
+    uint64_t* mask = reinterpret_cast<uint64_t*>(queue + kQueueMask);
+    uint32_t q = wave_id % nq;
+    uint64_t seen = ~all;
+    for (;;) {
+        const uint32_t nbusy = static_cast<uint32_t>(__popcll(__ballot(active)));
+        if (nbusy < refill && (!drained || qtail != qhead)) {
+            // Filter rounds until the ring can fill every idle lane (or the
+            // rays run out).  Each pass either consumes pool rays, takes a
+            // new pool, or finds a slice used up (at most nq times).
+            while (!drained && qtail - qhead < 64u - nbusy && qtail - qhead <= kSurvCap - 64u) {
+                if (pool_next >= pool_end) {
+                    const uint32_t beg = q * per + min(q, extra);
+                    const uint32_t len = per + (q < extra ? 1u : 0u);
+                    uint32_t base = 0;
+                    if (lane == 0) {
+                        base = atomicAdd(queue + q * kQueueStride, pool);
+                        if (base < len && len - base <= pool) atomicOr(mask, 1ull << q);
+                    }
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    if (base >= len) {
+                        seen |= 1ull << q;
+                        uint32_t lo = 0, hi = 0;
+                        if (lane == 0) {
+                            const uint64_t m = __hip_atomic_load(mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            lo = static_cast<uint32_t>(m);
+                            hi = static_cast<uint32_t>(m >> 32);
+                        }
+                        lo = __builtin_amdgcn_readfirstlane(lo);
+                        hi = __builtin_amdgcn_readfirstlane(hi);
+                        const uint64_t open = ~(seen | (static_cast<uint64_t>(hi) << 32) | lo);
+                        if (open == 0ull) {
+                            drained = true;
+                        } else {
+                            const uint64_t above = q < 63u ? open & (~0ull << (q + 1u)) : 0ull;
+                            q = static_cast<uint32_t>(__builtin_ctzll(above ? above : open));
+                        }
+                        continue;
+                    }
+                    pool_next = beg + base;
+                    pool_end = beg + base + min(pool, len - base);
+                }
+                const uint32_t avail = min(64u, pool_end - pool_next);
+                const uint32_t fr = pool_next + static_cast<uint32_t>(lane);
+                bool surv = false;
+                if (static_cast<uint32_t>(lane) < avail) {
+                    RayState f;
+                    uint32_t single = kInvalid;
+                    const uint32_t root = load(fr, f.o, f.d, single);
+                    if (root == kInvalid) {
+                        TraceHit h{kFltMax, kInvalid, 0.0f, 0.0f};
+                        if (single != kInvalid) {
+                            float u = 0.0f, v = 0.0f;
+                            uint32_t id;
+                            const float t = tri_hit(tris, single, f.o, f.d, u, v, id);
+                            if (t <= kMinTInit) h = TraceHit{t, id, u, v};
+                        }
+                        store(fr, h);
+                    } else {
+                        init_ray(f, root);
+                        surv = f.finite ? root_visit<true>(nodes, tris, f) : root_visit<false>(nodes, tris, f);
+                        if (!surv) store(fr, final_hit(f));
+                    }
+                }
+                const uint64_t sm = __ballot(surv);
+                if (surv) s_surv[(qtail + static_cast<uint32_t>(__popcll(sm & lt))) & (kSurvCap - 1u)] = fr;
+                qtail += static_cast<uint32_t>(__popcll(sm));
+                pool_next += avail;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // Idle lanes take survivors, in ring order.
+            const uint64_t idle = ~__ballot(active);
+            const uint32_t qn = qtail - qhead;
+            const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
+            if (((idle >> lane) & 1ull) && rank < qn) {
+                ray = s_surv[(qhead + rank) & (kSurvCap - 1u)];
+                uint32_t single = kInvalid;
+                const uint32_t root = load(ray, r.o, r.d, single);
+                init_ray(r, root);
+                active = true;
+            }
+            qhead += min(static_cast<uint32_t>(__popcll(idle)), qn);
+        }
+        if (__ballot(active) == 0ull && drained && qtail == qhead) break;
+        if (active && !(r.finite ? visit_step<false, LDS, true>(nodes, tris, r, s_stack, spill, cnt, status)
+                                 : visit_step<false, LDS, false>(nodes, tris, r, s_stack, spill, cnt, status))) {
+            store(ray, final_hit(r));
+            active = false;
+        }
+    }
+}
+
 }  // namespace pt

@@ -113,6 +113,7 @@ _SIGS = {
     "pt_build_bvh": [_vp, _vp],
     "pt_get_bvh": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pt_get_scene_info": [_vp, _vp, _vp, _vp],
+    "pt_verify_bvh": [_vp, _vp],
     "pt_trace_closest": [_vp, _vp, _u32, _vp, _vp, _vp],
     "pt_trace_closest_device": [_vp, _vp, _u32, _vp, _vp, _vp, _vp],
     "pt_trace_closest_device_noalloc": [_vp, _vp, _u32, _vp, _vp, _vp, _vp],
@@ -498,6 +499,13 @@ class Context:
         bmax = np.zeros(3, np.float32)
         self._c(_lib.pt_get_scene_info(self._h, _ptr(n), _ptr(bmin), _ptr(bmax)))
         return int(n[0]), bmin, bmax
+
+    def verify_bvh(self):
+        """The build self-check: raises PtError unless every scene's root box
+        equals its enclosing box; returns 0 (the mismatch count) otherwise."""
+        bad = np.zeros(1, np.uint32)
+        self._c(_lib.pt_verify_bvh(self._h, _ptr(bad)))
+        return int(bad[0])
 
     def trace_closest(self, rays):
         rays = _f32(rays).reshape(-1, 6)

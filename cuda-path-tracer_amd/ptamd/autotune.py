@@ -76,10 +76,11 @@ def trace_space(num_cus=256):
 
 def stack_space():
     """Closest-hit traversal stack and queue: LDS stack entries per lane
-    (more entries, fewer spills to scratch, lower occupancy) and the rays a
-    wave takes from the queue at a time."""
+    (more entries, fewer spills to scratch, lower occupancy), the rays a
+    wave takes from the queue at a time, and the root filter (on or off)."""
     return {"trace.lds_stack": [8, 16, 32],
-            "trace.pool": [128, 256, 512, 1024]}
+            "trace.pool": [128, 256, 512, 1024],
+            "trace.filter": [0, 1]}
 
 
 def render_space(num_cus=256):
@@ -90,6 +91,15 @@ def render_space(num_cus=256):
             "trace.refill": [32, 48],
             "trace.lds_stack": [8, 16],
             "trace.pool": [128, 256]}
+
+
+def render_stages(num_cus=256):
+    """Staged render search: the frame's structure first (path-persistent,
+    or bounce levels in 1, 2 or 4 chains), then how many waiting lanes a
+    path-persistent wave shades together, then the traversal knobs."""
+    return [{"render.paths": [1, 0], "render.chains": [1, 2, 4]},
+            {"render.shade_min": [16, 32, 48]},
+            render_space(num_cus)]
 
 
 def trace_stages(num_cus=256):

@@ -199,6 +199,16 @@ int pt_get_bvh_scene(pt_context* ctx, uint32_t scene, uint32_t* morton, uint32_t
                      float* bboxes);
 /* Total triangles and the enclosing box of scene 0. */
 int pt_get_scene_info(pt_context* ctx, uint32_t* ntris, float* bmin, float* bmax);
+/* The build's self-check (BVHBuilder.cu:258-265 asserts that the root box's
+ * volume equals the enclosing box's), made exact and returned as a status:
+ * PT_OK when every non-empty scene's root box equals, bound for bound, the
+ * enclosing box its Morton keys were computed in; PT_E_INTERNAL (message
+ * naming the first such scene) otherwise.  bad_scenes (may be NULL) receives
+ * the number of mismatched scenes.  Holds for boxes pt_upload_scene(s)
+ * computed itself (bmin/bmax NULL, as the reference computes them); a
+ * caller-given box larger than the geometry is reported as a mismatch.
+ * Host-synchronous. */
+int pt_verify_bvh(pt_context* ctx, uint32_t* bad_scenes);
 
 /* ---- closest hit ---------------------------------------------------------- */
 /* Batch form of CUDA::trace (trace.h:73-169): rays[i] = origin.xyz, dir.xyz.
@@ -359,8 +369,21 @@ int pt_device_compute_units(int device, uint32_t* cus);
  *                   32; deeper entries spill to scratch)
  *   "trace.pool"    rays a wave takes from the ray queue at a time on long
  *                   queues (64..4096)
+ *   "trace.filter"  1: closest-hit batches run every ray's root visit in
+ *                   converged 64-ray rounds and trace only the rays that go
+ *                   deeper (fast when most rays miss the scene); 0: off
  *   "render.graphs" 1: replay each frame's kernels as one captured hipGraph,
  *                   0: launch them one by one
+ *   "render.paths"  1: a plain frame traces every path from its primary
+ *                   ray to its end in one persistent launch (no bounce
+ *                   levels); 0: bounce levels (wavefront)
+ *   "render.shade_min" 1..64: in path-persistent frames, the lanes of a
+ *                   wave whose segments have ended shade together once this
+ *                   many wait (or none is traversing)
+ *   "render.chains" 1..8: with render.paths 0, a plain frame's paths split
+ *                   into this many chains of bounce levels on streams of
+ *                   their own (at most one per 16,384 paths), so one chain's
+ *                   level tail overlaps the others' work
  * Unknown names and values outside a knob's range return PT_E_INVALID_ARG. */
 int pt_tune_set(const char* name, uint32_t value);
 int pt_tune_get(const char* name, uint32_t* value);

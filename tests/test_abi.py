@@ -61,6 +61,13 @@ def test_tuning_knobs():
     for v in (63, 4097, 0):
         assert ptamd._lib.pt_tune_set(b"trace.pool", v) == ptamd.PT_E_INVALID_ARG
     assert ptamd.tune_get("trace.pool") == 256
+    chains = ptamd.tune_get("render.chains")
+    for v in (1, 8):
+        ptamd.tune_set("render.chains", v)
+        assert ptamd.tune_get("render.chains") == v
+    for v in (0, 9):
+        assert ptamd._lib.pt_tune_set(b"render.chains", v) == ptamd.PT_E_INVALID_ARG
+    ptamd.tune_set("render.chains", chains)
 
 
 def test_null_arguments_rejected():

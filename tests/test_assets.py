@@ -121,10 +121,20 @@ def test_errors(tmp_path):
     assert "out of range" in str(e.value)
 
 
-CORNELL = "/root/reference/assets/meshes/cornell-box/CornellBox-Original.obj"
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CORNELL = os.path.join(GOLDEN, "CornellBox-Original.obj")  # the reference's asset, copied as a fixture
 
 
-@pytest.mark.skipif(not os.path.exists(CORNELL), reason="reference assets not present")
+def test_reference_bunny():
+    tris, infos = ptamd.load_obj(os.path.join(GOLDEN, "bunny.obj"))
+    assert len(tris) == 4968  # "face count = 4968" in the file's header
+    assert np.all(infos["emission"] == 0)
+    # No normals in the file: every vertex normal is the face's geometric
+    # normal (PathTracingApp.cpp:74-77), a unit vector.
+    np.testing.assert_allclose(np.linalg.norm(infos["n0"], axis=1), 1.0, rtol=1e-5)
+    assert np.all(tris["id"] == np.arange(len(tris)))
+
+
 def test_reference_cornell_box():
     tris, infos = ptamd.load_obj(CORNELL)
     assert len(tris) > 20

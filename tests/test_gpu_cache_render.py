@@ -94,6 +94,42 @@ def test_cache_frame_matches_oracle(ctx, room, rr_start, strat):
     assert acc_o.sum() > 0 and max(err) <= 1e-3
 
 
+def test_cache_frame_ray_count(ctx, room):
+    """pt_last_frame_rays after a cache frame: without training paths it is
+    the plain frame's count with one bounce at the same seed (a pixel path
+    follows the plain path to x2); the training paths add between one and
+    train_levels segments each (at least two for a path that reached x2);
+    graph replays and eager frames count the same."""
+    st = ptamd.Settings(8, 3, 1.0, 1)
+    bmin, bmax = setup(ctx, room, st)
+    ctx.cache_init(3, bmin, bmax)
+    seed = 4242
+    ctx.reset_frames()
+    ctx.render_cache_frame(W, H, seed, ptamd.CacheSettings(0, 1, 8, train=False))
+    pixel_rays = ctx.last_frame_rays()
+    ctx.upload_scene(*room, settings=ptamd.Settings(1, 3, 1.0, 1))
+    ctx.set_primary_rays(dirs_for(W, H), EYE)
+    ctx.reset_frames()
+    ctx.render_pass(W, H, seed)
+    assert ctx.last_frame_rays() == pixel_rays
+    ntrain, extra = 900, 8
+    counts = {}
+    for graphs in (1, 0):
+        ptamd.tune_set("render.graphs", graphs)
+        try:
+            setup(ctx, room, st)
+            ctx.cache_init(3, bmin, bmax)
+            ctx.render_cache_frame(W, H, seed, ptamd.CacheSettings(ntrain, 1, extra, train=False))
+            counts[graphs] = ctx.last_frame_rays()
+        finally:
+            ptamd.tune_set("render.graphs", 1)
+    assert counts[0] == counts[1]
+    _, tgt = ctx.cache_last_batch(ntrain)
+    reached = int((~np.isnan(tgt[:, 0])).sum())
+    train_rays = counts[1] - pixel_rays
+    assert ntrain + reached <= train_rays <= ntrain * (2 + extra)
+
+
 def test_in_frame_training_equals_cache_train(ctx, room):
     st = ptamd.Settings(8, 3, 1.0, 1)
     bmin, bmax = setup(ctx, room, st)

@@ -176,11 +176,12 @@ def test_trace_invariant_under_tuning(ctx):
     ctx.upload_scene(tris, infos)
     b = orc.build(tris, bmin, bmax)
     oids, ot, ouv = orc.trace(tris, b, rays)
-    knobs = ("trace.blocks", "trace.refill", "trace.lds_stack", "trace.pool")
+    knobs = ("trace.blocks", "trace.refill", "trace.lds_stack", "trace.pool", "trace.filter")
     saved = {k: ptamd.tune_get(k) for k in knobs}
     try:
-        for setting in ((1, 1, 16, 256), (3, 64, 8, 4096), (64, 8, 32, 64), (256, 32, 8, 1024),
-                        (100000, 48, 32, 4096), (2048, 63, 16, 100), (32, 48, 8, 256)):
+        for setting in ((1, 1, 16, 256, 1), (3, 64, 8, 4096, 1), (64, 8, 32, 64, 0), (256, 32, 8, 1024, 1),
+                        (100000, 48, 32, 4096, 0), (2048, 63, 16, 100, 1), (32, 48, 8, 256, 0),
+                        (2048, 32, 16, 256, 0), (2048, 32, 32, 128, 1), (7, 2, 8, 64, 1)):
             say(f"[tuning invariance] {dict(zip(knobs, setting))}")
             for k, v in zip(knobs, setting):
                 ptamd.tune_set(k, v)
@@ -497,3 +498,23 @@ def test_traces_on_user_streams(ctx):
         s1.destroy()
         s2.destroy()
     assert ctx.get_status() == 0
+
+
+def test_verify_bvh_self_check(ctx):
+    """pt_verify_bvh, the build's root-box self-check (BVHBuilder.cu:258-265,
+    exact): passes for computed bounds, single and batched, fails naming the
+    scene when the given enclosing box is larger than the geometry."""
+    tris, infos = ptamd.gen_soup(100000, 31)
+    ctx.upload_scene(tris, infos)
+    assert ctx.verify_bvh() == 0
+    ctx.build_bvh(timed=False)
+    assert ctx.verify_bvh() == 0
+    counts = [3000, 1, 0, 70000, 2]
+    bt, bi, offs = ptamd.gen_batch(len(counts), 32, counts=counts)
+    ctx.upload_scenes(bt, offs, bi)
+    assert ctx.verify_bvh() == 0
+    bmin, bmax = ptamd.scene_bounds(tris)
+    ctx.upload_scene(tris, infos, bmin - 0.5, bmax + 0.5)
+    with pytest.raises(ptamd.PtError) as e:
+        ctx.verify_bvh()
+    assert "scene 0" in str(e.value)

@@ -116,6 +116,52 @@ def test_render_invariant_under_tuning(ctx, scene):
             ptamd.tune_set(k, v)
 
 
+def test_frame_structures_match_oracle(ctx, scene):
+    """Every frame structure gives the oracle's accumulator and segment
+    count: path-persistent frames (render.paths 1) at several shade batch
+    sizes, and bounce levels (render.paths 0) split into 1..8 chains, each on
+    its own stream, forked and joined inside the frame's graph or launched
+    eagerly."""
+    tris, infos, st = scene
+    ctx.upload_scene(tris, infos, settings=st)
+    w, h, n = 128, 64, 8  # 65,536 paths: up to 4 chains of 16,384
+    seeds = np.arange(200, 200 + n, dtype=np.uint64)
+    dirs = np.stack([ptamd.camera_corners(EYE, TGT, (0, 1, 0), 60.0, w / h, *ptamd.pass_jitter(5, p, w, h))
+                     for p in range(n)])
+    b = orc.build(tris, *ptamd.scene_bounds(tris))
+    o = np.zeros((h, w, 3))
+    segs = 0
+    for p in range(n):
+        _, stats = orc.render_pass_stats(tris, b, infos, st, w, h, dirs[p], EYE, p, int(seeds[p]), o)
+        segs += stats["segments"]
+    assert o.sum() > 0
+    saved = {k: ptamd.tune_get(k) for k in ("render.chains", "render.graphs", "render.paths", "render.shade_min")}
+    structures = [("render.paths", 1, "render.shade_min", m) for m in (1, 7, 32, 64)] + \
+                 [("render.paths", 0, "render.chains", c) for c in (1, 2, 3, 4, 8)]
+    try:
+        for graphs in (1, 0):
+            ptamd.tune_set("render.graphs", graphs)
+            for k1, v1, k2, v2 in structures:
+                say(f"[frame structure] {k1}={v1} {k2}={v2}, graphs={graphs}, {w}x{h}x{n} paths")
+                ptamd.tune_set(k1, v1)
+                ptamd.tune_set(k2, v2)
+                chains = (k2, v2)
+                ctx.reset_frames()
+                ctx.render_passes(w, h, seeds, dirs)
+                acc, spp = ctx.read_accumulator(w, h)
+                assert spp == n and np.array_equal(acc, o), (chains, graphs)
+                assert ctx.last_frame_rays() == segs, (chains, graphs)
+            # Two frames in a row reuse the captured graph (or the streams).
+            ctx.reset_frames()
+            ctx.render_passes(w, h, seeds[:4], dirs[:4])
+            ctx.render_passes(w, h, seeds[4:], dirs[4:])
+            acc, spp = ctx.read_accumulator(w, h)
+            assert spp == n and np.array_equal(acc, o), ("two frames", graphs)
+    finally:
+        for k, v in saved.items():
+            ptamd.tune_set(k, v)
+
+
 def test_batched_scene_frame(ctx):
     """One frame whose passes trace different scenes of a batch (including a
     1-triangle and an empty scene) equals the oracle pass by pass."""
@@ -128,11 +174,16 @@ def test_batched_scene_frame(ctx):
     seeds = np.arange(50, 50 + npass, dtype=np.uint64)
     dirs = np.stack([pass_dirs(p) for p in range(npass)])
     scenes = np.arange(npass, dtype=np.uint32)[::-1].copy()
-    say(f"[batched frame] {npass} scenes")
-    ctx.reset_frames()
-    ctx.render_passes(W, H, seeds, dirs, scenes)
-    acc, spp = ctx.read_accumulator(W, H)
-    rays = ctx.last_frame_rays()
+    results = []
+    for paths in (1, 0):
+        ptamd.tune_set("render.paths", paths)
+        say(f"[batched frame] {npass} scenes, paths={paths}")
+        ctx.reset_frames()
+        ctx.render_passes(W, H, seeds, dirs, scenes)
+        results.append((*ctx.read_accumulator(W, H), ctx.last_frame_rays()))
+    ptamd.tune_set("render.paths", 1)
+    assert np.array_equal(results[0][0], results[1][0]) and results[0][2] == results[1][2]
+    acc, spp, rays = results[0]
     assert spp == npass and rays >= W * H * npass
     o = np.zeros((H, W, 3))
     segs = 0
@@ -157,8 +208,8 @@ def lit_scene():
     return tris, infos, lights
 
 
-@pytest.mark.parametrize("graphs", [1, 0])
-def test_render_with_lights_matches_oracle(ctx, graphs):
+@pytest.mark.parametrize("graphs,paths", [(1, 1), (0, 0), (1, 0)])
+def test_render_with_lights_matches_oracle(ctx, graphs, paths):
     """Directional lights: Blinn-Phong at primary hits and the light term at
     bounce hits, each behind a shadow ray, bit-identical to the oracle."""
     tris, infos, lights = lit_scene()
@@ -168,13 +219,15 @@ def test_render_with_lights_matches_oracle(ctx, graphs):
     seeds = np.arange(300, 300 + n, dtype=np.uint64)
     dirs = np.stack([pass_dirs(p) for p in range(n)])
     ptamd.tune_set("render.graphs", graphs)
+    ptamd.tune_set("render.paths", paths)
     try:
-        say(f"[lights] {n}-pass frame, graphs={graphs}")
+        say(f"[lights] {n}-pass frame, graphs={graphs}, paths={paths}")
         ctx.reset_frames()
         ctx.render_passes(W, H, seeds, dirs)
         acc, spp = ctx.read_accumulator(W, H)
     finally:
         ptamd.tune_set("render.graphs", 1)
+        ptamd.tune_set("render.paths", 1)
     b = orc.build(tris, *ptamd.scene_bounds(tris))
     o = np.zeros((H, W, 3))
     for p in range(n):
@@ -299,8 +352,8 @@ def random_detail(n, seed):
     return det, texs
 
 
-@pytest.mark.parametrize("graphs", [1, 0])
-def test_render_with_detail_matches_oracle(ctx, scene, graphs):
+@pytest.mark.parametrize("graphs,paths", [(1, 1), (0, 0), (1, 0)])
+def test_render_with_detail_matches_oracle(ctx, scene, graphs, paths):
     """Textures, vertex colours and normal maps (extractInfo / unpackNormal):
     accumulators bit-identical to the oracle, with directional lights."""
     tris, infos, st = scene
@@ -310,15 +363,17 @@ def test_render_with_detail_matches_oracle(ctx, scene, graphs):
     ctx.upload_scene(tris, infos, settings=st, lights=lights)
     ctx.upload_detail(det, texs)
     ptamd.tune_set("render.graphs", graphs)
+    ptamd.tune_set("render.paths", paths)
     try:
         seeds = np.arange(40, 43, dtype=np.uint64)
         dirs = np.stack([pass_dirs(p) for p in range(3)])
         ctx.reset_frames()
-        say(f"[detail] 3 passes, graphs={graphs}")
+        say(f"[detail] 3 passes, graphs={graphs}, paths={paths}")
         ctx.render_passes(W, H, seeds, dirs)
         acc, spp = ctx.read_accumulator(W, H)
     finally:
         ptamd.tune_set("render.graphs", 1)
+        ptamd.tune_set("render.paths", 1)
     b = orc.build(tris, *ptamd.scene_bounds(tris))
     o = np.zeros((H, W, 3))
     for p in range(3):
