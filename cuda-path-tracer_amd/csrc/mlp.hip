@@ -151,21 +151,6 @@ __global__ __launch_bounds__(kThreads) void k_mlp_pack(const float* __restrict__
     }
 }
 
-// Input feature `idx` (0..95) of one sample from its 12 raw floats.
-__device__ __forceinline__ float feature(int idx, const float* p01, const float* raw) {
-    if (idx < 72) {
-        const int a = idx / 24, k = (idx % 24) / 2;
-        // sin/cos(2^k pi p) = sin/cos(2 pi frac(2^(k-1) p)); the scaling is exact.
-        const float rev = ldexpf(p01[a], k - 1);
-        const float fr = rev - floorf(rev);
-        float s, c;
-        sincos2pi(fr, s, c);
-        return (idx & 1) ? c : s;
-    }
-    if (idx < 81) return raw[3 + idx - 72];
-    return 0.0f;
-}
-
 __device__ __forceinline__ half4 to_half4(floatx4 v) {
     half4 h;
     h.x = static_cast<_Float16>(v.x);
@@ -178,6 +163,24 @@ __device__ __forceinline__ half4 to_half4(floatx4 v) {
 struct SceneBox {
     float mn[3], inv_ext[3];
 };
+
+// Input features idx and idx + 1 (idx even, 0..94) of one sample from its
+// 12 raw floats: below 72 the (sin, cos) pair of 2^k pi p for one axis and
+// frequency -- sin/cos(2^k pi p) = sin/cos(2 pi frac(2^(k-1) p)), the
+// hardware sine and cosine taking revolutions -- then the normal, incoming
+// direction and albedo, then zero padding.
+__device__ __forceinline__ void feature_pair(int idx, const float* p01, const float* raw, float& f0, float& f1) {
+    if (idx < 72) {
+        const int a = idx / 24, k = (idx % 24) / 2;
+        const float rev = ldexpf(p01[a], k - 1);
+        const float fr = rev - floorf(rev);
+        f0 = __builtin_amdgcn_sinf(fr);
+        f1 = __builtin_amdgcn_cosf(fr);
+    } else {
+        f0 = idx < 81 ? raw[3 + idx - 72] : 0.0f;
+        f1 = idx + 1 < 81 ? raw[3 + idx + 1 - 72] : 0.0f;
+    }
+}
 
 // Lane's view of one 16-sample block: the input fragments X[f] (k-block f).
 __device__ __forceinline__ void encode_block(const float* __restrict__ raw, uint32_t n, uint32_t s,
@@ -193,12 +196,10 @@ __device__ __forceinline__ void encode_block(const float* __restrict__ raw, uint
     for (int a = 0; a < 3; ++a) p01[a] = (rw[a] - box.mn[a]) * box.inv_ext[a];
 #pragma unroll
     for (int f = 0; f < fb(0); ++f) {
-        floatx4 v;
-        v.x = feature(16 * f + 4 * (lane / 16) + 0, p01, rw);
-        v.y = feature(16 * f + 4 * (lane / 16) + 1, p01, rw);
-        v.z = feature(16 * f + 4 * (lane / 16) + 2, p01, rw);
-        v.w = feature(16 * f + 4 * (lane / 16) + 3, p01, rw);
-        X[f] = to_half4(v);
+        float a, b, c, d;
+        feature_pair(16 * f + 4 * (lane / 16) + 0, p01, rw, a, b);
+        feature_pair(16 * f + 4 * (lane / 16) + 2, p01, rw, c, d);
+        X[f] = to_half4(floatx4{a, b, c, d});
     }
 }
 
@@ -233,20 +234,30 @@ __device__ __forceinline__ floatx4 relu4(floatx4 v) {
 __device__ __forceinline__ floatx4 forward_block(const uint2* __restrict__ frags,
                                                  const float* __restrict__ master, const half4* X,
                                                  half4 (*H)[4]) {
-    for (int r = 0; r < 4; ++r) {
-        floatx4 acc = bias4(master + b_off(0), r, kHid);
+    // The four row blocks' accumulators advance together (independent MFMA
+    // chains); each still sums its k-blocks in order.
+    {
+        floatx4 acc[4];
 #pragma unroll
-        for (int f = 0; f < fb(0); ++f) acc = mfma(lds_frag(frags, fwd_base(0) + r * fb(0) + f), X[f], acc);
-        H[1][r] = to_half4(relu4(acc));
+        for (int r = 0; r < 4; ++r) acc[r] = bias4(master + b_off(0), r, kHid);
+#pragma unroll
+        for (int f = 0; f < fb(0); ++f)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = mfma(lds_frag(frags, fwd_base(0) + r * fb(0) + f), X[f], acc[r]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) H[1][r] = to_half4(relu4(acc[r]));
     }
 #pragma unroll
     for (int l = 1; l < 4; ++l) {
-        for (int r = 0; r < 4; ++r) {
-            floatx4 acc = bias4(master + b_off(l), r, kHid);
+        floatx4 acc[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f) acc = mfma(lds_frag(frags, fwd_base(l) + r * 4 + f), H[l][f], acc);
-            H[l + 1][r] = to_half4(relu4(acc));
-        }
+        for (int r = 0; r < 4; ++r) acc[r] = bias4(master + b_off(l), r, kHid);
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[r] = mfma(lds_frag(frags, fwd_base(l) + r * 4 + f), H[l][f], acc[r]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) H[l + 1][r] = to_half4(relu4(acc[r]));
     }
     floatx4 z = bias4(master + b_off(4), 0, kOut);
 #pragma unroll
@@ -254,29 +265,43 @@ __device__ __forceinline__ floatx4 forward_block(const uint2* __restrict__ frags
     return z;
 }
 
+// Global -> LDS copy of `count` elements by the workgroup, eight loads in
+// flight per thread before their stores (a load-store loop would wait out
+// one memory round trip per element).
+template <class T>
+__device__ __forceinline__ void stage(T* dst, const T* __restrict__ src, int count) {
+    constexpr int kBatch = 8;
+    const int step = static_cast<int>(blockDim.x);
+    for (int base = static_cast<int>(threadIdx.x); base < count; base += kBatch * step) {
+        T v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int k = base + j * step;
+            if (k < count) v[j] = src[k];
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int k = base + j * step;
+            if (k < count) dst[k] = v[j];
+        }
+    }
+}
+
 __device__ __forceinline__ void stage_frags(uint2* dst, const uint2* __restrict__ src, int nfrags) {
-    for (int k = threadIdx.x; k < nfrags * 64; k += blockDim.x) dst[k] = src[k];
+    stage(dst, src, nfrags * 64);
     __syncthreads();
 }
 
 // Input features kperm(q, g, 0..7) of one sample for the 16x16x32 B operand
-// (lane group g).  Frequencies use the hardware sine and cosine, whose
-// argument is in revolutions: sin/cos(2^k pi p) = sin/cos(2 pi frac(2^(k-1) p)).
+// (lane group g): pairs kperm(q, g, j), +1 for even j.
 __device__ __forceinline__ half8 encode32(int q, int g, const float* p01, const float* raw) {
     half8 x;
 #pragma unroll
     for (int j = 0; j < 8; j += 2) {
-        const int f = kperm(q, g, j);  // even: (sin, cos) of one argument, or inputs past 72
-        if (f < 72) {
-            const int a = f / 24, k = (f % 24) / 2;
-            const float rev = ldexpf(p01[a], k - 1);
-            const float fr = rev - floorf(rev);
-            x[j] = static_cast<_Float16>(__builtin_amdgcn_sinf(fr));
-            x[j + 1] = static_cast<_Float16>(__builtin_amdgcn_cosf(fr));
-        } else {
-            x[j] = static_cast<_Float16>(f < 81 ? raw[3 + f - 72] : 0.0f);
-            x[j + 1] = static_cast<_Float16>(f + 1 < 81 ? raw[3 + f + 1 - 72] : 0.0f);
-        }
+        float f0, f1;
+        feature_pair(kperm(q, g, j), p01, raw, f0, f1);
+        x[j] = static_cast<_Float16>(f0);
+        x[j + 1] = static_cast<_Float16>(f1);
     }
     return x;
 }
@@ -314,7 +339,7 @@ __global__ __launch_bounds__(kThreads) void k_mlp_infer(const uint4* __restrict_
                                                         SceneBox box, float* __restrict__ out) {
     __shared__ uint4 s_frag[kF32Frags * 64];
     __shared__ float s_bias[kLayers][kHid];
-    for (int k = threadIdx.x; k < kF32Frags * 64; k += blockDim.x) s_frag[k] = f32[k];
+    stage(s_frag, f32, kF32Frags * 64);
     for (int k = threadIdx.x; k < kLayers * kHid; k += blockDim.x) {
         const int l = k / kHid, o = k % kHid;
         s_bias[l][o] = o < dim(l + 1) ? master[b_off(l) + o] : 0.0f;
@@ -443,7 +468,7 @@ __host__ __device__ constexpr int del_row(int l) { return kHid * l; }
 constexpr int kWTiles = fwd_base(kLayers);  // 76 weight tiles (16 x 16), 19 per wave
 constexpr int kTilesPerWave = kWTiles / 4;
 static_assert(kWTiles % 4 == 0, "weight tiles split evenly over four waves");
-constexpr uint32_t kMaxTrainBlocks = 128;  // workgroups (gradient partials) of one step
+constexpr uint32_t kMaxTrainBlocks = 256;  // workgroups (gradient partials) of one step
 
 // Deltas are gradients of n * loss (a static loss scale of n keeps them well
 // inside fp16's normal range); they are clamped to fp16's finite range so an
@@ -496,13 +521,26 @@ __global__ __launch_bounds__(kThreads) void k_mlp_train(
     __shared__ _Float16 s_act[kActRows][kRowLd];
     __shared__ _Float16 s_del[kDelRows][kRowLd];
     if (blockIdx.x == 0 && threadIdx.x == 0 && advance) step[0] += advance;
-    for (int k = threadIdx.x; k < kBwdFrags * 64; k += blockDim.x) s_bwd[k] = bwd[k];
+    stage(s_bwd, bwd, kBwdFrags * 64);
     stage_frags(s_fwd, fwd, kFwdFrags);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x / 64;
     floatx4 acc[kTilesPerWave];
 #pragma unroll
     for (int t = 0; t < kTilesPerWave; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    // This wave's dW tiles: LDS rows of their layer inputs and deltas
+    // (wave-uniform, computed once).
+    uint32_t arow[kTilesPerWave], drow[kTilesPerWave];
+#pragma unroll
+    for (int t = 0; t < kTilesPerWave; ++t) {
+        const int task = wave + 4 * t;
+        int l = 0;
+        while (l + 1 < kLayers && task >= fwd_base(l + 1)) ++l;
+        const int k = task - fwd_base(l);
+        const int ri = k % fb(l), co = k / fb(l);
+        arow[t] = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(act_row(l) + 16 * ri));
+        drow[t] = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>((l == 4 ? kHid * 4 : del_row(l)) + 16 * co));
+    }
     // Bias sums: thread t owns delta row t (layers 0..3) and, for t < 3,
     // output bias t (delta4 row t).
     float bsum = 0.0f, bsum4 = 0.0f;
@@ -548,15 +586,18 @@ __global__ __launch_bounds__(kThreads) void k_mlp_train(
             lds_put(s_del, del_row(3) + 16 * r, col0, Dn[r]);
         }
         for (int l = 3; l >= 1; --l) {
-            half4 Dp[4];
-            for (int r = 0; r < 4; ++r) {
-                floatx4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+            floatx4 a[4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) a = mfma(lds_frag(s_bwd, bwd_base(l) + r * 4 + f), Dn[f], a);
-                Dp[r] = mask4(a, H[l][r]);
-                lds_put(s_del, del_row(l - 1) + 16 * r, col0, Dp[r]);
+            for (int r = 0; r < 4; ++r) a[r] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = mfma(lds_frag(s_bwd, bwd_base(l) + r * 4 + f), Dn[f], a[r]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                Dn[r] = mask4(a[r], H[l][r]);
+                lds_put(s_del, del_row(l - 1) + 16 * r, col0, Dn[r]);
             }
-            for (int r = 0; r < 4; ++r) Dn[r] = Dp[r];
         }
         __syncthreads();
         // dW tiles of this wave over the round's 64 samples (K = 4 x 16).
@@ -564,13 +605,8 @@ __global__ __launch_bounds__(kThreads) void k_mlp_train(
         for (int kb = 0; kb < 4; ++kb) {
 #pragma unroll
             for (int t = 0; t < kTilesPerWave; ++t) {
-                const int task = wave + 4 * t;
-                int l = 0;
-                while (l + 1 < kLayers && task >= fwd_base(l + 1)) ++l;
-                const int k = task - fwd_base(l);
-                const int ri = k % fb(l), co = k / fb(l);
-                const half4 a = lds_get(s_act, act_row(l) + 16 * ri, 16 * kb);
-                const half4 b = lds_get(s_del, (l == 4 ? kHid * 4 : del_row(l)) + 16 * co, 16 * kb);
+                const half4 a = lds_get(s_act, static_cast<int>(arow[t]), 16 * kb);
+                const half4 b = lds_get(s_del, static_cast<int>(drow[t]), 16 * kb);
                 acc[t] = mfma(a, b, acc[t]);
             }
         }
@@ -617,16 +653,33 @@ __global__ __launch_bounds__(kThreads) void k_mlp_train(
 // fragments: forward (16x16x16, layer input as k), backward (16x16x16, layer
 // output as k) and inference (16x16x32, permuted k).  grad_out: only the
 // gradient, no update.
+// One workgroup per 64 parameters: the four waves sum the partials
+// s = w, w + 4, w + 8, ... of their parameter (coalesced rows), then the
+// first wave adds the four sums in wave order -- a fixed order, so the
+// gradient is the same bits on every call.
+constexpr int kAdamParams = 64;
+
 __global__ __launch_bounds__(kThreads) void k_mlp_adam_pack(float* __restrict__ master, float* __restrict__ m,
                                                             float* __restrict__ v, const float* __restrict__ part,
                                                             uint32_t nparts, float inv_n, float lr, float b1,
                                                             float b2, float eps, const uint32_t* __restrict__ step,
                                                             float* __restrict__ grad_out, _Float16* __restrict__ fwd,
                                                             _Float16* __restrict__ bwd, _Float16* __restrict__ f32) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kParams) return;
-    float g = 0.0f;
-    for (uint32_t s = 0; s < nparts; ++s) g += part[static_cast<size_t>(s) * kParams + p];
+    __shared__ float s_sum[kThreads / kAdamParams][kAdamParams];
+    const int lane = threadIdx.x % kAdamParams, wave = threadIdx.x / kAdamParams;
+    const int p = blockIdx.x * kAdamParams + lane;
+    constexpr int kWaves = kThreads / kAdamParams;
+    float acc = 0.0f;
+    if (p < kParams) {
+        uint32_t s = static_cast<uint32_t>(wave);
+#pragma unroll 8
+        for (; s < nparts; s += kWaves) acc += part[static_cast<size_t>(s) * kParams + p];
+    }
+    s_sum[wave][lane] = acc;
+    __syncthreads();
+    if (wave != 0 || p >= kParams) return;
+    float g = ((s_sum[0][lane] + s_sum[1][lane]) + s_sum[2][lane]) + s_sum[3][lane];
+    static_assert(kWaves == 4, "four partial sums per parameter");
     g *= inv_n;
     if (grad_out) {
         grad_out[p] = g;
@@ -758,7 +811,7 @@ void RadianceCache::launch_train(const float* d_raw, const float* d_target, uint
     hipLaunchKernelGGL(k_mlp_train, dim3(nb), dim3(kThreads), 0, st, fwd.as<uint2>(), bwd.as<uint2>(),
                        master.as<float>(), d_raw, d_target, n, b, part.as<float>(),
                        d_loss ? d_loss : loss.as<float>(), dstep.as<uint32_t>(), d_grad_out ? 0u : 1u);
-    hipLaunchKernelGGL(k_mlp_adam_pack, dim3(blocks_for(kParams, kThreads)), dim3(kThreads), 0, st,
+    hipLaunchKernelGGL(k_mlp_adam_pack, dim3(blocks_for(kParams, kAdamParams)), dim3(kThreads), 0, st,
                        master.as<float>(), m.as<float>(), v.as<float>(), part.as<float>(), nb,
                        1.0f / static_cast<float>(n), lr, beta1, beta2, eps, dstep.as<uint32_t>(), d_grad_out,
                        fwd.as<_Float16>(), bwd.as<_Float16>(), f32.as<_Float16>());
