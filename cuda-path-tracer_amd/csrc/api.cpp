@@ -683,6 +683,7 @@ namespace {
 uint32_t* tune_slot(const char* name) {
     if (!name) return nullptr;
     const std::string k(name);
+    if (k == "build.staged") return &pt::build_tuning().staged;
     if (k == "trace.blocks") return &pt::trace_tuning().blocks;
     if (k == "trace.refill") return &pt::trace_tuning().refill;
     if (k == "trace.lds_stack") return &pt::trace_tuning().lds;
@@ -698,6 +699,7 @@ uint32_t* tune_slot(const char* name) {
 bool tune_valid(const uint32_t* slot, uint32_t v) {
     if (slot == &pt::render_tuning().graphs) return true;
     if (slot == &pt::trace_tuning().filter) return v <= 1;
+    if (slot == &pt::build_tuning().staged) return v <= 1;
     if (slot == &pt::render_tuning().chains) return v >= 1 && v <= 8;
     if (slot == &pt::render_tuning().paths) return v <= 1;
     if (slot == &pt::render_tuning().shade_min) return v >= 1 && v <= 64;

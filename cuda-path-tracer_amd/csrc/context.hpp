@@ -109,6 +109,7 @@ struct DeviceBvh {
     DevBuf nodes, tris, root_box;
     DevBuf tile_hist, digit_total;  // radix sort scratch
     DevBuf span_list, span_top, span_count;  // refit: per-level node lists
+    DevBuf slots;                            // refit: each listed node's position in its list
 };
 
 // One or more scenes: scene s owns triangles [off[s], off[s+1]) and its own
@@ -148,6 +149,12 @@ void build_lbvh(DeviceScene& scene, hipStream_t stream);
 void dump_bvh(DeviceScene& scene, uint32_t scene_index, hipStream_t stream, uint32_t* morton,
               uint32_t* perm, uint32_t* children, uint32_t* parents_leaf,
               uint32_t* parents_internal, float* bboxes);
+
+// LBVH build options (tunable).
+struct BuildTuning {
+    uint32_t staged = 1;  // refit the upper levels with their node lists staged in LDS (0: in global memory)
+};
+BuildTuning& build_tuning();
 
 // Launch shape of the persistent closest-hit kernel (tunable).
 struct TraceTuning {
@@ -247,7 +254,7 @@ struct RenderTuning {
     uint32_t graphs = 1;  // replay each frame's kernels as one captured hipGraph
     uint32_t chains = 4;  // level chains of a plain frame (1..8), one stream each
     uint32_t paths = 1;   // plain frames path-persistent (one launch, no bounce levels)
-    uint32_t shade_min = 32;  // path-persistent frames: lanes that shade together (1..64)
+    uint32_t shade_min = 8;   // path-persistent frames: lanes that shade together (1..64)
 };
 RenderTuning& render_tuning();
 

@@ -18,7 +18,8 @@
 //                   1024-leaf block (LDS counters/boxes, workgroup scope)
 //   k_refit_mid     nodes inside each 65536-leaf block, one workgroup each
 //   k_refit_top     the remaining nodes, one workgroup
-// Boxes are written straight into the parents' traversal node records.
+// Each node's traversal record is written whole by the thread that
+// completes the node's box.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,10 +48,16 @@ constexpr int kDigitsPerThread = kRadix / kSortThreads;        // 4
 // node's refit counter.
 constexpr int kRefitBlock = 1024;
 constexpr uint32_t kMidBlock = 65536;
-// Level-1 nodes of one 65536-leaf block: each crosses one of its 63 inner
-// 1024-leaf boundaries, and a boundary is crossed only by the <= 64 ancestors
-// of its two leaves' lowest common ancestor.
-constexpr uint32_t kMidCap = 63 * 64;
+// Upper-level node lists.  A level-1 node goes to the list of the 1024-leaf
+// block holding the start of its leaf range, a level-2 node to that of the
+// 65536-leaf block; either way its range crosses that block's end, and a
+// boundary is crossed only by the <= 64 ancestors of its two leaves' lowest
+// common ancestor, so a list never holds more than kListCap nodes.  (Lists
+// per small block keep the appends' atomics spread over many counters.)
+// One 65536-leaf block's 64 level-1 lists together hold at most kMidCap.
+constexpr uint32_t kListCap = 64;
+constexpr uint32_t kMidCap = 64 * kListCap;
+constexpr uint32_t kMaxMidBlocks = (1u << 30) / kMidBlock;  // builds hold < 2^30 triangles
 constexpr uint32_t kLevelShift = 30;
 constexpr uint32_t kCountMask = (1u << kLevelShift) - 1u;
 
@@ -317,13 +324,74 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
     return __clz(static_cast<int>(ki ^ kj));
 }
 
+// Wide nodes: the doubling search for a node's range, the binary search for
+// its end and the search for its split each take one dependent key load per
+// step on the owning lane, so the few nodes with huge ranges (the root's
+// reaches every key of its scene) would keep their wave, and the kernel,
+// waiting for ~3 log2(range) loads in a row.  A node whose range exceeds
+// kTreeLane keys is instead searched by its whole wave, 63 probes per step.
+// The predicates are monotone in the distance from i (the keys are sorted,
+// and among equal keys the index-augmented prefix shrinks with the
+// distance), so the wave finds exactly the value the lane-serial searches
+// of the reference find: the last distance at which the predicate holds.
+constexpr int64_t kTreeLane = 64;
+
+// Largest x in [lo, hi) with pred(x), for a monotone pred known to hold at lo
+// (or taken to) and to fail at hi; all 64 lanes of the wave call it with the
+// same arguments.
+template <class Pred>
+__device__ __forceinline__ int64_t wave_search(int64_t lo, int64_t hi, Pred pred) {
+    const int lane = threadIdx.x & 63;
+    while (hi - lo > 1) {
+        const int64_t step = (hi - lo + 63) / 64;
+        const int64_t x = lo + (lane + 1) * step;
+        const uint64_t m = __ballot(lane < 63 && x < hi && pred(x));
+        if (m != 0ull) lo += (64 - __clzll(static_cast<long long>(m))) * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
+// One wide node's search parameters, broadcast from its lane.
+struct WideNode {
+    const uint32_t* sk;
+    int64_t n, i, d;
+    uint32_t ki;
+    int lane;
+    __device__ __forceinline__ int bcast(int v) const { return __builtin_amdgcn_readlane(v, lane); }
+    __device__ __forceinline__ int64_t bcast64(int64_t v) const {
+        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
+        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
+        return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+    }
+    // delta(i, i + x d) of the wide node.
+    __device__ __forceinline__ int delta(int64_t x) const { return pt::delta(sk, n, i, ki, i + x * d); }
+};
+
+__device__ __forceinline__ WideNode broadcast(int b, const uint32_t* keys, uint32_t base, int64_t n, int64_t i,
+                                              uint32_t ki, int64_t d) {
+    WideNode w;
+    w.lane = b;
+    const uint32_t bb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), b));
+    w.sk = keys + bb;
+    w.n = w.bcast64(n);
+    w.i = w.bcast64(i);
+    w.d = w.bcast64(d);
+    w.ki = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ki), b));
+    return w;
+}
+
 // radix_tree_construction_kernel (BVHBuilder.cu:135-196), one thread per
 // internal node, on its scene's segment of sorted keys with local indices
 // (the index augmentation of delta uses them, as in a single-scene build).
 // Also initialises the node's refit counter with its refit level (from its
-// global leaf range) and appends level-1 nodes to their 65536-block's list
-// (span_list[block * kMidCap ...], count in span_count[1 + block]) and level-2
-// nodes to the top list (span_top, count in span_count[0]).
+// global leaf range) and appends level-1 nodes to their 1024-leaf block's
+// list (span_list[block * kListCap ...], count in span_count[block]) and
+// level-2 nodes to their 65536-leaf block's (span_top[...], count in
+// span_count[n_lists1 + block]), with each listed node's list and position
+// in slots[node].  The node's traversal record
+// is written whole by the refit, once its box is known: a 16-byte store into
+// each 64-byte record here would cost a partial-line write per node.
 __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__ keys, uint32_t n,
                                                     const uint32_t* __restrict__ off,
                                                     const uint32_t* __restrict__ scene_of,
@@ -335,32 +403,76 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ span_list,
                                                     uint32_t* __restrict__ span_top,
                                                     uint32_t* __restrict__ span_count,
+                                                    uint32_t n_lists1,
+                                                    uint32_t* __restrict__ slots,
                                                     uint32_t* __restrict__ status) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const uint32_t sc = scene_of[g];
-    const uint32_t base = off[sc];
-    const int64_t N = off[sc + 1] - base;
-    const int64_t i = g - base;
-    if (i >= N - 1) return;  // unused slot of the scene (or a 1-triangle scene)
+    const int lane = threadIdx.x & 63;
+    // Every lane stays in the wave through the wave-wide searches below;
+    // `valid` marks the lanes that own an internal node.
+    bool valid = g < n;
+    uint32_t base = 0;
+    int64_t N = 0, i = 0;
+    if (valid) {
+        const uint32_t sc = scene_of[g];
+        base = off[sc];
+        N = off[sc + 1] - base;
+        i = g - base;
+        valid = i < N - 1;  // else an unused slot of the scene (or a 1-triangle scene)
+    }
     const uint32_t* sk = keys + base;
-    const uint32_t ki = sk[i];
-    const int dr = delta(sk, N, i, ki, i + 1) - delta(sk, N, i, ki, i - 1);
-    const int64_t d = (dr > 0) - (dr < 0);
-    const int dmin = delta(sk, N, i, ki, i - d);
-    int64_t lmax = 2;
-    while (delta(sk, N, i, ki, i + lmax * d) > dmin) lmax *= 2;
+    uint32_t ki = 0;
+    int64_t d = 0, lmax = 2;
+    int dmin = 0;
+    bool wide = false;
+    if (valid) {
+        ki = sk[i];
+        const int dr = delta(sk, N, i, ki, i + 1) - delta(sk, N, i, ki, i - 1);
+        d = (dr > 0) - (dr < 0);
+        dmin = delta(sk, N, i, ki, i - d);
+        while (delta(sk, N, i, ki, i + lmax * d) > dmin) {
+            if (lmax == kTreeLane) {  // range beyond kTreeLane: searched by the whole wave
+                wide = true;
+                break;
+            }
+            lmax *= 2;
+        }
+    }
     int64_t l = 0;
-    while (lmax > 1) {
-        lmax /= 2;
-        if (delta(sk, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
+    for (uint64_t wm = __ballot(wide); wm != 0ull; wm &= wm - 1ull) {
+        const WideNode wn = broadcast(static_cast<int>(__builtin_ctzll(wm)), keys, base, N, i, ki, d);
+        // The doubling search continued from kTreeLane: one probe per lane.
+        const bool up = lane <= 40 && wn.delta(static_cast<int64_t>(kTreeLane) << (lane + 1)) > wn.bcast(dmin);
+        const int64_t wmax = static_cast<int64_t>(kTreeLane) << (__builtin_ctzll(~__ballot(up)) + 1);
+        const int lim = wn.bcast(dmin);
+        const int64_t wl = wave_search(wmax / 2, wmax, [&](int64_t x) { return wn.delta(x) > lim; });
+        if (lane == wn.lane) {
+            lmax = wmax;
+            l = wl;
+        }
+    }
+    if (valid && !wide) {
+        while (lmax > 1) {
+            lmax /= 2;
+            if (delta(sk, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
+        }
     }
     const int64_t j = i + l * d;
-    const int dnode = delta(sk, N, i, ki, j);
+    const int dnode = valid ? delta(sk, N, i, ki, j) : 0;
     int64_t s = 0;
-    while (l > 1) {
-        l = (l + 1) / 2;
-        if (delta(sk, N, i, ki, i + (s + l) * d) > dnode) s += l;
+    for (uint64_t wm = __ballot(wide); wm != 0ull; wm &= wm - 1ull) {
+        const WideNode wn = broadcast(static_cast<int>(__builtin_ctzll(wm)), keys, base, N, i, ki, d);
+        const int lim = wn.bcast(dnode);
+        const int64_t ws = wave_search(0, wn.bcast64(l), [&](int64_t x) { return wn.delta(x) > lim; });
+        if (lane == wn.lane) s = ws;
+    }
+    if (!valid) return;
+    if (!wide) {
+        int64_t t = l;
+        while (t > 1) {
+            t = (t + 1) / 2;
+            if (delta(sk, N, i, ki, i + (s + t) * d) > dnode) s += t;
+        }
     }
     s = i + s * d + (d < 0 ? d : 0);
     uint32_t type = 0;
@@ -376,33 +488,22 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
     } else {
         parents_leaf[base + s + 1] = g;
     }
-    const uint32_t w = ((base + static_cast<uint32_t>(s)) << 2) | type;
-    children[g] = w;
-    nodes[4 * static_cast<size_t>(g) + 3] = make_float4(__uint_as_float(w), 0.0f, 0.0f, 0.0f);
+    children[g] = ((base + static_cast<uint32_t>(s)) << 2) | type;
     const int64_t lo = base + (i < j ? i : j), hi = base + (i > j ? i : j);
     uint32_t level = 0;
     if ((lo / kRefitBlock) != (hi / kRefitBlock)) level = ((lo / kMidBlock) != (hi / kMidBlock)) ? 2u : 1u;
     counters[g] = level << kLevelShift;
-    // List appends, aggregated per wave: one atomic per distinct 65536-block
-    // among the wave's level-1 nodes (neighbouring nodes mostly share it).
-    const uint32_t blk = static_cast<uint32_t>(lo / kMidBlock);
-    const uint64_t lt = (1ull << (threadIdx.x & 63u)) - 1ull;
-    uint64_t pending = __ballot(level == 1);
-    while (pending) {
-        const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(pending));
-        const uint32_t b = __shfl(blk, static_cast<int>(lead), 64);
-        const uint64_t same = pending & __ballot(level == 1 && blk == b);
-        uint32_t first = 0;
-        if ((threadIdx.x & 63u) == lead) first = atomicAdd(span_count + 1 + b, static_cast<uint32_t>(__popcll(same)));
-        first = __shfl(first, static_cast<int>(lead), 64);
-        if ((same >> (threadIdx.x & 63u)) & 1ull) {
-            const uint32_t slot = first + static_cast<uint32_t>(__popcll(same & lt));
-            if (slot < kMidCap) span_list[static_cast<size_t>(b) * kMidCap + slot] = g;
-            else raise_status(status, PT_STATUS_REFIT_OVERFLOW);  // bound argued above
+    if (level != 0) {
+        const uint32_t blk = static_cast<uint32_t>(level == 1 ? lo / kRefitBlock : lo / kMidBlock);
+        const uint32_t slot = atomicAdd(span_count + (level == 1 ? 0u : n_lists1) + blk, 1u);
+        if (slot < kListCap) {
+            (level == 1 ? span_list : span_top)[static_cast<size_t>(blk) * kListCap + slot] = g;
+            // The list within its 65536-leaf block (level 1) or overall (level 2), and the place in it.
+            slots[g] = ((level == 1 ? blk % (kMidBlock / kRefitBlock) : blk) << 6) | slot;
+        } else {
+            raise_status(status, PT_STATUS_REFIT_OVERFLOW);  // bound argued above
         }
-        pending &= ~same;
     }
-    if (level == 2) span_top[atomicAdd(span_count, 1u)] = g;
     if (i == 0) parents_internal[g] = kInvalid;  // BVHBuilder.cu:224
 }
 
@@ -435,6 +536,16 @@ __device__ __forceinline__ Box unite(const Box& a, const Box& b) {
     return o;
 }
 
+// Node p's whole traversal record: both child boxes and the child word, in
+// one 64-byte line.
+__device__ __forceinline__ void store_record(float4* nodes, uint32_t p, const Box& lb, const Box& rb, uint32_t w) {
+    float4* rec = nodes + 4 * static_cast<size_t>(p);
+    rec[0] = make_float4(lb.mnx, lb.mxx, lb.mny, lb.mxy);
+    rec[1] = make_float4(lb.mnz, lb.mxz, rb.mnx, rb.mxx);
+    rec[2] = make_float4(rb.mny, rb.mxy, rb.mnz, rb.mxz);
+    rec[3] = make_float4(__uint_as_float(w), 0.0f, 0.0f, 0.0f);
+}
+
 __device__ __forceinline__ void store_root(float* root_box, const Box& b) {
     root_box[0] = b.mnx; root_box[1] = b.mny; root_box[2] = b.mnz;
     root_box[3] = b.mxx; root_box[4] = b.mxy; root_box[5] = b.mxz;
@@ -459,8 +570,10 @@ struct LdsBoxes {
 // reach a node stops, the second unites the two child boxes.  Every node
 // below the spanning ones has its whole leaf range -- hence both children --
 // inside the block, so counters and boxes live in LDS and the hand-off needs
-// only workgroup-scope ordering.  Each finished box is also stored into the
-// parent's traversal record.
+// only workgroup-scope ordering.  The thread completing a node writes the
+// node's whole traversal record (both child boxes, leaves' included, and the
+// child word); a box whose parent is finished by a later phase is also
+// stored into that parent's record.
 __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     const pt_triangle* __restrict__ in_tris, const uint32_t* __restrict__ perm,
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ scene_of,
@@ -516,13 +629,16 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
         const Box rb = (w & 2u) ? s_node.get(r - base) : s_leaf.get(r - base);
         b = unite(lb, rb);
         s_node.put(p - base, b);
+        store_record(nodes, p, lb, rb, w);
         const uint32_t pp = parents_internal[p];
         if (pp == kInvalid) {
             store_root(root_box + 6 * static_cast<size_t>(scene_of[p]), b);
             return;
         }
-        store_slot(nodes, pp, (children[pp] >> 2) == p, b);
-        if (counters[pp] >> kLevelShift) return;
+        if (counters[pp] >> kLevelShift) {  // finished by a later phase, from its record
+            store_slot(nodes, pp, (children[pp] >> 2) == p, b);
+            return;
+        }
         p = pp;
     }
 }
@@ -540,8 +656,7 @@ __device__ __forceinline__ void refit_level(const uint32_t* __restrict__ childre
                                             uint32_t* __restrict__ counters,
                                             float4* __restrict__ nodes,
                                             float* __restrict__ root_box,
-                                            const uint32_t* __restrict__ list, uint32_t count,
-                                            uint32_t level) {
+                                            const uint32_t* list, uint32_t count, uint32_t level) {
     for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
         const uint32_t s = list[idx];
         const uint32_t w = children[s];
@@ -560,6 +675,7 @@ __device__ __forceinline__ void refit_level(const uint32_t* __restrict__ childre
                 const Box lb{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y};
                 const Box rb{r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
                 const Box b = unite(lb, rb);
+                nodes[4 * static_cast<size_t>(p) + 3] = make_float4(__uint_as_float(children[p]), 0.0f, 0.0f, 0.0f);
                 const uint32_t pp = parents_internal[p];
                 if (pp == kInvalid) {
                     store_root(root_box + 6 * static_cast<size_t>(scene_of[p]), b);
@@ -573,7 +689,130 @@ __device__ __forceinline__ void refit_level(const uint32_t* __restrict__ childre
     }
 }
 
-// Level-1 nodes: one workgroup per 65536-leaf block.
+// The same refit with the list staged in LDS (lists of up to kStageCap
+// nodes): every listed node's record (the child boxes the earlier phases
+// stored), its parent and its count of same-level children are gathered
+// first, so the climb itself touches only LDS.  Each completed node's
+// record is written whole; a box whose parent belongs to a later phase also
+// goes into that parent's record.
+constexpr uint32_t kStageCap = 1024;
+constexpr uint32_t kUpOutside = 0xffffffffu;  // parent handled by a later phase, or none
+
+struct LevelStage {
+    float box[12][kStageCap];  // left child box, then right, in record order
+    uint32_t node[kStageCap];  // the listed node
+    uint32_t word[kStageCap];  // its child word
+    uint32_t par[kStageCap];   // its parent (kInvalid at a scene root)
+    uint32_t up[kStageCap];    // parent's slot << 1 | right side, or kUpOutside
+    uint32_t wait[kStageCap];  // same-level children not yet complete
+};
+
+__device__ __forceinline__ void refit_level_staged(LevelStage& S, const uint32_t* __restrict__ children,
+                                                   const uint32_t* __restrict__ scene_of,
+                                                   const uint32_t* __restrict__ parents_internal,
+                                                   const uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ slots, const uint32_t* list_start,
+                                                   float4* __restrict__ nodes, float* __restrict__ root_box,
+                                                   const uint32_t* list, uint32_t count, uint32_t level) {
+    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
+        const uint32_t s = list[idx];
+        const float4* rec = nodes + 4 * static_cast<size_t>(s);
+        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+        const float v[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+        for (int k = 0; k < 12; ++k) S.box[k][idx] = v[k];
+        const uint32_t w = children[s];
+        const uint32_t l = w >> 2, r = l + 1;
+        S.wait[idx] = static_cast<uint32_t>((w & 1u) && (counters[l] >> kLevelShift) == level) +
+                      static_cast<uint32_t>((w & 2u) && (counters[r] >> kLevelShift) == level);
+        S.node[idx] = s;
+        S.word[idx] = w;
+        const uint32_t pp = parents_internal[s];
+        S.par[idx] = pp;
+        uint32_t up = kUpOutside;
+        if (pp != kInvalid && (counters[pp] >> kLevelShift) == level) {
+            const uint32_t sl = slots[pp];
+            up = ((list_start[sl >> 6] + (sl & 63u)) << 1) | static_cast<uint32_t>((children[pp] >> 2) != s);
+        }
+        S.up[idx] = up;
+    }
+    __syncthreads();
+    for (uint32_t first = threadIdx.x; first < count; first += blockDim.x) {
+        if (S.wait[first] != 0u) continue;
+        uint32_t idx = first;
+        for (;;) {
+            const Box lb{S.box[0][idx], S.box[1][idx], S.box[2][idx], S.box[3][idx], S.box[4][idx], S.box[5][idx]};
+            const Box rb{S.box[6][idx], S.box[7][idx], S.box[8][idx], S.box[9][idx], S.box[10][idx], S.box[11][idx]};
+            const Box b = unite(lb, rb);
+            const uint32_t s = S.node[idx];
+            store_record(nodes, s, lb, rb, S.word[idx]);
+            const uint32_t pp = S.par[idx];
+            if (pp == kInvalid) {
+                store_root(root_box + 6 * static_cast<size_t>(scene_of[s]), b);
+                break;
+            }
+            const uint32_t up = S.up[idx];
+            if (up == kUpOutside) {
+                store_slot(nodes, pp, (children[pp] >> 2) == s, b);
+                break;
+            }
+            const uint32_t ps = up >> 1;
+            const int o = (up & 1u) ? 6 : 0;
+            S.box[o + 0][ps] = b.mnx; S.box[o + 1][ps] = b.mxx; S.box[o + 2][ps] = b.mny;
+            S.box[o + 3][ps] = b.mxy; S.box[o + 4][ps] = b.mnz; S.box[o + 5][ps] = b.mxz;
+            const uint32_t old = __hip_atomic_fetch_sub(&S.wait[ps], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old != 1u) break;  // the other same-level child completes the parent
+            idx = ps;
+        }
+    }
+}
+
+// The lists of one level, concatenated in LDS-held order: list k starts at
+// start[k] (exclusive prefix of the counts, one thread per list).
+__device__ __forceinline__ uint32_t concat_lists(const uint32_t* __restrict__ counts, uint32_t nlists,
+                                                 const uint32_t* __restrict__ lists, uint32_t* start,
+                                                 uint32_t* out, uint32_t* status) {
+    __shared__ uint32_t s_carry;
+    __shared__ uint32_t s_w[16];
+    if (threadIdx.x == 0) s_carry = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t b0 = 0; b0 < nlists; b0 += blockDim.x) {
+        const uint32_t k = b0 + threadIdx.x;
+        uint32_t c = 0;
+        if (k < nlists) {
+            c = counts[k];
+            if (c > kListCap) {  // counted past a full list (flagged by the append)
+                raise_status(status, PT_STATUS_REFIT_OVERFLOW);
+                c = kListCap;
+            }
+        }
+        uint32_t x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[wave] = x;
+        __syncthreads();
+        uint32_t pre = s_carry;
+        for (int w = 0; w < wave; ++w) pre += s_w[w];
+        if (k < nlists) start[k] = pre + x - c;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) s_carry = pre + x;
+        __syncthreads();
+    }
+    const uint32_t total = s_carry;
+    if (threadIdx.x == 0) start[nlists] = total;
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < nlists * kListCap; e += blockDim.x) {
+        const uint32_t k = e / kListCap, j = e % kListCap;
+        if (j < start[k + 1] - start[k]) out[start[k] + j] = lists[static_cast<size_t>(k) * kListCap + j];
+    }
+    __syncthreads();
+    return total;
+}
+
+// Level-1 nodes: one workgroup per 65536-leaf block (its 64 lists).
 __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__ children,
                                                     const uint32_t* __restrict__ scene_of,
                                                     const uint32_t* __restrict__ parents_internal,
@@ -581,13 +820,26 @@ __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__
                                                     float4* __restrict__ nodes,
                                                     float* __restrict__ root_box,
                                                     const uint32_t* __restrict__ span_list,
-                                                    const uint32_t* __restrict__ span_count) {
-    const uint32_t blk = blockIdx.x;
-    refit_level(children, scene_of, parents_internal, counters, nodes, root_box,
-                span_list + static_cast<size_t>(blk) * kMidCap, min(span_count[1 + blk], kMidCap), 1u);
+                                                    const uint32_t* __restrict__ span_count, uint32_t n_lists1,
+                                                    const uint32_t* __restrict__ slots, uint32_t stage_max,
+                                                    uint32_t* __restrict__ status) {
+    constexpr uint32_t kLists = kMidBlock / kRefitBlock;
+    __shared__ LevelStage S;
+    __shared__ uint32_t s_list[kMidCap];
+    __shared__ uint32_t s_start[kLists + 1];
+    const uint32_t k0 = blockIdx.x * kLists;
+    const uint32_t count = concat_lists(span_count + k0, min(kLists, n_lists1 - k0),
+                                        span_list + static_cast<size_t>(k0) * kListCap, s_start, s_list, status);
+    if (count <= stage_max) {
+        refit_level_staged(S, children, scene_of, parents_internal, counters, slots, s_start, nodes, root_box, s_list,
+                           count, 1u);
+    } else {
+        refit_level(children, scene_of, parents_internal, counters, nodes, root_box, s_list, count, 1u);
+    }
 }
 
-// Level-2 nodes: one workgroup.
+// Level-2 nodes: one workgroup (the lists of all 65536-leaf blocks,
+// concatenated into top_list).
 __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ children,
                                                     const uint32_t* __restrict__ scene_of,
                                                     const uint32_t* __restrict__ parents_internal,
@@ -595,8 +847,19 @@ __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__
                                                     float4* __restrict__ nodes,
                                                     float* __restrict__ root_box,
                                                     const uint32_t* __restrict__ span_top,
-                                                    const uint32_t* __restrict__ span_count) {
-    refit_level(children, scene_of, parents_internal, counters, nodes, root_box, span_top, span_count[0], 2u);
+                                                    const uint32_t* __restrict__ span_count, uint32_t n_lists2,
+                                                    uint32_t* __restrict__ top_list,
+                                                    const uint32_t* __restrict__ slots, uint32_t stage_max,
+                                                    uint32_t* __restrict__ status) {
+    __shared__ LevelStage S;
+    __shared__ uint32_t s_start[kMaxMidBlocks + 1];
+    const uint32_t count = concat_lists(span_count, n_lists2, span_top, s_start, top_list, status);
+    if (count <= stage_max) {
+        refit_level_staged(S, children, scene_of, parents_internal, counters, slots, s_start, nodes, root_box,
+                           top_list, count, 2u);
+    } else {
+        refit_level(children, scene_of, parents_internal, counters, nodes, root_box, top_list, count, 2u);
+    }
 }
 
 // Parity dump of scene [base, base + n): internal node boxes in the
@@ -645,6 +908,11 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a
 
 }  // namespace
 
+BuildTuning& build_tuning() {
+    static BuildTuning t;
+    return t;
+}
+
 void build_lbvh(DeviceScene& s, hipStream_t st) {
     DeviceBvh& b = s.bvh;
     const uint32_t n = s.n;
@@ -669,16 +937,18 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     b.tile_hist.reserve(static_cast<size_t>(num_tiles) * kRadix * 4);
     b.digit_total.reserve(kRadix * 4);
     const uint32_t mid_blocks = div_up(n, kMidBlock);
-    b.span_list.reserve(static_cast<size_t>(mid_blocks) * kMidCap * 4);
-    b.span_top.reserve(n4);
-    b.span_count.reserve((1 + static_cast<size_t>(mid_blocks)) * 4);
+    const uint32_t n_lists1 = div_up(n, kRefitBlock);
+    b.span_list.reserve(static_cast<size_t>(n_lists1) * kListCap * 4);
+    b.span_top.reserve(static_cast<size_t>(mid_blocks) * kListCap * 4 * 2);  // lists, then their concatenation
+    b.span_count.reserve((static_cast<size_t>(n_lists1) + mid_blocks) * 4);
+    b.slots.reserve(n4);
 
     uint32_t* kin = b.keys.as<uint32_t>();
     uint32_t* vin = b.vals.as<uint32_t>();
-    hipLaunchKernelGGL(k_morton, dim3(div_up(std::max<uint64_t>(n, 1 + mid_blocks), 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_morton, dim3(div_up(std::max<uint64_t>(n, n_lists1 + mid_blocks), 256)), dim3(256), 0, st,
                        s.tris.as<pt_triangle>(), n, s.off.as<uint32_t>(), nscenes,
                        s.morton_box.as<float>(), kin, vin, b.scene_of.as<uint32_t>(),
-                       b.span_count.as<uint32_t>(), 1 + mid_blocks);
+                       b.span_count.as<uint32_t>(), n_lists1 + mid_blocks);
     PT_HIP(hipGetLastError());
 
     if (n >= 2) {
@@ -724,9 +994,11 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                            s.off.as<uint32_t>(), b.scene_of.as<uint32_t>(), b.children.as<uint32_t>(),
                            b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                            b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.span_list.as<uint32_t>(),
-                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>(), s.status.as<uint32_t>());
+                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>(), n_lists1,
+                           b.slots.as<uint32_t>(), s.status.as<uint32_t>());
         PT_HIP(hipGetLastError());
     }
+    const uint32_t stage_max = build_tuning().staged ? kStageCap : 0u;
     hipLaunchKernelGGL(k_refit_local, dim3(div_up(n, kRefitBlock)), dim3(kRefitBlock), 0, st,
                        s.tris.as<pt_triangle>(), b.vals.as<uint32_t>(), s.off.as<uint32_t>(),
                        b.scene_of.as<uint32_t>(), b.children.as<uint32_t>(),
@@ -737,13 +1009,16 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
         hipLaunchKernelGGL(k_refit_mid, dim3(mid_blocks), dim3(1024), 0, st, b.children.as<uint32_t>(),
                            b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                            b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(),
-                           b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>());
+                           b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>(), n_lists1,
+                           b.slots.as<uint32_t>(), stage_max, s.status.as<uint32_t>());
     }
     if (n > kMidBlock) {
         hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
                            b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                            b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(),
-                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>());
+                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>() + n_lists1, mid_blocks,
+                           b.span_top.as<uint32_t>() + static_cast<size_t>(mid_blocks) * kListCap,
+                           b.slots.as<uint32_t>(), stage_max, s.status.as<uint32_t>());
     }
     PT_HIP(hipGetLastError());
 }

@@ -966,7 +966,7 @@ struct GraphKey {
     const void* accum;
     const void* counters;
     const void* lights;
-    uint32_t blocks, refill, lds, pool, chains;
+    uint32_t blocks, refill, lds, pool, chains, paths, shade_min;
     hipStream_t stream;
 };
 
@@ -1244,6 +1244,8 @@ void render_passes(Renderer& r, const DeviceScene& scene, uint32_t npasses, cons
         key.lds = trace_tuning().lds;
         key.pool = trace_tuning().pool;
         key.chains = nchains;
+        key.paths = render_tuning().paths;
+        key.shade_min = render_tuning().shade_min;
         key.stream = st;
         replay(r.graph_exec, r.graph_key, key, st,
                [&] { launch_frame(r, scene, fc, nchains, st); });

@@ -98,7 +98,7 @@ def render_stages(num_cus=256):
     or bounce levels in 1, 2 or 4 chains), then how many waiting lanes a
     path-persistent wave shades together, then the traversal knobs."""
     return [{"render.paths": [1, 0], "render.chains": [1, 2, 4]},
-            {"render.shade_min": [16, 32, 48]},
+            {"render.shade_min": [4, 8, 16]},
             render_space(num_cus)]
 
 

@@ -363,6 +363,10 @@ int pt_device_compute_units(int device, uint32_t* cus);
 /* ---- launch tuning (no reference counterpart) -------------------------------
  * Process-wide launch-shape knobs read by the kernels' host launchers; results
  * do not depend on them (tests check this).  Names:
+ *   "build.staged"  1: the LBVH refit's upper levels (nodes whose leaves span
+ *                   more than one 1024-leaf block) run with each block's
+ *                   node list staged in LDS when it holds at most 1024
+ *                   nodes; 0: always from global memory
  *   "trace.blocks"  persistent closest-hit workgroups (256 threads each)
  *   "trace.refill"  refill idle lanes when fewer than this many of 64 are busy
  *   "trace.lds_stack" traversal stack entries per lane kept in LDS (8, 16 or

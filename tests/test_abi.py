@@ -68,6 +68,14 @@ def test_tuning_knobs():
     for v in (0, 9):
         assert ptamd._lib.pt_tune_set(b"render.chains", v) == ptamd.PT_E_INVALID_ARG
     ptamd.tune_set("render.chains", chains)
+    for name in ("build.staged", "trace.filter", "render.paths"):
+        v = ptamd.tune_get(name)
+        assert v == 1
+        for ok in (0, 1):
+            ptamd.tune_set(name, ok)
+            assert ptamd.tune_get(name) == ok
+        assert ptamd._lib.pt_tune_set(name.encode(), 2) == ptamd.PT_E_INVALID_ARG
+        ptamd.tune_set(name, v)
 
 
 def test_null_arguments_rejected():

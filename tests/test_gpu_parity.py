@@ -322,6 +322,33 @@ def test_batched_scenes_build_and_trace(ctx):
     check_build(ctx, *ptamd.gen_soup(5000, 2), tag="after-batch")
 
 
+@pytest.mark.parametrize("staged", [1, 0])
+def test_build_parity_both_refit_paths(ctx, staged):
+    """The refit's upper levels (nodes whose leaves span more than one
+    1024-leaf block) from node lists staged in LDS (build.staged 1) or from
+    global memory (0; also the path of any list longer than 1024 nodes, such
+    as the top list of the config-3 batch, ~5,600 nodes): bit-identical trees
+    on a 100,003-triangle soup and on sampled scenes of the 512-scene batch
+    (log-uniform 1,024..262,144 triangles, 24.0M in all)."""
+    saved = ptamd.tune_get("build.staged")
+    ptamd.tune_set("build.staged", staged)
+    try:
+        check_build(ctx, *ptamd.gen_soup(100003, 4), tag=f"staged={staged}")
+        tris, infos, offs = ptamd.gen_batch(512, 3)
+        say(f"[batched build] staged={staged}, 512 scenes n={len(tris)}")
+        ctx.upload_scenes(tris, offs, infos)
+        assert ctx.verify_bvh() == 0
+        for s in list(range(0, 512, 41)) + [int(np.argmax(np.diff(offs)))]:
+            t = tris[offs[s]:offs[s + 1]]
+            o = orc.build(t, *ptamd.scene_bounds(t))
+            g = ctx.get_bvh_scene(s)
+            for k in ("children", "parents_leaf", "parents_internal"):
+                assert np.array_equal(g[k], o[k]), f"scene {s}: {k}"
+            assert np.array_equal(g["bboxes"].view(np.uint32), o["bboxes"].view(np.uint32)), f"scene {s}"
+    finally:
+        ptamd.tune_set("build.staged", saved)
+
+
 def test_batched_512_scene_build_parity(ctx):
     """Config-3 shape at reduced sizes: 512 soups, log-uniform [64, 4096]."""
     tris, infos, offs = ptamd.gen_batch(512, 7, lo=64, hi=4096)
