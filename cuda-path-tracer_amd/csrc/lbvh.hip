@@ -324,63 +324,6 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
     return __clz(static_cast<int>(ki ^ kj));
 }
 
-// Wide nodes: the doubling search for a node's range, the binary search for
-// its end and the search for its split each take one dependent key load per
-// step on the owning lane, so the few nodes with huge ranges (the root's
-// reaches every key of its scene) would keep their wave, and the kernel,
-// waiting for ~3 log2(range) loads in a row.  A node whose range exceeds
-// kTreeLane keys is instead searched by its whole wave, 63 probes per step.
-// The predicates are monotone in the distance from i (the keys are sorted,
-// and among equal keys the index-augmented prefix shrinks with the
-// distance), so the wave finds exactly the value the lane-serial searches
-// of the reference find: the last distance at which the predicate holds.
-constexpr int64_t kTreeLane = 64;
-
-// Largest x in [lo, hi) with pred(x), for a monotone pred known to hold at lo
-// (or taken to) and to fail at hi; all 64 lanes of the wave call it with the
-// same arguments.
-template <class Pred>
-__device__ __forceinline__ int64_t wave_search(int64_t lo, int64_t hi, Pred pred) {
-    const int lane = threadIdx.x & 63;
-    while (hi - lo > 1) {
-        const int64_t step = (hi - lo + 63) / 64;
-        const int64_t x = lo + (lane + 1) * step;
-        const uint64_t m = __ballot(lane < 63 && x < hi && pred(x));
-        if (m != 0ull) lo += (64 - __clzll(static_cast<long long>(m))) * step;
-        hi = min(hi, lo + step);
-    }
-    return lo;
-}
-
-// One wide node's search parameters, broadcast from its lane.
-struct WideNode {
-    const uint32_t* sk;
-    int64_t n, i, d;
-    uint32_t ki;
-    int lane;
-    __device__ __forceinline__ int bcast(int v) const { return __builtin_amdgcn_readlane(v, lane); }
-    __device__ __forceinline__ int64_t bcast64(int64_t v) const {
-        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), lane));
-        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), lane));
-        return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
-    }
-    // delta(i, i + x d) of the wide node.
-    __device__ __forceinline__ int delta(int64_t x) const { return pt::delta(sk, n, i, ki, i + x * d); }
-};
-
-__device__ __forceinline__ WideNode broadcast(int b, const uint32_t* keys, uint32_t base, int64_t n, int64_t i,
-                                              uint32_t ki, int64_t d) {
-    WideNode w;
-    w.lane = b;
-    const uint32_t bb = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), b));
-    w.sk = keys + bb;
-    w.n = w.bcast64(n);
-    w.i = w.bcast64(i);
-    w.d = w.bcast64(d);
-    w.ki = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ki), b));
-    return w;
-}
-
 // radix_tree_construction_kernel (BVHBuilder.cu:135-196), one thread per
 // internal node, on its scene's segment of sorted keys with local indices
 // (the index augmentation of delta uses them, as in a single-scene build).
@@ -407,72 +350,30 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ slots,
                                                     uint32_t* __restrict__ status) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    // Every lane stays in the wave through the wave-wide searches below;
-    // `valid` marks the lanes that own an internal node.
-    bool valid = g < n;
-    uint32_t base = 0;
-    int64_t N = 0, i = 0;
-    if (valid) {
-        const uint32_t sc = scene_of[g];
-        base = off[sc];
-        N = off[sc + 1] - base;
-        i = g - base;
-        valid = i < N - 1;  // else an unused slot of the scene (or a 1-triangle scene)
-    }
+    if (g >= n) return;
+    const uint32_t sc = scene_of[g];
+    const uint32_t base = off[sc];
+    const int64_t N = off[sc + 1] - base;
+    const int64_t i = g - base;
+    if (i >= N - 1) return;  // unused slot of the scene (or a 1-triangle scene)
     const uint32_t* sk = keys + base;
-    uint32_t ki = 0;
-    int64_t d = 0, lmax = 2;
-    int dmin = 0;
-    bool wide = false;
-    if (valid) {
-        ki = sk[i];
-        const int dr = delta(sk, N, i, ki, i + 1) - delta(sk, N, i, ki, i - 1);
-        d = (dr > 0) - (dr < 0);
-        dmin = delta(sk, N, i, ki, i - d);
-        while (delta(sk, N, i, ki, i + lmax * d) > dmin) {
-            if (lmax == kTreeLane) {  // range beyond kTreeLane: searched by the whole wave
-                wide = true;
-                break;
-            }
-            lmax *= 2;
-        }
-    }
+    const uint32_t ki = sk[i];
+    const int dr = delta(sk, N, i, ki, i + 1) - delta(sk, N, i, ki, i - 1);
+    const int64_t d = (dr > 0) - (dr < 0);
+    const int dmin = delta(sk, N, i, ki, i - d);
+    int64_t lmax = 2;
+    while (delta(sk, N, i, ki, i + lmax * d) > dmin) lmax *= 2;
     int64_t l = 0;
-    for (uint64_t wm = __ballot(wide); wm != 0ull; wm &= wm - 1ull) {
-        const WideNode wn = broadcast(static_cast<int>(__builtin_ctzll(wm)), keys, base, N, i, ki, d);
-        // The doubling search continued from kTreeLane: one probe per lane.
-        const bool up = lane <= 40 && wn.delta(static_cast<int64_t>(kTreeLane) << (lane + 1)) > wn.bcast(dmin);
-        const int64_t wmax = static_cast<int64_t>(kTreeLane) << (__builtin_ctzll(~__ballot(up)) + 1);
-        const int lim = wn.bcast(dmin);
-        const int64_t wl = wave_search(wmax / 2, wmax, [&](int64_t x) { return wn.delta(x) > lim; });
-        if (lane == wn.lane) {
-            lmax = wmax;
-            l = wl;
-        }
-    }
-    if (valid && !wide) {
-        while (lmax > 1) {
-            lmax /= 2;
-            if (delta(sk, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
-        }
+    while (lmax > 1) {
+        lmax /= 2;
+        if (delta(sk, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
     }
     const int64_t j = i + l * d;
-    const int dnode = valid ? delta(sk, N, i, ki, j) : 0;
+    const int dnode = delta(sk, N, i, ki, j);
     int64_t s = 0;
-    for (uint64_t wm = __ballot(wide); wm != 0ull; wm &= wm - 1ull) {
-        const WideNode wn = broadcast(static_cast<int>(__builtin_ctzll(wm)), keys, base, N, i, ki, d);
-        const int lim = wn.bcast(dnode);
-        const int64_t ws = wave_search(0, wn.bcast64(l), [&](int64_t x) { return wn.delta(x) > lim; });
-        if (lane == wn.lane) s = ws;
-    }
-    if (!valid) return;
-    if (!wide) {
-        int64_t t = l;
-        while (t > 1) {
-            t = (t + 1) / 2;
-            if (delta(sk, N, i, ki, i + (s + t) * d) > dnode) s += t;
-        }
+    while (l > 1) {
+        l = (l + 1) / 2;
+        if (delta(sk, N, i, ki, i + (s + l) * d) > dnode) s += l;
     }
     s = i + s * d + (d < 0 ? d : 0);
     uint32_t type = 0;
