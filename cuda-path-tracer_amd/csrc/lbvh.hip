@@ -153,7 +153,8 @@ __device__ __forceinline__ uint32_t sort_digit(uint32_t k, uint32_t v,
     return ((BY_SCENE ? scene_of[v] : k) >> shift) & (kRadix - 1u);
 }
 
-// Per-tile digit histogram: tile_hist[digit * num_tiles + tile].
+// Per-tile digit histogram: tile_hist[tile * kRadix + digit] (one contiguous
+// row per tile).
 template <bool BY_SCENE>
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ vals,
@@ -176,27 +177,55 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __re
 #pragma unroll
     for (int j = 0; j < kDigitsPerThread; ++j) {
         const int d = tid + j * kSortThreads;
-        tile_hist[static_cast<size_t>(d) * num_tiles + blockIdx.x] = h[d];
+        tile_hist[static_cast<size_t>(blockIdx.x) * kRadix + d] = h[d];
     }
 }
 
-// One block per digit: exclusive scan of that digit's row of tile counts (in
-// place) and the digit's total.
-__global__ __launch_bounds__(kSortThreads) void k_sort_rowscan(uint32_t* __restrict__ tile_hist,
-                                                               uint32_t num_tiles,
-                                                               uint32_t* __restrict__ digit_total) {
-    __shared__ uint32_t s_w[4];
-    uint32_t* row = tile_hist + static_cast<size_t>(blockIdx.x) * num_tiles;
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < num_tiles; b += kSortThreads) {
-        uint32_t i = b + threadIdx.x;
-        uint32_t v = i < num_tiles ? row[i] : 0u;
-        uint32_t tot;
-        uint32_t ex = block_excl_scan_256(v, s_w, tot);
-        if (i < num_tiles) row[i] = carry + ex;
-        carry += tot;
+// Exclusive scan of each digit's counts over the tiles (in place) and the
+// digit's total.  One workgroup per 64 digits: lane = digit, so every load
+// and store of a tile's 64 counts is one contiguous 256-byte access; the 16
+// waves split the tiles into 16 ranges, and each wave's range starts at the
+// sum of the earlier ranges.
+constexpr int kScanDigits = 64;
+constexpr int kScanWaves = 16;
+
+__global__ __launch_bounds__(kScanWaves * 64) void k_sort_rowscan(uint32_t* __restrict__ tile_hist,
+                                                                  uint32_t num_tiles,
+                                                                  uint32_t* __restrict__ digit_total) {
+    __shared__ uint32_t s_part[kScanWaves][kScanDigits];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const uint32_t d = blockIdx.x * kScanDigits + lane;
+    const uint32_t per = (num_tiles + kScanWaves - 1) / kScanWaves;
+    const uint32_t t0 = min(num_tiles, wave * per), t1 = min(num_tiles, t0 + per);
+    // Loads in batches of kScanBatch, issued together (the counts come from
+    // other XCDs' writes: each load is a far round trip).
+    constexpr uint32_t kScanBatch = 16;
+    uint32_t sum = 0;
+    for (uint32_t t = t0; t < t1; t += kScanBatch) {
+        uint32_t v[kScanBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kScanBatch; ++k)
+            v[k] = t + k < t1 ? tile_hist[static_cast<size_t>(t + k) * kRadix + d] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanBatch; ++k) sum += v[k];
     }
-    if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
+    s_part[wave][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int w = 0; w < wave; ++w) run += s_part[w][lane];
+    for (uint32_t t = t0; t < t1; t += kScanBatch) {
+        uint32_t v[kScanBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kScanBatch; ++k)
+            v[k] = t + k < t1 ? tile_hist[static_cast<size_t>(t + k) * kRadix + d] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanBatch; ++k) {
+            if (t + k < t1) tile_hist[static_cast<size_t>(t + k) * kRadix + d] = run;
+            run += v[k];
+        }
+    }
+    if (wave == kScanWaves - 1) digit_total[d] = run;
 }
 
 // Stable scatter of one 4096-key tile.  Each wave ranks 64 keys at a time
@@ -285,7 +314,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     for (int j = 0; j < kDigitsPerThread; ++j) {
         const int d = kDigitsPerThread * tid + j;
         s_dstart[d] = dstart;
-        s_gbase[d] = gpre + tile_hist[static_cast<size_t>(d) * num_tiles + tile];
+        s_gbase[d] = gpre + tile_hist[static_cast<size_t>(tile) * kRadix + d];
         dstart += tot[j];
         gpre += gt[j];
     }
@@ -871,7 +900,7 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                 hipLaunchKernelGGL(k_sort_hist<false>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
                                    b.scene_of.as<uint32_t>(), n, shift, num_tiles, b.tile_hist.as<uint32_t>());
             }
-            hipLaunchKernelGGL(k_sort_rowscan, dim3(kRadix), dim3(kSortThreads), 0, st,
+            hipLaunchKernelGGL(k_sort_rowscan, dim3(kRadix / kScanDigits), dim3(kScanWaves * 64), 0, st,
                                b.tile_hist.as<uint32_t>(), num_tiles, b.digit_total.as<uint32_t>());
             if (by_scene) {
                 hipLaunchKernelGGL(k_sort_scatter<true>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
