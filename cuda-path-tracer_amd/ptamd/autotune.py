@@ -87,8 +87,8 @@ def render_space(num_cus=256):
     """Traversal knobs for a progressive render, searched together on the
     render's own frames: its bounce rays traverse deeper than the closest-hit
     microbench's shell rays and favour other lane and queue settings."""
-    return {"trace.blocks": [num_cus * 8, num_cus * 12],
-            "trace.refill": [32, 48],
+    return {"trace.blocks": [num_cus * 8, num_cus * 12, num_cus * 16],
+            "trace.refill": [32, 48, 56, 64],
             "trace.lds_stack": [8, 16],
             "trace.pool": [128, 256]}
 
@@ -96,10 +96,12 @@ def render_space(num_cus=256):
 def render_stages(num_cus=256):
     """Staged render search: the frame's structure first (path-persistent,
     or bounce levels in 1, 2 or 4 chains), then how many waiting lanes a
-    path-persistent wave shades together, then the traversal knobs."""
+    path-persistent wave shades together, then the traversal knobs, then the
+    shade batch again (it trades against the lane refill threshold)."""
     return [{"render.paths": [1, 0], "render.chains": [1, 2, 4]},
             {"render.shade_min": [4, 8, 16]},
-            render_space(num_cus)]
+            render_space(num_cus),
+            {"render.shade_min": [4, 8, 16, 32]}]
 
 
 def trace_stages(num_cus=256):
