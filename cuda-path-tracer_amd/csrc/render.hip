@@ -597,12 +597,17 @@ __global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, 
                                                            uint32_t pool_large, uint32_t shade_min) {
     __shared__ uint32_t s_stack[LDS][kPersistThreads];
     __shared__ uint32_t s_level[kMaxLevels];
+    // Plain frames keep each lane's path throughput and radiance in LDS
+    // while it traverses: the registers saved take the kernel to 6 waves per
+    // SIMD.  (With lights or surface detail the kernel stays at 4 waves per
+    // SIMD either way, and the LDS round trips only cost.)
+    constexpr bool kCarryLds = !LIGHTS && !DETAIL;
+    __shared__ float s_carry[kCarryLds ? 6 : 1][kPersistThreads];
     for (uint32_t k = threadIdx.x; k < kMaxLevels; k += blockDim.x) s_level[k] = 0u;
     __syncthreads();
     const uint32_t npaths = fc.npix * fc.npasses;
     const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u);
     const int lane = threadIdx.x & 63;
-    const uint64_t lt = (1ull << lane) - 1ull;
     // Waves past one per kRayPoolSmall paths skip the loop (but reach the
     // final barrier).
     if (wave_id == 0u || static_cast<uint64_t>(wave_id) * kRayPoolSmall < npaths) {
@@ -612,9 +617,10 @@ __global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, 
         ho.init(queue, npaths, npaths / 4u >= gridDim.x * blockDim.x ? pool_large : kRayPoolSmall, wave_id);
         RayState r;
         uint32_t state = kLaneIdle;
-        uint32_t path = 0, pix = 0, pass = 0, bounce = 0;
-        uint32_t root = kInvalid, single = kInvalid;
-        V3 absorbed = v3(1.0f, 1.0f, 1.0f), acc = v3(0.0f, 0.0f, 0.0f);
+        uint32_t path = 0, bounce = 0;  // the path's pass and pixel follow from its index
+        uint32_t root = kInvalid;
+        V3 absorbed_r = v3(1.0f, 1.0f, 1.0f), acc_r = v3(0.0f, 0.0f, 0.0f);
+        const int tid = threadIdx.x;
         const uint32_t w = static_cast<uint32_t>(fc.w), h = static_cast<uint32_t>(fc.h);
         for (;;) {
             // New paths for idle lanes.
@@ -625,13 +631,14 @@ __global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, 
                     ho.next_pool();
                     if (ho.drained) break;
                     const uint32_t avail = ho.end - ho.next;
-                    const uint32_t rank = static_cast<uint32_t>(__popcll(idle & lt));
+                    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(idle >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(idle), 0u));
                     const bool take = ((idle >> lane) & 1ull) && rank < avail;
                     const uint32_t took = min(static_cast<uint32_t>(__popcll(idle)), avail);
                     if (take) {
                         const uint32_t slot = ho.next + rank;
-                        pass = slot / fc.npix;
-                        pix = tiled_pixel(slot - pass * fc.npix, w, h);
+                        const uint32_t pass = slot / fc.npix;
+                        const uint32_t pix = tiled_pixel(slot - pass * fc.npix, w, h);
                         path = pass * fc.npix + pix;
                         const PassParams& P = pp[pass];
                         const float tx = static_cast<float>(pix % w) / static_cast<float>(fc.w);
@@ -644,10 +651,14 @@ __global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, 
                         r.d = mix(mix(d00, d10, tx), mix(d01, d11, tx), ty);
                         const uint32_t sb = off[P.scene], se = off[P.scene + 1];
                         root = se - sb >= 2 ? sb : kInvalid;
-                        single = se - sb == 1 ? sb : kInvalid;
                         bounce = 0;
-                        absorbed = v3(1.0f, 1.0f, 1.0f);
-                        acc = v3(0.0f, 0.0f, 0.0f);
+                        if constexpr (kCarryLds) {
+                            s_carry[0][tid] = 1.0f; s_carry[1][tid] = 1.0f; s_carry[2][tid] = 1.0f;
+                            s_carry[3][tid] = 0.0f; s_carry[4][tid] = 0.0f; s_carry[5][tid] = 0.0f;
+                        } else {
+                            absorbed_r = v3(1.0f, 1.0f, 1.0f);
+                            acc_r = v3(0.0f, 0.0f, 0.0f);
+                        }
                         init_ray(r, root);
                         state = kLaneTrace;
                     }
@@ -665,6 +676,9 @@ __global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, 
                 if (root == kInvalid) {
                     r.minT = kMinTInit;
                     r.best = kInvalid;
+                    // A scene of fewer than two triangles: its one triangle, if any.
+                    const uint32_t psc = pp[path / fc.npix].scene;
+                    const uint32_t single = off[psc + 1] - off[psc] == 1u ? off[psc] : kInvalid;
                     if (single != kInvalid) {
                         float u = 0.0f, v = 0.0f;
                         uint32_t id;
@@ -685,7 +699,14 @@ __global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, 
                 if (state == kLaneShade) {
                     atomicAdd(&s_level[bounce], 1u);  // this segment, now complete
                     bool cont = false;
+                    V3 absorbed = absorbed_r, acc = acc_r;
+                    if constexpr (kCarryLds) {
+                        absorbed = v3(s_carry[0][tid], s_carry[1][tid], s_carry[2][tid]);
+                        acc = v3(s_carry[3][tid], s_carry[4][tid], s_carry[5][tid]);
+                    }
                     if (r.best != kInvalid) {
+                        const uint32_t pass = path / fc.npix;
+                        const uint32_t pix = path - pass * fc.npix;
                         const PassParams& P = pp[pass];
                         HitShade hs = hit_shade<DETAIL>(fc, infos, r.best, r.minT, r.bu, r.bv, r.o, r.d);
                         const uint32_t sb = off[P.scene], se = off[P.scene + 1];
@@ -702,6 +723,13 @@ __global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, 
                             r.d = nd;
                             init_ray(r, root);
                             state = kLaneTrace;
+                            if constexpr (kCarryLds) {
+                                s_carry[0][tid] = absorbed.x; s_carry[1][tid] = absorbed.y; s_carry[2][tid] = absorbed.z;
+                                s_carry[3][tid] = acc.x; s_carry[4][tid] = acc.y; s_carry[5][tid] = acc.z;
+                            } else {
+                                absorbed_r = absorbed;
+                                acc_r = acc;
+                            }
                         }
                     }
                     if (!cont) {
