@@ -556,14 +556,29 @@ def main():
         hrays.free()
         w2, h2, eye2, tgt2 = 1920, 1080, (0.0, 0.3, -3.2), (0.0, 0.0, 0.0)
         ctx.set_primary_rays(ptamd.camera_corners(eye2, tgt2, (0, 1, 0), 60.0, w2 / h2), eye2)
+        hframes = 4
         hdirs = np.stack([ptamd.camera_corners(eye2, tgt2, (0, 1, 0), 60.0, w2 / h2, *ptamd.pass_jitter(6, p, w2, h2))
-                          for p in range(3)])
-        hseeds = np.arange(2000, 2003, dtype=np.uint64)
+                          for p in range(hframes + 1)])
+        hseeds = np.arange(2000, 2001 + hframes, dtype=np.uint64)
         ctx.render_passes(w2, h2, hseeds[:1], hdirs[:1])  # warm-up (allocation, capture)
+        # The render's knobs tuned on its own one-pass frames, as for the
+        # other render legs (restored afterwards).
+        hrtune, hsaved = None, None
+        if not args.no_tune:
+            from ptamd import autotune
+
+            stages = autotune.render_stages(autotune.device_cus())
+            hsaved = {k: ptamd.tune_get(k) for st_ in stages for k in st_}
+            hbest, htable = autotune.tune_staged(lambda: ctx.render_passes(w2, h2, hseeds[:1], hdirs[:1]),
+                                                 stream, stages, reps=2)
+            hrtune = {"chosen": hbest, "candidates": len(htable)}
         ctx.reset_frames()
         hr_ms = timed(stream, lambda: [ctx.render_passes(w2, h2, hseeds[p:p + 1], hdirs[p:p + 1])
-                                       for p in range(1, 3)]) / 2
+                                       for p in range(1, hframes + 1)]) / hframes
         hsegs = ctx.last_frame_rays()
+        if hsaved:
+            for k, v in hsaved.items():
+                ptamd.tune_set(k, v)
         h_err = None
         if not args.profile:
             acc, spp = ctx.read_accumulator(w2, h2)
@@ -576,8 +591,11 @@ def main():
                 "algorithmic_GB_per_s": round(hbytes / (h_ms * 1e-3) / 1e9, 1),
                 "hbm_frac": round(hbytes / (h_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "render_1920x1080_b6": {"ms_per_pass": round(hr_ms, 3), "spp_per_s": round(1e3 / hr_ms, 2),
+                                        "passes": hframes, "passes_per_frame": 1,
                                         "segments_per_pass": int(hsegs),
                                         "Msegments_per_s": round(hsegs / (hr_ms * 1e-3) / 1e6, 1)}}
+        if hrtune:
+            hair["render_1920x1080_b6"]["tuning"] = hrtune
         if h_err is not None:
             hair["render_1920x1080_b6"].update({"rmse": h_err[0], "rmse_spp": h_err[2], "rmse_ref_spp": 128,
                                                 "rmse_ref_noise": h_err[1]})
