@@ -343,13 +343,18 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
 
 // ---- radix tree -----------------------------------------------------------------
 
-// delta (BVHBuilder.cu:116-133): common prefix length of keys i and j,
-// augmented with the index bits when the keys are equal; -1 outside [0, n).
-__device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t n, int64_t i,
-                                     uint32_t ki, int64_t j) {
-    if (j < 0 || j >= n) return -1;
+// delta (BVHBuilder.cu:116-133) of key i and the key x places from it
+// (forward or backward): common prefix length, augmented with the index
+// bits when the keys are equal; -1 outside [0, n).  Indices are 32-bit (a
+// scene holds < 2^30 keys, so an offset clamped to 2^30 is already out of
+// range).
+__device__ __forceinline__ int delta_at(const uint32_t* __restrict__ keys, uint32_t n, uint32_t i, uint32_t ki,
+                                        bool fwd, uint32_t x) {
+    x = min(x, 1u << 30);
+    const int32_t j = fwd ? static_cast<int32_t>(i + x) : static_cast<int32_t>(i) - static_cast<int32_t>(x);
+    if (j < 0 || static_cast<uint32_t>(j) >= n) return -1;
     const uint32_t kj = keys[j];
-    if (ki == kj) return 32 + __clz(static_cast<int>(static_cast<uint32_t>(i ^ j)));
+    if (ki == kj) return 32 + __clz(static_cast<int>(i ^ static_cast<uint32_t>(j)));
     return __clz(static_cast<int>(ki ^ kj));
 }
 
@@ -386,25 +391,30 @@ __global__ __launch_bounds__(256) void k_radix_tree(const uint32_t* __restrict__
     const int64_t i = g - base;
     if (i >= N - 1) return;  // unused slot of the scene (or a 1-triangle scene)
     const uint32_t* sk = keys + base;
-    const uint32_t ki = sk[i];
-    const int dr = delta(sk, N, i, ki, i + 1) - delta(sk, N, i, ki, i - 1);
-    const int64_t d = (dr > 0) - (dr < 0);
-    const int dmin = delta(sk, N, i, ki, i - d);
-    int64_t lmax = 2;
-    while (delta(sk, N, i, ki, i + lmax * d) > dmin) lmax *= 2;
-    int64_t l = 0;
+    const uint32_t n_loc = static_cast<uint32_t>(N), il = static_cast<uint32_t>(i);
+    const uint32_t ki = sk[il];
+    // Direction, the range's far end j and the split (Karras 2012, Alg. 2;
+    // the reference's exponential, binary and ceil-halving searches), on
+    // 32-bit offsets from i.
+    const int dp = delta_at(sk, n_loc, il, ki, true, 1), dm = delta_at(sk, n_loc, il, ki, false, 1);
+    const bool fwd = dp > dm;  // never equal: augmented keys are distinct
+    const int dmin = fwd ? dm : dp;
+    uint32_t lmax = 2;
+    while (delta_at(sk, n_loc, il, ki, fwd, lmax) > dmin) lmax *= 2;  // a probe at 2^30 fails: no overflow
+    uint32_t l = 0;
     while (lmax > 1) {
         lmax /= 2;
-        if (delta(sk, N, i, ki, i + (l + lmax) * d) > dmin) l += lmax;
+        if (delta_at(sk, n_loc, il, ki, fwd, l + lmax) > dmin) l += lmax;
     }
-    const int64_t j = i + l * d;
-    const int dnode = delta(sk, N, i, ki, j);
-    int64_t s = 0;
-    while (l > 1) {
-        l = (l + 1) / 2;
-        if (delta(sk, N, i, ki, i + (s + l) * d) > dnode) s += l;
+    const int dnode = delta_at(sk, n_loc, il, ki, fwd, l);
+    uint32_t sx = 0, r = l;
+    while (r > 1) {
+        r = (r + 1) / 2;
+        if (delta_at(sk, n_loc, il, ki, fwd, sx + r) > dnode) sx += r;
     }
-    s = i + s * d + (d < 0 ? d : 0);
+    const int64_t d = fwd ? 1 : -1;
+    const int64_t j = i + static_cast<int64_t>(l) * d;
+    const int64_t s = i + static_cast<int64_t>(sx) * d + (d < 0 ? d : 0);
     uint32_t type = 0;
     if ((i < j ? i : j) != s) {
         type |= 1u;
