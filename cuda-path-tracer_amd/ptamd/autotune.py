@@ -87,7 +87,7 @@ def render_space(num_cus=256):
     """Traversal knobs for a progressive render, searched together on the
     render's own frames: its bounce rays traverse deeper than the closest-hit
     microbench's shell rays and favour other lane and queue settings."""
-    return {"trace.blocks": [num_cus * 8, num_cus * 12, num_cus * 16],
+    return {"trace.blocks": [num_cus * 6, num_cus * 8, num_cus * 12, num_cus * 16],
             "trace.refill": [32, 48, 56, 64],
             "trace.lds_stack": [8, 16],
             "trace.pool": [128, 256]}
