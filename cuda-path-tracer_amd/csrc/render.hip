@@ -585,8 +585,12 @@ __global__ __launch_bounds__(64) void k_paths_setup(uint32_t* __restrict__ count
     zero_queue(queue, 1u);
 }
 
+// At most 80 VGPRs (6 waves per SIMD): the plain variant fits without
+// spilling, the surface-detail variants too (109 VGPRs uncapped), and the lit
+// variants spill a little in their shading, which costs no more than the
+// waves it buys.
 template <int LDS, bool LIGHTS, bool DETAIL>
-__global__ __launch_bounds__(kPersistThreads) void k_paths(const FrameConst fc, const PassParams* __restrict__ pp,
+__global__ __launch_bounds__(kPersistThreads, 6) void k_paths(const FrameConst fc, const PassParams* __restrict__ pp,
                                                            const pt_triangle_info* __restrict__ infos,
                                                            const float4* __restrict__ nodes,
                                                            const float4* __restrict__ tris,
