@@ -144,7 +144,20 @@ def cache_leg(ctx, stream, args, seed_mesh):
     def plain_frame(f):
         ctx.render_passes(w, h, plain_seeds[f:f + 1], plain_dirs[f][None])
 
+    # Each side runs at the traversal knobs tuned on its own frames (frame
+    # structure, shade batch and launch shape for the plain frames; launch
+    # shape for the cache frames, whose pixel paths run as two levels).
+    import ptamd.autotune as autotune
+
+    start_knobs = knob_state()
+    tuning = {}
+    if not args.no_tune:
+        ctx.reset_frames()
+        best, _ = autotune.tune_staged(lambda: plain_frame(0), stream, autotune.render_stages(autotune.device_cus()),
+                                       reps=3)
+        tuning["plain"] = best
     plain = run(checkpoints, plain_frame)
+    set_knobs(start_knobs)
     log(f"[bench] cache leg: plain {plain_max} spp in {plain[-1][1]:.0f} ms, RMSE {max(plain[-1][2], 0) ** 0.5:.5f}")
 
     cs = ptamd.CacheSettings(ntrain, steps, extra, train=True)
@@ -158,9 +171,15 @@ def cache_leg(ctx, stream, args, seed_mesh):
     # 1,024-spp RMSE, so the time to that RMSE is measured, not extrapolated.
     target = plain[-1][2]
     points = sorted(set([1 << k for k in range(9)] + [frames] + [frames * k // 4 for k in range(5, 17)]))
+    if not args.no_tune:
+        ctx.cache_init(7, bmin, bmax)
+        ctx.reset_frames()
+        best, _ = autotune.tune(lambda: cache_frame(0), stream, autotune.render_space(autotune.device_cus()), reps=3)
+        tuning["cache"] = best
     ctx.cache_init(7, bmin, bmax)
     cached = run(points, cache_frame, reset=lambda: ctx.cache_init(7, bmin, bmax), stop_mse=target,
                  min_frames=frames)
+    set_knobs(start_knobs)
     at_frames = next(m for c, t, m in cached if c == frames)
     log(f"[bench] cache leg: {cached[-1][0]} cache frames in {cached[-1][1]:.0f} ms, "
         f"RMSE {max(cached[-1][2], 0) ** 0.5:.5f} ({frames} frames: {max(at_frames, 0) ** 0.5:.5f})")
@@ -207,6 +226,7 @@ def cache_leg(ctx, stream, args, seed_mesh):
         "equal_time_rmse": {"ms": t_at_frames, "cache": rmse(at_frames),
                             "plain": None if plain_at_frames is None else rmse(plain_at_frames)},
         "ms_per_frame": {"plain": round(t_plain / plain_max, 3), "cache": round(cached[-1][1] / cached[-1][0], 3)},
+        "tuning": tuning,
     }
 
     # MLP kernels alone on this workload's sizes: inference over every pixel's
@@ -330,6 +350,23 @@ def same_bits(a, b):
     return bool(np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8)))
 
 
+KNOBS = ("trace.blocks", "trace.refill", "trace.lds_stack", "trace.pool", "trace.filter", "render.paths",
+         "render.chains", "render.shade_min")
+
+
+def knob_state():
+    import ptamd
+
+    return {k: ptamd.tune_get(k) for k in KNOBS}
+
+
+def set_knobs(state):
+    import ptamd
+
+    for k, v in state.items():
+        ptamd.tune_set(k, v)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,6 +414,10 @@ def main():
     ctx = ptamd.Context(0)
     stream = hip.Stream()
     ctx.set_stream(stream.handle)
+    # Library defaults: every leg after the closest hit starts from these (a
+    # leg that tunes restores them afterwards), not from the closest hit's
+    # choice.
+    defaults = knob_state()
     ctx.upload_scene(tris, infos)
     d_rays = hip.DeviceArray.from_host(rays)
     n = len(rays)
@@ -418,6 +459,7 @@ def main():
     e1.synchronize()
     hip.synchronize()
     wall = time.time() - w0
+    closest_hit_knobs = knob_state()
     ms = e0.elapsed_ms(e1)
     ms_per_step = ms / args.steps
     mrays = n / (ms_per_step * 1e-3) / 1e6
@@ -467,6 +509,8 @@ def main():
             "mesh_871200": {"ms": round(build_ms_mesh, 4), "roofline": build_roofline(len(tris), build_ms_mesh)}}
     log(f"[bench] LBVH build: {build_ms_1m:.3f} ms @ 2^20 soup ({lbvh['soup_2^20']['roofline']['frac']} of HBM; "
         f"first build with allocation and upload {first_build_ms:.1f} ms), {build_ms_mesh:.3f} ms @ 871,200 mesh")
+
+    set_knobs(defaults)
 
     # Progressive render legs (spp/s): config 0 (16 passes per frame) and
     # config 1 (one pass per frame), frames replayed as captured hipGraphs.
@@ -549,9 +593,11 @@ def main():
         def hstep():
             ctx.trace_closest_device(hrays.ptr, hn, d_id.ptr, d_t.ptr, 0, stream.handle)
 
+        set_knobs(closest_hit_knobs)  # the launch shape tuned for the closest-hit microbench
         hstep()
         stream.synchronize()
         h_ms = timed(stream, hstep, 3)
+        set_knobs(defaults)
         hair_parity = (d_id.to_host(), d_t.to_host())
         hrays.free()
         w2, h2, eye2, tgt2 = 1920, 1080, (0.0, 0.3, -3.2), (0.0, 0.0, 0.0)
