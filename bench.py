@@ -145,8 +145,7 @@ def cache_leg(ctx, stream, args, seed_mesh):
         ctx.render_passes(w, h, plain_seeds[f:f + 1], plain_dirs[f][None])
 
     # Each side runs at the traversal knobs tuned on its own frames (frame
-    # structure, shade batch and launch shape for the plain frames; launch
-    # shape for the cache frames, whose pixel paths run as two levels).
+    # structure, shade batch and launch shape).
     import ptamd.autotune as autotune
 
     start_knobs = knob_state()
@@ -174,7 +173,9 @@ def cache_leg(ctx, stream, args, seed_mesh):
     if not args.no_tune:
         ctx.cache_init(7, bmin, bmax)
         ctx.reset_frames()
-        best, _ = autotune.tune(lambda: cache_frame(0), stream, autotune.render_space(autotune.device_cus()), reps=3)
+        best, _ = autotune.tune_staged(lambda: cache_frame(0), stream,
+                                       [{"render.paths": [1, 0]}, {"render.shade_min": [4, 8, 16]},
+                                        autotune.render_space(autotune.device_cus())], reps=3)
         tuning["cache"] = best
     ctx.cache_init(7, bmin, bmax)
     cached = run(points, cache_frame, reset=lambda: ctx.cache_init(7, bmin, bmax), stop_mse=target,

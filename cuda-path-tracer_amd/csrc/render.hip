@@ -589,7 +589,11 @@ __global__ __launch_bounds__(64) void k_paths_setup(uint32_t* __restrict__ count
 // spilling, the surface-detail variants too (109 VGPRs uncapped), and the lit
 // variants spill a little in their shading, which costs no more than the
 // waves it buys.
-template <int LDS, bool LIGHTS, bool DETAIL>
+// CACHE: the pixel paths of a cache frame (one pass, pixel = path), each
+// ending in the cache at its second hit (record_cache_vertex) with its
+// throughput in ps.ar/ag/ab -- the same values the levels k_extend /
+// k_shade<true> leave.
+template <int LDS, bool LIGHTS, bool DETAIL, bool CACHE>
 __global__ __launch_bounds__(kPersistThreads, 6) void k_paths(const FrameConst fc, const PassParams* __restrict__ pp,
                                                            const pt_triangle_info* __restrict__ infos,
                                                            const float4* __restrict__ nodes,
@@ -598,7 +602,7 @@ __global__ __launch_bounds__(kPersistThreads, 6) void k_paths(const FrameConst f
                                                            const pt_dir_light* __restrict__ lights, PathState ps,
                                                            uint32_t* __restrict__ counters,
                                                            uint32_t* __restrict__ queue, uint32_t refill,
-                                                           uint32_t pool_large, uint32_t shade_min) {
+                                                           uint32_t pool_large, uint32_t shade_min, CacheIO io) {
     __shared__ uint32_t s_stack[LDS][kPersistThreads];
     __shared__ uint32_t s_level[kMaxLevels];
     // Plain frames keep each lane's path throughput and radiance in LDS
@@ -719,8 +723,10 @@ __global__ __launch_bounds__(kPersistThreads, 6) void k_paths(const FrameConst f
                         } else {
                             bounce_hit<LIGHTS>(hs, r.d, fc, lights, nodes, tris, sb, se, absorbed, acc);
                         }
+                        if (CACHE && bounce == 1) record_cache_vertex(io, false, pix, path, hs, r.d, absorbed, acc);
                         V3 no{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
-                        if (bounce < fc.bounces) cont = next_bounce(pix, P.pass, bounce, P.seedw, fc, hs, absorbed, no, nd);
+                        const uint32_t limit = CACHE ? 1u : fc.bounces;
+                        if (bounce < limit) cont = next_bounce(pix, P.pass, bounce, P.seedw, fc, hs, absorbed, no, nd);
                         if (cont) {
                             ++bounce;
                             r.o = no;
@@ -740,6 +746,11 @@ __global__ __launch_bounds__(kPersistThreads, 6) void k_paths(const FrameConst f
                         ps.cr[path] = acc.x;
                         ps.cg[path] = acc.y;
                         ps.cb[path] = acc.z;
+                        if constexpr (CACHE) {  // the weight of the cache's answer (k_cache_resolve)
+                            ps.ar[path] = absorbed.x;
+                            ps.ag[path] = absorbed.y;
+                            ps.ab[path] = absorbed.z;
+                        }
                         state = kLaneIdle;
                     }
                 }
@@ -841,13 +852,15 @@ __global__ __launch_bounds__(kThreads) void k_generate_cache(const FrameConst fc
                                                              RayQueue q, PathState ps,
                                                              uint32_t* __restrict__ counters,
                                                              uint32_t* __restrict__ ray_counters,
-                                                             uint32_t nlevels, CacheIO io) {
+                                                             uint32_t nlevels, CacheIO io, uint32_t count0) {
     const uint32_t npaths = fc.npix + io.ntrain;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid == 0) {
-        // The frame's levels trace the pixel paths; the training paths' rays
-        // (slots npix ..) are read by k_train_paths.
-        counters[0] = fc.npix;
+        // The pixel paths' first-level count: npix when the levels trace
+        // them, 0 when a path-persistent launch does and counts its own
+        // segments; the training paths' rays (slots npix ..) are read by
+        // k_train_paths.
+        counters[0] = count0;
         for (uint32_t k = 1; k <= nlevels; ++k) counters[k] = 0u;
         counters[kTrainLevel0Slot] = 0u;
         counters[kTrainLevel1Slot] = 0u;
@@ -1029,8 +1042,14 @@ void ensure_chain_streams(Renderer& r, uint32_t nchains) {
 
 template <int LDS>
 auto paths_kernel(const FrameConst& fc) {
-    return fc.detail ? (fc.nlights ? k_paths<LDS, true, true> : k_paths<LDS, false, true>)
-                     : (fc.nlights ? k_paths<LDS, true, false> : k_paths<LDS, false, false>);
+    return fc.detail ? (fc.nlights ? k_paths<LDS, true, true, false> : k_paths<LDS, false, true, false>)
+                     : (fc.nlights ? k_paths<LDS, true, false, false> : k_paths<LDS, false, false, false>);
+}
+
+template <int LDS>
+auto cache_paths_kernel(const FrameConst& fc) {
+    return fc.detail ? (fc.nlights ? k_paths<LDS, true, true, true> : k_paths<LDS, false, true, true>)
+                     : (fc.nlights ? k_paths<LDS, true, false, true> : k_paths<LDS, false, false, true>);
 }
 
 void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, uint32_t nchains, hipStream_t st) {
@@ -1051,7 +1070,7 @@ void launch_frame(Renderer& r, const DeviceScene& scene, const FrameConst& fc, u
                            scene.infos.as<pt_triangle_info>(), scene.bvh.nodes.as<float4>(),
                            scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>(), scene.lights.as<pt_dir_light>(),
                            L.ps, cnt, cnt + 64, refill, pool,
-                           std::min(64u, std::max(1u, render_tuning().shade_min)));
+                           std::min(64u, std::max(1u, render_tuning().shade_min)), CacheIO{});
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, L.ps,
                            r.accum.as<double>());
         return;
@@ -1315,12 +1334,14 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
     auto shade = fc.detail ? (fc.nlights ? k_shade<true, true, true> : k_shade<true, false, true>)
                            : (fc.nlights ? k_shade<true, true, false> : k_shade<true, false, false>);
     auto train_paths = fc.detail ? k_train_paths<true> : k_train_paths<false>;
+    const bool paths = render_tuning().paths != 0;
     hipLaunchKernelGGL(k_generate_cache, dim3(grid), dim3(kThreads), 0, st, fc, pp, L.q[0], L.ps, cnt, rcnt,
-                       nlev, io);
+                       nlev, io, paths ? 0u : fc.npix);
     // The training paths run on the side stream from here, beside the pixel
-    // paths' two levels, inference, resolve and accumulation: they read the
-    // first level's queue slots from npix on and write only training-path
-    // state, the training batch and counter slots the levels do not read.
+    // paths (two levels, or one path-persistent launch), inference, resolve
+    // and accumulation: they read the first level's queue slots from npix on
+    // and write only training-path state, the training batch and counter
+    // slots the pixel paths do not read.
     // The join comes before the training targets and the Adam steps, which
     // change the weights the inference reads.
     if (io.ntrain) {
@@ -1333,15 +1354,29 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
                            scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
         PT_HIP(hipEventRecord(r.join_ev, r.side));
     }
-    // Levels 0 and 1 of the pixel paths (a pixel path ends in the cache at x2).
-    for (uint32_t lvl = 0; lvl < 2; ++lvl) {
-        hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
-                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
-                           scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill, pool);
-        hipLaunchKernelGGL(shade, dim3(shade_grid_for(npaths)), dim3(kShadeThreads), 0, st, fc, pp,
-                           scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
-                           L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
-                           scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>());
+    if (paths) {
+        // The pixel paths in one path-persistent launch (render.paths), each
+        // lane carrying a path from its primary ray to the cache.
+        const uint32_t ppgrid = std::min(std::max(1u, tune.blocks), grid_for(fc.npix));
+        const uint32_t ppool = pool_for(tune.pool, fc.npix, ppgrid);
+        auto kern = tune.lds <= 8 ? cache_paths_kernel<8>(fc) : cache_paths_kernel<16>(fc);
+        hipLaunchKernelGGL(kern, dim3(ppgrid), dim3(kPersistThreads), 0, st, fc, pp,
+                           scene.infos.as<pt_triangle_info>(), scene.bvh.nodes.as<float4>(),
+                           scene.bvh.tris.as<float4>(), scene.off.as<uint32_t>(), scene.lights.as<pt_dir_light>(),
+                           L.ps, cnt, rcnt, refill, ppool, std::min(64u, std::max(1u, render_tuning().shade_min)),
+                           io);
+    } else {
+        // Levels 0 and 1 of the pixel paths (a pixel path ends in the cache at x2).
+        for (uint32_t lvl = 0; lvl < 2; ++lvl) {
+            hipLaunchKernelGGL(extend, dim3(pgrid), dim3(kPersistThreads), 0, st, fc, pp,
+                               scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
+                               scene.off.as<uint32_t>(), L.q[lvl & 1], L.hq, cnt, rcnt, lvl, refill, pool);
+            hipLaunchKernelGGL(shade, dim3(shade_grid_for(npaths)), dim3(kShadeThreads), 0, st, fc, pp,
+                               scene.infos.as<pt_triangle_info>(), L.q[lvl & 1], L.hq,
+                               L.q[(lvl + 1) & 1], L.ps, cnt, lvl, io, scene.lights.as<pt_dir_light>(),
+                               scene.bvh.nodes.as<float4>(), scene.bvh.tris.as<float4>(),
+                               scene.off.as<uint32_t>());
+        }
     }
     cache.infer(io.qraw, fc.npix, io.qout, st);
     hipLaunchKernelGGL(k_cache_resolve, dim3(grid_for(fc.npix)), dim3(kThreads), 0, st, fc, io, L.ps);
@@ -1364,7 +1399,7 @@ void launch_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
 struct CacheGraphKey {
     FrameConst fc;
     CacheIO io;
-    uint32_t steps, train, blocks, refill, lds, pool;
+    uint32_t steps, train, blocks, refill, lds, pool, paths, shade_min;
     const void* ptrs[13];
     float box[6], adam[4];
     hipStream_t stream;
@@ -1440,6 +1475,8 @@ void render_cache_frame(Renderer& r, const DeviceScene& scene, RadianceCache& ca
         key.refill = trace_tuning().refill;
         key.lds = trace_tuning().lds;
         key.pool = trace_tuning().pool;
+        key.paths = render_tuning().paths;
+        key.shade_min = render_tuning().shade_min;
         const void* ptrs[13] = {scene.bvh.nodes.as<void>(), scene.bvh.tris.as<void>(), scene.infos.as<void>(),
                                 scene.off.as<void>(),       r.queues.as<void>(),     r.params.as<void>(),
                                 r.accum.as<void>(),         r.counters.as<void>(),   cache.master.as<void>(),

@@ -60,8 +60,18 @@ def rel_err(g, o):
     return np.abs(g - o).mean(axis=(0, 1)) / den
 
 
-@pytest.mark.parametrize("rr_start,strat", [(3, 1), (0, 0)])
-def test_cache_frame_matches_oracle(ctx, room, rr_start, strat):
+@pytest.mark.parametrize("rr_start,strat,paths", [(3, 1, 1), (0, 0, 1), (3, 1, 0)])
+def test_cache_frame_matches_oracle(ctx, room, rr_start, strat, paths):
+    """Cache frames against the oracle, with the pixel paths traced
+    path-persistently (render.paths 1) or as two levels (0)."""
+    ptamd.tune_set("render.paths", paths)
+    try:
+        check_cache_frames(ctx, room, rr_start, strat)
+    finally:
+        ptamd.tune_set("render.paths", 1)
+
+
+def check_cache_frames(ctx, room, rr_start, strat):
     st = ptamd.Settings(8, rr_start, 1.0, strat)
     bmin, bmax = setup(ctx, room, st)
     ctx.cache_init(5, bmin, bmax)
@@ -104,9 +114,17 @@ def test_cache_frame_ray_count(ctx, room):
     bmin, bmax = setup(ctx, room, st)
     ctx.cache_init(3, bmin, bmax)
     seed = 4242
-    ctx.reset_frames()
-    ctx.render_cache_frame(W, H, seed, ptamd.CacheSettings(0, 1, 8, train=False))
-    pixel_rays = ctx.last_frame_rays()
+    per_structure = []
+    for paths in (1, 0):
+        ptamd.tune_set("render.paths", paths)
+        try:
+            ctx.reset_frames()
+            ctx.render_cache_frame(W, H, seed, ptamd.CacheSettings(0, 1, 8, train=False))
+            per_structure.append(ctx.last_frame_rays())
+        finally:
+            ptamd.tune_set("render.paths", 1)
+    assert per_structure[0] == per_structure[1]  # path-persistent pixel paths count as the levels do
+    pixel_rays = per_structure[0]
     ctx.upload_scene(*room, settings=ptamd.Settings(1, 3, 1.0, 1))
     ctx.set_primary_rays(dirs_for(W, H), EYE)
     ctx.reset_frames()
@@ -150,18 +168,21 @@ def test_in_frame_training_equals_cache_train(ctx, room):
 def test_cache_frames_graph_equals_eager(ctx, room):
     st = ptamd.Settings(8, 3, 1.0, 1)
     out = {}
-    for graphs in (1, 0):
+    for graphs, paths in ((1, 1), (0, 1), (1, 0)):  # replays, eager frames, levels instead of paths
         ptamd.tune_set("render.graphs", graphs)
+        ptamd.tune_set("render.paths", paths)
         try:
             bmin, bmax = setup(ctx, room, st)
             ctx.cache_init(13, bmin, bmax)
             for f in range(5):
                 ctx.render_cache_frame(W, H, 500 + f, ptamd.CacheSettings(512, 2, 8, train=True))
-            out[graphs] = (ctx.read_accumulator(W, H)[0], ctx.cache_get_params())
+            out[(graphs, paths)] = (ctx.read_accumulator(W, H)[0], ctx.cache_get_params())
         finally:
             ptamd.tune_set("render.graphs", 1)
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+            ptamd.tune_set("render.paths", 1)
+    for key in ((0, 1), (1, 0)):
+        np.testing.assert_array_equal(out[key][0], out[(1, 1)][0])
+        np.testing.assert_array_equal(out[key][1], out[(1, 1)][1])
 
 
 def test_cache_frame_errors(ctx, room):
