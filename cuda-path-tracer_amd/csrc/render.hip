@@ -212,10 +212,13 @@ __device__ __forceinline__ V3 ld3(const float* p) { return v3(p[0], p[1], p[2]);
 // 12-20): a shadow ray from pos - dir * 1e-3 towards the light; any hit
 // (closest-hit t < FLT_MAX, trace.h:45-50) shadows it.  The any-hit search
 // answers the same question and stops at its first hit.
+template <bool LDSS = false>
 __device__ __forceinline__ V3 light_L(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                      uint32_t b, uint32_t e, V3 pos, const pt_dir_light& lt, uint32_t* status) {
+                                      uint32_t b, uint32_t e, V3 pos, const pt_dir_light& lt, uint32_t* status,
+                                      const ShadowStack& ss = ShadowStack{}) {
     const V3 ld = ld3(lt.direction);
-    if (any_hit_in(nodes, tris, b, e, sub(pos, scale(ld, 1e-3f)), neg(ld), status)) return v3(0.0f, 0.0f, 0.0f);
+    if (any_hit_in<LDSS>(nodes, tris, b, e, sub(pos, scale(ld, 1e-3f)), neg(ld), status, ss))
+        return v3(0.0f, 0.0f, 0.0f);
     return scale(ld3(lt.color), lt.intensity);
 }
 
@@ -310,11 +313,12 @@ __device__ __forceinline__ HitShade hit_shade(const FrameConst& fc, const pt_tri
 // throughput *= albedo, radiance += throughput * L * intensity.  LIGHTS:
 // whether the scene has directional lights (kept out of the no-light kernels,
 // whose registers and stack it would otherwise size).
-template <bool LIGHTS>
+template <bool LIGHTS, bool LDSS = false>
 __device__ __forceinline__ void bounce_hit(HitShade& hs, V3 d, const FrameConst& fc,
                                            const pt_dir_light* __restrict__ lights,
                                            const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                           uint32_t sb, uint32_t se, V3& absorbed, V3& acc) {
+                                           uint32_t sb, uint32_t se, V3& absorbed, V3& acc,
+                                           const ShadowStack& ss = ShadowStack{}) {
     V3 L = v3(0.0f, 0.0f, 0.0f);
     if (hs.emissive) {
         const float c = gmax(dot(neg(d), hs.nrm), 0.0f);
@@ -323,7 +327,7 @@ __device__ __forceinline__ void bounce_hit(HitShade& hs, V3 d, const FrameConst&
     } else if (LIGHTS) {
         for (uint32_t k = 0; k < fc.nlights; ++k) {
             const V3 ld = ld3(lights[k].direction);
-            L = add(L, scale(light_L(nodes, tris, sb, se, hs.pos, lights[k], fc.status),
+            L = add(L, scale(light_L<LDSS>(nodes, tris, sb, se, hs.pos, lights[k], fc.status, ss),
                              gmax(dot(neg(ld), hs.nrm), 0.0f)));
         }
     }
@@ -366,11 +370,12 @@ __device__ __forceinline__ bool next_bounce(uint32_t key, uint32_t pass, uint32_
 // lights with shadow rays, the light term L summed over the lights seen so
 // far (cuda_path_tracer.cu:194-214, 258-259); the path's throughput starts
 // at the hit's albedo (1 on emitters).
-template <bool LIGHTS>
+template <bool LIGHTS, bool LDSS = false>
 __device__ __forceinline__ void primary_hit(HitShade& hs, V3 d, const FrameConst& fc,
                                             const pt_dir_light* __restrict__ lights,
                                             const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                            uint32_t sb, uint32_t se, V3& absorbed, V3& acc) {
+                                            uint32_t sb, uint32_t se, V3& absorbed, V3& acc,
+                                            const ShadowStack& ss = ShadowStack{}) {
     acc = v3(0.0f, 0.0f, 0.0f);
     if (hs.emissive) {
         acc = hs.e;
@@ -379,7 +384,7 @@ __device__ __forceinline__ void primary_hit(HitShade& hs, V3 d, const FrameConst
         V3 Lc = v3(0.0f, 0.0f, 0.0f);
         for (uint32_t k = 0; k < fc.nlights; ++k) {
             const V3 ld = ld3(lights[k].direction);
-            Lc = add(Lc, light_L(nodes, tris, sb, se, hs.pos, lights[k], fc.status));
+            Lc = add(Lc, light_L<LDSS>(nodes, tris, sb, se, hs.pos, lights[k], fc.status, ss));
             const V3 hw = scale(sub(neg(d), ld), 0.5f);
             acc = add(acc, blinn_phong(Lc, hs.albedo, Lc, v3(0.5f, 0.5f, 0.5f), hs.nrm, neg(ld), hw, hs.spec));
         }
@@ -718,10 +723,17 @@ __global__ __launch_bounds__(kPersistThreads, 6) void k_paths(const FrameConst f
                         const PassParams& P = pp[pass];
                         HitShade hs = hit_shade<DETAIL>(fc, infos, r.best, r.minT, r.bu, r.bv, r.o, r.d);
                         const uint32_t sb = off[P.scene], se = off[P.scene + 1];
+                        // Shadow rays keep their deferred nodes in this lane's
+                        // (now empty) traversal stack.
+                        ShadowStack ss;
+                        ss.lds = &s_stack[0][threadIdx.x];
+                        ss.stride = kPersistThreads;
+                        ss.depth = LDS;
+                        ss.spill = spill;
                         if (bounce == 0) {
-                            primary_hit<LIGHTS>(hs, r.d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+                            primary_hit<LIGHTS, true>(hs, r.d, fc, lights, nodes, tris, sb, se, absorbed, acc, ss);
                         } else {
-                            bounce_hit<LIGHTS>(hs, r.d, fc, lights, nodes, tris, sb, se, absorbed, acc);
+                            bounce_hit<LIGHTS, true>(hs, r.d, fc, lights, nodes, tris, sb, se, absorbed, acc, ss);
                         }
                         if (CACHE && bounce == 1) record_cache_vertex(io, false, pix, path, hs, r.d, absorbed, acc);
                         V3 no{0.0f, 0.0f, 0.0f}, nd{0.0f, 0.0f, 0.0f};
@@ -771,6 +783,7 @@ __global__ __launch_bounds__(kPersistThreads, 6) void k_paths(const FrameConst f
 // record_cache_vertex, next_bounce), so the training batches are the same
 // bits as when these paths ran through the levels.
 constexpr int kSuffixThreads = 64;
+constexpr int kTrainLds = 16;  // traversal stack entries per lane kept in LDS
 
 template <bool DETAIL>
 __global__ __launch_bounds__(kSuffixThreads) void k_train_paths(const FrameConst fc,
@@ -782,6 +795,8 @@ __global__ __launch_bounds__(kSuffixThreads) void k_train_paths(const FrameConst
                                                                 const float4* __restrict__ nodes,
                                                                 const float4* __restrict__ tris,
                                                                 const uint32_t* __restrict__ off) {
+    __shared__ uint32_t s_tstack[kTrainLds][kSuffixThreads];
+    uint32_t spill[kStackMax - kTrainLds];
     const PassParams& P = pp[0];
     const uint32_t seedw = P.seedw ^ kTrainSalt;
     const uint32_t sb = off[P.scene], se = off[P.scene + 1];
@@ -793,7 +808,27 @@ __global__ __launch_bounds__(kSuffixThreads) void k_train_paths(const FrameConst
         V3 acc = v3(0.0f, 0.0f, 0.0f);
         atomicAdd(counters + kTrainLevel0Slot, 1u);
         for (uint32_t b = 0;; ++b) {
-            const TraceHit h = trace_ray_in(nodes, tris, sb, se, o, d, fc.status);
+            TraceHit h;
+            if (se - sb < 2) {
+                h = trace_ray_in(nodes, tris, sb, se, o, d, fc.status);
+            } else {
+                // The reference-order walk with its first kTrainLds deferred
+                // nodes in LDS (a lone wave per SIMD waits out every stack
+                // read; scratch would make each pop a memory round trip).
+                RayState r;
+                r.o = o;
+                r.d = d;
+                init_ray(r, sb);
+                TraceCount cnt{0, 0, 0, 0};
+                if (r.finite) {
+                    while (visit_step<false, kTrainLds, true>(nodes, tris, r, s_tstack, spill, cnt, fc.status)) {
+                    }
+                } else {
+                    while (visit_step<false, kTrainLds, false>(nodes, tris, r, s_tstack, spill, cnt, fc.status)) {
+                    }
+                }
+                h = final_hit(r);
+            }
             if (h.id == kInvalid) break;
             HitShade hs = hit_shade<DETAIL>(fc, infos, h.id, h.t, h.u, h.v, o, d);
             if (b == 0) {

@@ -192,14 +192,26 @@ __device__ __forceinline__ TraceHit trace_ray_in(const float4* __restrict__ node
 // the closest-hit search culls nothing (its bound is still the initial
 // one), so it visits the same boxes as this search does and finds a hit
 // exactly when this one does; this one stops at the first.
+// Where a lane keeps a shadow ray's deferred nodes: its column of an LDS
+// stack (entry k at lds[k * stride], the first `depth` entries) and a
+// private array for the rest; lds == nullptr: a private array throughout.
+struct ShadowStack {
+    uint32_t* lds = nullptr;
+    uint32_t stride = 0;
+    int depth = 0;
+    uint32_t* spill = nullptr;
+};
+
+template <bool LDSS = false>
 __device__ __forceinline__ bool any_hit_in(const float4* __restrict__ nodes, const float4* __restrict__ tris,
-                                          uint32_t b, uint32_t e, V3 o, V3 d, uint32_t* status) {
+                                          uint32_t b, uint32_t e, V3 o, V3 d, uint32_t* status,
+                                          const ShadowStack& ss = ShadowStack{}) {
     float u, v;
     uint32_t id;
     if (e - b == 0) return false;
     if (e - b == 1) return tri_hit(tris, b, o, d, u, v, id) <= kMinTInit;
     const V3 dinv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    uint32_t stack[kStackMax];
+    uint32_t stack[LDSS ? 1 : kStackMax];
     int sp = 0;
     uint32_t node = b;
     for (;;) {
@@ -215,14 +227,23 @@ __device__ __forceinline__ bool any_hit_in(const float4* __restrict__ nodes, con
         if ((!in0 && h0 <= kMinTInit) || (!in1 && h1 <= kMinTInit)) return true;
         const bool go0 = in0 && h0 <= kMinTInit, go1 = in1 && h1 <= kMinTInit;
         if (go0 && go1) {
-            if (sp < kStackMax) stack[sp++] = l + 1;
-            else raise_status(status, kStatusStackOverflow);
+            if constexpr (LDSS) {
+                if (sp < ss.depth) ss.lds[static_cast<uint32_t>(sp) * ss.stride] = l + 1;
+                else if (sp < kStackMax) ss.spill[sp - ss.depth] = l + 1;
+                else raise_status(status, kStatusStackOverflow);
+                if (sp < kStackMax) ++sp;
+            } else {
+                if (sp < kStackMax) stack[sp++] = l + 1;
+                else raise_status(status, kStatusStackOverflow);
+            }
             node = l;
         } else if (go0 || go1) {
             node = go0 ? l : l + 1;
         } else {
             if (sp == 0) return false;
-            node = stack[--sp];
+            --sp;
+            if constexpr (LDSS) node = sp < ss.depth ? ss.lds[static_cast<uint32_t>(sp) * ss.stride] : ss.spill[sp - ss.depth];
+            else node = stack[sp];
         }
     }
 }
@@ -249,10 +270,10 @@ struct RayState {
     bool finite;
 };
 
-template <bool COUNT, int LDS, bool FINITE>
+template <bool COUNT, int LDS, bool FINITE, int COLS = kPersistThreads>
 __device__ __forceinline__ bool visit_step(const float4* __restrict__ nodes,
                                            const float4* __restrict__ tris, RayState& r,
-                                           uint32_t (*s_stack)[kPersistThreads],
+                                           uint32_t (*s_stack)[COLS],
                                            uint32_t* spill, TraceCount& cnt, uint32_t* status) {
     const float4* rec = nodes + 4 * static_cast<size_t>(r.node);
     const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
