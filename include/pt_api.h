@@ -380,7 +380,8 @@ int pt_device_compute_units(int device, uint32_t* cus);
  *                   0: launch them one by one
  *   "render.paths"  1: a plain frame traces every path from its primary
  *                   ray to its end in one persistent launch (no bounce
- *                   levels); 0: bounce levels (wavefront)
+ *                   levels), and a cache frame its pixel paths from their
+ *                   primary ray to the cache; 0: bounce levels (wavefront)
  *   "render.shade_min" 1..64: in path-persistent frames, the lanes of a
  *                   wave whose segments have ended shade together once this
  *                   many wait (or none is traversing)
