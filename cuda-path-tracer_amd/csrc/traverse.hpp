@@ -186,12 +186,6 @@ __device__ __forceinline__ TraceHit trace_ray_in(const float4* __restrict__ node
     return traverse_impl<false, false>(nodes, tris, o, d, dinv, cnt, status, b);
 }
 
-// Whether a ray hits any triangle of the scene in [b, e) that the closest-hit
-// search would accept (the shadow-ray question of computeL,
-// LightAlgorithms.h:12-20: closest-hit t < FLT_MAX).  Until its first hit
-// the closest-hit search culls nothing (its bound is still the initial
-// one), so it visits the same boxes as this search does and finds a hit
-// exactly when this one does; this one stops at the first.
 // Where a lane keeps a shadow ray's deferred nodes: its column of an LDS
 // stack (entry k at lds[k * stride], the first `depth` entries) and a
 // private array for the rest; lds == nullptr: a private array throughout.
@@ -202,6 +196,14 @@ struct ShadowStack {
     uint32_t* spill = nullptr;
 };
 
+// Whether a ray hits any triangle of the scene in [b, e) that the closest-hit
+// search would accept (the shadow-ray question of computeL,
+// LightAlgorithms.h:12-20: closest-hit t < FLT_MAX).  Until its first hit
+// the closest-hit search culls nothing (its bound is still the initial
+// one), so it visits the same boxes as this search does and finds a hit
+// exactly when this one does; this one stops at the first.  LDSS: the
+// deferred nodes go to `ss` (a lane whose own traversal stack is empty
+// lends it to its shadow rays) instead of a private array.
 template <bool LDSS = false>
 __device__ __forceinline__ bool any_hit_in(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                                           uint32_t b, uint32_t e, V3 o, V3 d, uint32_t* status,
