@@ -6,7 +6,8 @@
 // off[s+1]-1 is unused), so each node's index lies inside its own leaf range.
 // Child and parent indices are global; the parity dump converts to local.
 //
-//   k_morton        30-bit Morton key (the scene's box) + index + scene id
+//   k_morton        30-bit Morton key (the scene's box) + index + scene id,
+//                   and the first sort pass's per-tile digit counts
 //   k_sort_hist     \
 //   k_sort_rowscan   } LSD radix sort, 10-bit digits, stable: 3 passes over
 //   k_sort_scatter  /  the 30-bit key, then 1-2 over the scene id when
@@ -32,12 +33,10 @@
 namespace pt {
 namespace {
 
-constexpr int kSortThreads = 256;
-constexpr int kSortKeysPerThread = 16;
-constexpr int kSortTile = kSortThreads * kSortKeysPerThread;  // 4096 keys
+constexpr int kSortTile = 4096;                                // keys per sort tile
 constexpr int kDigitBits = 10;                                 // 3 passes over a 30-bit key
 constexpr uint32_t kRadix = 1u << kDigitBits;                  // 1024 digits
-constexpr int kDigitsPerThread = kRadix / kSortThreads;        // 4
+constexpr int kScatterThreads = 512;                           // scatter: 8 keys per thread, 8 waves per tile
 // Refit levels.  A node's level is the smallest k whose block of sorted
 // leaves contains the node's whole leaf range:
 //   level 0: inside a 1024-leaf block    -> k_refit_local (LDS, one workgroup per block)
@@ -95,55 +94,53 @@ __device__ __forceinline__ uint32_t scene_of_slot(const uint32_t* __restrict__ o
 
 // morton_encoder (BVHBuilder.cu:12-32): centroid ((v0 + v1) + v2) / 3, mapped
 // into the scene's enclosing box with invScale computed on the host
-// (box[6*s..] = min.xyz, invScale.xyz).  Also zeroes the refit list counters.
-__global__ __launch_bounds__(256) void k_morton(const pt_triangle* __restrict__ tris, uint32_t n,
-                                                const uint32_t* __restrict__ off, uint32_t nscenes,
-                                                const float* __restrict__ box,
-                                                uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals,
-                                                uint32_t* __restrict__ scene_of,
-                                                uint32_t* __restrict__ span_count,
-                                                uint32_t n_span_counts) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_span_counts) span_count[i] = 0u;
-    if (i >= n) return;
-    const uint32_t sc = scene_of_slot(off, nscenes, i);
-    const float* bx = box + 6 * static_cast<size_t>(sc);
-    const float* t = reinterpret_cast<const float*>(tris + i);
-    float cx = ((t[0] + t[3]) + t[6]) / 3.0f;
-    float cy = ((t[1] + t[4]) + t[7]) / 3.0f;
-    float cz = ((t[2] + t[5]) + t[8]) / 3.0f;
-    uint32_t qx = quantize1024((cx - bx[0]) * bx[3]);
-    uint32_t qy = quantize1024((cy - bx[1]) * bx[4]);
-    uint32_t qz = quantize1024((cz - bx[2]) * bx[5]);
-    keys[i] = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
-    vals[i] = i;
-    scene_of[i] = sc;
+// (box[6*s..] = min.xyz, invScale.xyz).  One workgroup per 4096-key sort
+// tile, which also counts the tile's first-pass digits (key bits 0-9) into
+// its row of tile_hist -- the first sort pass then starts at the scan.  Also
+// zeroes the refit list counters.
+constexpr int kMortonThreads = 1024;
+
+__global__ __launch_bounds__(kMortonThreads) void k_morton(const pt_triangle* __restrict__ tris, uint32_t n,
+                                                           const uint32_t* __restrict__ off, uint32_t nscenes,
+                                                           const float* __restrict__ box,
+                                                           uint32_t* __restrict__ keys,
+                                                           uint32_t* __restrict__ vals,
+                                                           uint32_t* __restrict__ scene_of,
+                                                           uint32_t* __restrict__ tile_hist,
+                                                           uint32_t* __restrict__ span_count,
+                                                           uint32_t n_span_counts) {
+    __shared__ uint32_t h[kRadix];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = blockIdx.x * kMortonThreads + tid; i < n_span_counts; i += gridDim.x * kMortonThreads)
+        span_count[i] = 0u;
+    for (uint32_t d = tid; d < kRadix; d += kMortonThreads) h[d] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortTile / kMortonThreads; ++r) {
+        const uint32_t i = base + r * kMortonThreads + tid;
+        if (i >= n) break;
+        const uint32_t sc = scene_of_slot(off, nscenes, i);
+        const float* bx = box + 6 * static_cast<size_t>(sc);
+        const float* t = reinterpret_cast<const float*>(tris + i);
+        float cx = ((t[0] + t[3]) + t[6]) / 3.0f;
+        float cy = ((t[1] + t[4]) + t[7]) / 3.0f;
+        float cz = ((t[2] + t[5]) + t[8]) / 3.0f;
+        uint32_t qx = quantize1024((cx - bx[0]) * bx[3]);
+        uint32_t qy = quantize1024((cy - bx[1]) * bx[4]);
+        uint32_t qz = quantize1024((cz - bx[2]) * bx[5]);
+        const uint32_t key = spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2);
+        keys[i] = key;
+        vals[i] = i;
+        scene_of[i] = sc;
+        atomicAdd(&h[key & (kRadix - 1u)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t d = tid; d < kRadix; d += kMortonThreads)
+        tile_hist[static_cast<size_t>(blockIdx.x) * kRadix + d] = h[d];
 }
 
 // ---- radix sort ---------------------------------------------------------------
-
-// Exclusive scan of one value per thread over a 256-thread block (4 waves);
-// `total` receives the block sum.  Contains two barriers.
-__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_w,
-                                                        uint32_t& total) {
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) s_w[wave] = x;
-    __syncthreads();
-    uint32_t pre = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) pre += (k < wave) ? s_w[k] : 0u;
-    total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-    __syncthreads();
-    return pre + x - v;
-}
 
 // Digit of an element: bits [shift, shift+10) of its key, or -- in the
 // batched build's last passes -- of the scene id of its original slot.
@@ -155,8 +152,10 @@ __device__ __forceinline__ uint32_t sort_digit(uint32_t k, uint32_t v,
 
 // Per-tile digit histogram: tile_hist[tile * kRadix + digit] (one contiguous
 // row per tile).
+constexpr int kHistThreads = 1024;
+
 template <bool BY_SCENE>
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
+__global__ __launch_bounds__(kHistThreads) void k_sort_hist(const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ vals,
                                                             const uint32_t* __restrict__ scene_of,
                                                             uint32_t n, int shift,
@@ -164,21 +163,17 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __re
                                                             uint32_t* __restrict__ tile_hist) {
     __shared__ uint32_t h[kRadix];
     const int tid = threadIdx.x;
-#pragma unroll
-    for (int j = 0; j < kDigitsPerThread; ++j) h[tid + j * kSortThreads] = 0;
+    for (int d = tid; d < static_cast<int>(kRadix); d += kHistThreads) h[d] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int r = 0; r < kSortKeysPerThread; ++r) {
-        uint32_t i = base + r * kSortThreads + tid;
+#pragma unroll
+    for (int r = 0; r < kSortTile / kHistThreads; ++r) {
+        uint32_t i = base + r * kHistThreads + tid;
         if (i < n) atomicAdd(&h[sort_digit<BY_SCENE>(keys[i], BY_SCENE ? vals[i] : 0u, scene_of, shift)], 1u);
     }
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kDigitsPerThread; ++j) {
-        const int d = tid + j * kSortThreads;
+    for (int d = tid; d < static_cast<int>(kRadix); d += kHistThreads)
         tile_hist[static_cast<size_t>(blockIdx.x) * kRadix + d] = h[d];
-    }
 }
 
 // Exclusive scan of each digit's counts over the tiles (in place) and the
@@ -228,33 +223,64 @@ __global__ __launch_bounds__(kScanWaves * 64) void k_sort_rowscan(uint32_t* __re
     if (wave == kScanWaves - 1) digit_total[d] = run;
 }
 
-// Stable scatter of one 4096-key tile.  Each wave ranks 64 keys at a time
-// against the keys of its chunk so far: lanes holding the same digit are
-// found with ten ballots, their order is the popcount of lower peer lanes,
-// and a per-wave LDS counter per digit carries the count between rounds.
-// Keys are then staged in LDS in digit order and written out in runs.
-// Thread t owns digits 4t .. 4t+3 in the per-digit scans.
-template <bool BY_SCENE>
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+// Block-wide exclusive scan of one value per thread over WAVES waves;
+// `total` receives the block sum.  Contains two barriers.
+template <int WAVES>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0, sum = 0;
+#pragma unroll
+    for (int k = 0; k < WAVES; ++k) {
+        const uint32_t w = s_w[k];
+        pre += (k < wave) ? w : 0u;
+        sum += w;
+    }
+    total = sum;
+    __syncthreads();
+    return pre + x - v;
+}
+
+// Stable scatter of one 4096-key tile by THREADS threads.  Each wave ranks
+// 64 keys at a time against the keys of its chunk so far: lanes holding the
+// same digit are found with ten ballots, their order is the popcount of
+// lower peer lanes, and a per-wave LDS counter per digit carries the count
+// between rounds.  Keys are then staged in LDS in digit order and written
+// out in runs.  Thread t owns digits DPT*t .. DPT*t+DPT-1 in the per-digit
+// scans.
+template <bool BY_SCENE, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_sort_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
     const uint32_t* __restrict__ scene_of, uint32_t n, int shift,
     uint32_t num_tiles, const uint32_t* __restrict__ tile_hist,
     const uint32_t* __restrict__ digit_total) {
-    __shared__ uint32_t s_wc[4][kRadix];
+    constexpr int WAVES = THREADS / 64;
+    constexpr int KPT = kSortTile / THREADS;  // keys per thread
+    constexpr int DPT = kRadix / THREADS;     // digits per thread
+    static_assert(KPT * THREADS == kSortTile && DPT * THREADS == static_cast<int>(kRadix), "tile shape");
+    __shared__ uint32_t s_wc[WAVES][kRadix];
     __shared__ uint32_t s_dstart[kRadix];
     __shared__ uint32_t s_gbase[kRadix];
     __shared__ uint32_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[kSortTile];
-    __shared__ uint32_t s_w[4];
+    __shared__ uint32_t s_w[WAVES];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < WAVES; ++k)
 #pragma unroll
-        for (int j = 0; j < kDigitsPerThread; ++j) s_wc[k][tid + j * kSortThreads] = 0;
+        for (int j = 0; j < DPT; ++j) s_wc[k][tid + j * THREADS] = 0;
     __syncthreads();
 
     const uint32_t tile = blockIdx.x;
@@ -262,15 +288,20 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t count = min(static_cast<uint32_t>(kSortTile), n - base);
     const uint64_t lt = (1ull << lane) - 1ull;
 
-    uint32_t key[kSortKeysPerThread], val[kSortKeysPerThread], rk[kSortKeysPerThread];
-    uint32_t dig[kSortKeysPerThread];
+    uint32_t key[KPT], val[KPT], rk[KPT];
+    uint32_t dig[KPT];
 #pragma unroll
-    for (int r = 0; r < kSortKeysPerThread; ++r) {
-        const uint32_t li = wave * (kSortKeysPerThread * 64) + r * 64 + lane;
+    for (int r = 0; r < KPT; ++r) {
+        const uint32_t li = wave * (KPT * 64) + r * 64 + lane;
         const bool valid = li < count;
-        const uint32_t k = valid ? keys_in[base + li] : 0u;
-        const uint32_t v = valid ? vals_in[base + li] : 0u;
-        const uint32_t d = valid ? sort_digit<BY_SCENE>(k, v, scene_of, shift) : 0u;
+        key[r] = valid ? keys_in[base + li] : 0u;
+        val[r] = valid ? vals_in[base + li] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < KPT; ++r) {
+        const uint32_t li = wave * (KPT * 64) + r * 64 + lane;
+        const bool valid = li < count;
+        const uint32_t d = valid ? sort_digit<BY_SCENE>(key[r], val[r], scene_of, shift) : 0u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < kDigitBits; ++b) {
@@ -284,35 +315,35 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
             old = s_wc[wave][d];
             if (rank == 0) s_wc[wave][d] = old + static_cast<uint32_t>(__popcll(peers));
         }
-        key[r] = k;
-        val[r] = v;
         dig[r] = d;
         rk[r] = old + rank;
     }
     __syncthreads();
 
-    // Digits 4t..4t+3: offsets of each wave's run, tile-local start, global base.
-    uint32_t tot[kDigitsPerThread], gt[kDigitsPerThread];
+    // Digits DPT*t ..: offsets of each wave's run, tile-local start, global base.
+    uint32_t tot[DPT], gt[DPT];
     uint32_t tsum = 0, gsum = 0;
 #pragma unroll
-    for (int j = 0; j < kDigitsPerThread; ++j) {
-        const int d = kDigitsPerThread * tid + j;
-        const uint32_t c0 = s_wc[0][d], c1 = s_wc[1][d], c2 = s_wc[2][d], c3 = s_wc[3][d];
-        tot[j] = c0 + c1 + c2 + c3;
+    for (int j = 0; j < DPT; ++j) {
+        const int d = DPT * tid + j;
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < WAVES; ++k) {  // each digit's words are read and rewritten by its own thread only
+            const uint32_t c = s_wc[k][d];
+            s_wc[k][d] = run;
+            run += c;
+        }
+        tot[j] = run;
         gt[j] = digit_total[d];
         tsum += tot[j];
         gsum += gt[j];
-        s_wc[0][d] = 0;  // each digit's words are read and rewritten by its own thread only
-        s_wc[1][d] = c0;
-        s_wc[2][d] = c0 + c1;
-        s_wc[3][d] = c0 + c1 + c2;
     }
     uint32_t tmp;
-    uint32_t dstart = block_excl_scan_256(tsum, s_w, tmp);
-    uint32_t gpre = block_excl_scan_256(gsum, s_w, tmp);
+    uint32_t dstart = block_excl_scan<WAVES>(tsum, s_w, tmp);
+    uint32_t gpre = block_excl_scan<WAVES>(gsum, s_w, tmp);
 #pragma unroll
-    for (int j = 0; j < kDigitsPerThread; ++j) {
-        const int d = kDigitsPerThread * tid + j;
+    for (int j = 0; j < DPT; ++j) {
+        const int d = DPT * tid + j;
         s_dstart[d] = dstart;
         s_gbase[d] = gpre + tile_hist[static_cast<size_t>(tile) * kRadix + d];
         dstart += tot[j];
@@ -321,8 +352,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     __syncthreads();
 
 #pragma unroll
-    for (int r = 0; r < kSortKeysPerThread; ++r) {
-        const uint32_t li = wave * (kSortKeysPerThread * 64) + r * 64 + lane;
+    for (int r = 0; r < KPT; ++r) {
+        const uint32_t li = wave * (KPT * 64) + r * 64 + lane;
         if (li < count) {
             const uint32_t d = dig[r];
             const uint32_t pos = s_dstart[d] + s_wc[wave][d] + rk[r];
@@ -331,7 +362,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
         }
     }
     __syncthreads();
-    for (uint32_t i = tid; i < count; i += kSortThreads) {
+    for (uint32_t i = tid; i < count; i += THREADS) {
         const uint32_t k = s_keys[i];
         const uint32_t v = s_vals[i];
         const uint32_t d = sort_digit<BY_SCENE>(k, v, scene_of, shift);
@@ -885,10 +916,10 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
 
     uint32_t* kin = b.keys.as<uint32_t>();
     uint32_t* vin = b.vals.as<uint32_t>();
-    hipLaunchKernelGGL(k_morton, dim3(div_up(std::max<uint64_t>(n, n_lists1 + mid_blocks), 256)), dim3(256), 0, st,
-                       s.tris.as<pt_triangle>(), n, s.off.as<uint32_t>(), nscenes,
-                       s.morton_box.as<float>(), kin, vin, b.scene_of.as<uint32_t>(),
-                       b.span_count.as<uint32_t>(), n_lists1 + mid_blocks);
+    hipLaunchKernelGGL(k_morton, dim3(num_tiles), dim3(kMortonThreads), 0, st, s.tris.as<pt_triangle>(), n,
+                       s.off.as<uint32_t>(), nscenes, s.morton_box.as<float>(), kin, vin,
+                       b.scene_of.as<uint32_t>(), b.tile_hist.as<uint32_t>(), b.span_count.as<uint32_t>(),
+                       n_lists1 + mid_blocks);
     PT_HIP(hipGetLastError());
 
     if (n >= 2) {
@@ -903,21 +934,22 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
         for (int pass = 0; pass < passes; ++pass) {
             const bool by_scene = pass >= kKeyPasses;
             const int shift = kDigitBits * (by_scene ? pass - kKeyPasses : pass);
+            // The first pass's counts come from k_morton.
             if (by_scene) {
-                hipLaunchKernelGGL(k_sort_hist<true>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                hipLaunchKernelGGL(k_sort_hist<true>, dim3(num_tiles), dim3(kHistThreads), 0, st, kin, vin,
                                    b.scene_of.as<uint32_t>(), n, shift, num_tiles, b.tile_hist.as<uint32_t>());
-            } else {
-                hipLaunchKernelGGL(k_sort_hist<false>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+            } else if (pass > 0) {
+                hipLaunchKernelGGL(k_sort_hist<false>, dim3(num_tiles), dim3(kHistThreads), 0, st, kin, vin,
                                    b.scene_of.as<uint32_t>(), n, shift, num_tiles, b.tile_hist.as<uint32_t>());
             }
             hipLaunchKernelGGL(k_sort_rowscan, dim3(kRadix / kScanDigits), dim3(kScanWaves * 64), 0, st,
                                b.tile_hist.as<uint32_t>(), num_tiles, b.digit_total.as<uint32_t>());
             if (by_scene) {
-                hipLaunchKernelGGL(k_sort_scatter<true>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                hipLaunchKernelGGL((k_sort_scatter<true, kScatterThreads>), dim3(num_tiles), dim3(kScatterThreads), 0, st, kin, vin,
                                    kout, vout, b.scene_of.as<uint32_t>(), n, shift, num_tiles,
                                    b.tile_hist.as<uint32_t>(), b.digit_total.as<uint32_t>());
             } else {
-                hipLaunchKernelGGL(k_sort_scatter<false>, dim3(num_tiles), dim3(kSortThreads), 0, st, kin, vin,
+                hipLaunchKernelGGL((k_sort_scatter<false, kScatterThreads>), dim3(num_tiles), dim3(kScatterThreads), 0, st, kin, vin,
                                    kout, vout, b.scene_of.as<uint32_t>(), n, shift, num_tiles,
                                    b.tile_hist.as<uint32_t>(), b.digit_total.as<uint32_t>());
             }
