@@ -544,7 +544,10 @@ struct LdsBoxes {
 // only workgroup-scope ordering.  The thread completing a node writes the
 // node's whole traversal record (both child boxes, leaves' included, and the
 // child word); a box whose parent is finished by a later phase is also
-// stored into that parent's record.
+// stored into that parent's record.  A node inside the block has its index
+// inside its leaf range, so the block's nodes are the indices [base, base +
+// kRefitBlock): their child words, parents and levels are loaded into LDS
+// once, coalesced, and the climb reads no global memory.
 __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     const pt_triangle* __restrict__ in_tris, const uint32_t* __restrict__ perm,
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ scene_of,
@@ -553,14 +556,26 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     float4* __restrict__ nodes, float4* __restrict__ tri_recs, float* __restrict__ root_box,
     uint32_t n) {
     __shared__ uint32_t s_cnt[kRefitBlock];
+    __shared__ uint32_t s_word[kRefitBlock];   // child word of node base + j
+    __shared__ uint32_t s_par[kRefitBlock];    // its parent
+    __shared__ uint32_t s_local[kRefitBlock];  // 1: refit here (level 0)
     __shared__ LdsBoxes s_leaf;
     __shared__ LdsBoxes s_node;
     const int tid = threadIdx.x;
     const uint32_t base = blockIdx.x * kRefitBlock;
     const uint32_t k = base + tid;
     s_cnt[tid] = 0u;
+    if (k < n) {
+        s_word[tid] = children[k];
+        s_par[tid] = parents_internal[k];
+        s_local[tid] = (counters[k] >> kLevelShift) == 0u ? 1u : 0u;
+    } else {
+        s_local[tid] = 0u;
+    }
     __syncthreads();
     if (k >= n) return;
+    // Node q is refit by this block: inside [base, base + kRefitBlock) and of level 0.
+    auto local = [&](uint32_t q) { return q - base < static_cast<uint32_t>(kRefitBlock) && s_local[q - base]; };
     const pt_triangle* t = in_tris + perm[k];
     const V3 v0 = v3(t->v0[0], t->v0[1], t->v0[2]);
     const V3 v1 = v3(t->v1[0], t->v1[1], t->v1[2]);
@@ -586,7 +601,7 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     }
     s_leaf.put(tid, b);
     uint32_t p = parents_leaf[k];
-    if (counters[p] >> kLevelShift) {
+    if (!local(p)) {
         store_slot(nodes, p, (children[p] >> 2) == k, b);  // finished by a later phase
         return;
     }
@@ -594,19 +609,19 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
         const uint32_t old = __hip_atomic_fetch_add(s_cnt + (p - base), 1u, __ATOMIC_ACQ_REL,
                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
         if (old == 0u) return;
-        const uint32_t w = children[p];
+        const uint32_t w = s_word[p - base];
         const uint32_t l = w >> 2, r = l + 1;
         const Box lb = (w & 1u) ? s_node.get(l - base) : s_leaf.get(l - base);
         const Box rb = (w & 2u) ? s_node.get(r - base) : s_leaf.get(r - base);
         b = unite(lb, rb);
         s_node.put(p - base, b);
         store_record(nodes, p, lb, rb, w);
-        const uint32_t pp = parents_internal[p];
+        const uint32_t pp = s_par[p - base];
         if (pp == kInvalid) {
             store_root(root_box + 6 * static_cast<size_t>(scene_of[p]), b);
             return;
         }
-        if (counters[pp] >> kLevelShift) {  // finished by a later phase, from its record
+        if (!local(pp)) {  // finished by a later phase, from its record
             store_slot(nodes, pp, (children[pp] >> 2) == p, b);
             return;
         }
