@@ -693,35 +693,45 @@ struct LevelStage {
     uint32_t wait[kStageCap];  // same-level children not yet complete
 };
 
-__device__ __forceinline__ void refit_level_staged(LevelStage& S, const uint32_t* __restrict__ children,
-                                                   const uint32_t* __restrict__ scene_of,
-                                                   const uint32_t* __restrict__ parents_internal,
-                                                   const uint32_t* __restrict__ counters,
-                                                   const uint32_t* __restrict__ slots, const uint32_t* list_start,
-                                                   float4* __restrict__ nodes, float* __restrict__ root_box,
-                                                   const uint32_t* list, uint32_t count, uint32_t level) {
-    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
-        const uint32_t s = list[idx];
-        const float4* rec = nodes + 4 * static_cast<size_t>(s);
-        const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-        const float v[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-        for (int k = 0; k < 12; ++k) S.box[k][idx] = v[k];
-        const uint32_t w = children[s];
-        const uint32_t l = w >> 2, r = l + 1;
-        S.wait[idx] = static_cast<uint32_t>((w & 1u) && (counters[l] >> kLevelShift) == level) +
-                      static_cast<uint32_t>((w & 2u) && (counters[r] >> kLevelShift) == level);
-        S.node[idx] = s;
-        S.word[idx] = w;
-        const uint32_t pp = parents_internal[s];
-        S.par[idx] = pp;
-        uint32_t up = kUpOutside;
-        if (pp != kInvalid && (counters[pp] >> kLevelShift) == level) {
-            const uint32_t sl = slots[pp];
-            up = ((list_start[sl >> 6] + (sl & 63u)) << 1) | static_cast<uint32_t>((children[pp] >> 2) != s);
-        }
-        S.up[idx] = up;
+// A listed node's links: its child word, parent, the parent's slot in the
+// same list (or kUpOutside) and its number of same-level children.
+__device__ __forceinline__ void stage_links(LevelStage& S, uint32_t idx, uint32_t s,
+                                            const uint32_t* __restrict__ children,
+                                            const uint32_t* __restrict__ parents_internal,
+                                            const uint32_t* __restrict__ counters,
+                                            const uint32_t* __restrict__ slots, const uint32_t* list_start,
+                                            uint32_t level) {
+    const uint32_t w = children[s];
+    const uint32_t l = w >> 2, r = l + 1;
+    S.wait[idx] = static_cast<uint32_t>((w & 1u) && (counters[l] >> kLevelShift) == level) +
+                  static_cast<uint32_t>((w & 2u) && (counters[r] >> kLevelShift) == level);
+    S.node[idx] = s;
+    S.word[idx] = w;
+    const uint32_t pp = parents_internal[s];
+    S.par[idx] = pp;
+    uint32_t up = kUpOutside;
+    if (pp != kInvalid && (counters[pp] >> kLevelShift) == level) {
+        const uint32_t sl = slots[pp];
+        up = ((list_start[sl >> 6] + (sl & 63u)) << 1) | static_cast<uint32_t>((children[pp] >> 2) != s);
     }
-    __syncthreads();
+    S.up[idx] = up;
+}
+
+// A listed node's two child boxes, from its record (stored by earlier phases).
+__device__ __forceinline__ void stage_boxes(LevelStage& S, uint32_t idx, uint32_t s,
+                                            const float4* __restrict__ nodes) {
+    const float4* rec = nodes + 4 * static_cast<size_t>(s);
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    const float v[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+    for (int k = 0; k < 12; ++k) S.box[k][idx] = v[k];
+}
+
+// The climb over a staged list (after a barrier): every node with no
+// same-level child starts one; the thread completing a node's second
+// same-level child goes on to the node.
+__device__ __forceinline__ void climb_staged(LevelStage& S, const uint32_t* __restrict__ children,
+                                             const uint32_t* __restrict__ scene_of, float4* __restrict__ nodes,
+                                             float* __restrict__ root_box, uint32_t count) {
     for (uint32_t first = threadIdx.x; first < count; first += blockDim.x) {
         if (S.wait[first] != 0u) continue;
         uint32_t idx = first;
@@ -751,6 +761,22 @@ __device__ __forceinline__ void refit_level_staged(LevelStage& S, const uint32_t
             idx = ps;
         }
     }
+}
+
+__device__ __forceinline__ void refit_level_staged(LevelStage& S, const uint32_t* __restrict__ children,
+                                                   const uint32_t* __restrict__ scene_of,
+                                                   const uint32_t* __restrict__ parents_internal,
+                                                   const uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ slots, const uint32_t* list_start,
+                                                   float4* __restrict__ nodes, float* __restrict__ root_box,
+                                                   const uint32_t* list, uint32_t count, uint32_t level) {
+    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
+        const uint32_t s = list[idx];
+        stage_boxes(S, idx, s, nodes);
+        stage_links(S, idx, s, children, parents_internal, counters, slots, list_start, level);
+    }
+    __syncthreads();
+    climb_staged(S, children, scene_of, nodes, root_box, count);
 }
 
 // The lists of one level, concatenated in LDS-held order: list k starts at
@@ -798,7 +824,16 @@ __device__ __forceinline__ uint32_t concat_lists(const uint32_t* __restrict__ co
     return total;
 }
 
-// Level-1 nodes: one workgroup per 65536-leaf block (its 64 lists).
+// The level-2 stage prepared by k_refit_mid's last workgroup for k_refit_top:
+// [0] the list's length, then per listed node its child word, parent,
+// parent's slot and same-level child count (kStageCap each).
+constexpr uint32_t kTopLinksWords = 1 + 4 * kStageCap;
+
+// Level-1 nodes: one workgroup per 65536-leaf block (its 64 lists).  With
+// level-2 nodes present, one more workgroup concatenates their lists into
+// top_list and stages their links (children, parents, counters and slots,
+// none of which this launch writes) into top_links, so that k_refit_top
+// only has to gather the child boxes.
 __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__ children,
                                                     const uint32_t* __restrict__ scene_of,
                                                     const uint32_t* __restrict__ parents_internal,
@@ -807,12 +842,30 @@ __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__
                                                     float* __restrict__ root_box,
                                                     const uint32_t* __restrict__ span_list,
                                                     const uint32_t* __restrict__ span_count, uint32_t n_lists1,
+                                                    const uint32_t* __restrict__ span_top, uint32_t n_lists2,
+                                                    uint32_t* __restrict__ top_list, uint32_t* __restrict__ top_links,
                                                     const uint32_t* __restrict__ slots, uint32_t stage_max,
                                                     uint32_t* __restrict__ status) {
     constexpr uint32_t kLists = kMidBlock / kRefitBlock;
     __shared__ LevelStage S;
-    __shared__ uint32_t s_list[kMidCap];
-    __shared__ uint32_t s_start[kLists + 1];
+    __shared__ uint32_t s_buf[kMaxMidBlocks + 1 > kMidCap + kLists + 1 ? kMaxMidBlocks + 1 : kMidCap + kLists + 1];
+    const uint32_t n_mid = (n_lists1 + kLists - 1) / kLists;
+    if (blockIdx.x == n_mid) {  // the level-2 preparation
+        uint32_t* s_start = s_buf;
+        const uint32_t count = concat_lists(span_count + n_lists1, n_lists2, span_top, s_start, top_list, status);
+        if (threadIdx.x == 0) top_links[0] = count;
+        if (count > stage_max) return;
+        for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
+            stage_links(S, idx, top_list[idx], children, parents_internal, counters, slots, s_start, 2u);
+            top_links[1 + idx] = S.word[idx];
+            top_links[1 + kStageCap + idx] = S.par[idx];
+            top_links[1 + 2 * kStageCap + idx] = S.up[idx];
+            top_links[1 + 3 * kStageCap + idx] = S.wait[idx];
+        }
+        return;
+    }
+    uint32_t* s_list = s_buf;
+    uint32_t* s_start = s_buf + kMidCap;
     const uint32_t k0 = blockIdx.x * kLists;
     const uint32_t count = concat_lists(span_count + k0, min(kLists, n_lists1 - k0),
                                         span_list + static_cast<size_t>(k0) * kListCap, s_start, s_list, status);
@@ -824,25 +877,29 @@ __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__
     }
 }
 
-// Level-2 nodes: one workgroup (the lists of all 65536-leaf blocks,
-// concatenated into top_list).
+// Level-2 nodes: one workgroup, over the list and links k_refit_mid prepared.
 __global__ __launch_bounds__(1024) void k_refit_top(const uint32_t* __restrict__ children,
                                                     const uint32_t* __restrict__ scene_of,
                                                     const uint32_t* __restrict__ parents_internal,
                                                     uint32_t* __restrict__ counters,
                                                     float4* __restrict__ nodes,
                                                     float* __restrict__ root_box,
-                                                    const uint32_t* __restrict__ span_top,
-                                                    const uint32_t* __restrict__ span_count, uint32_t n_lists2,
-                                                    uint32_t* __restrict__ top_list,
-                                                    const uint32_t* __restrict__ slots, uint32_t stage_max,
-                                                    uint32_t* __restrict__ status) {
+                                                    const uint32_t* __restrict__ top_list,
+                                                    const uint32_t* __restrict__ top_links, uint32_t stage_max) {
     __shared__ LevelStage S;
-    __shared__ uint32_t s_start[kMaxMidBlocks + 1];
-    const uint32_t count = concat_lists(span_count, n_lists2, span_top, s_start, top_list, status);
+    const uint32_t count = top_links[0];
     if (count <= stage_max) {
-        refit_level_staged(S, children, scene_of, parents_internal, counters, slots, s_start, nodes, root_box,
-                           top_list, count, 2u);
+        for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
+            const uint32_t s = top_list[idx];
+            S.node[idx] = s;
+            S.word[idx] = top_links[1 + idx];
+            S.par[idx] = top_links[1 + kStageCap + idx];
+            S.up[idx] = top_links[1 + 2 * kStageCap + idx];
+            S.wait[idx] = top_links[1 + 3 * kStageCap + idx];
+            stage_boxes(S, idx, s, nodes);
+        }
+        __syncthreads();
+        climb_staged(S, children, scene_of, nodes, root_box, count);
     } else {
         refit_level(children, scene_of, parents_internal, counters, nodes, root_box, top_list, count, 2u);
     }
@@ -927,6 +984,7 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     b.span_list.reserve(static_cast<size_t>(n_lists1) * kListCap * 4);
     b.span_top.reserve(static_cast<size_t>(mid_blocks) * kListCap * 4 * 2);  // lists, then their concatenation
     b.span_count.reserve((static_cast<size_t>(n_lists1) + mid_blocks) * 4);
+    b.top_links.reserve(kTopLinksWords * 4);
     b.slots.reserve(n4);
 
     uint32_t* kin = b.keys.as<uint32_t>();
@@ -993,19 +1051,21 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                        b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.tris.as<float4>(),
                        b.root_box.as<float>(), n);
     if (n > static_cast<uint32_t>(kRefitBlock)) {
-        hipLaunchKernelGGL(k_refit_mid, dim3(mid_blocks), dim3(1024), 0, st, b.children.as<uint32_t>(),
+        // One more workgroup prepares the level-2 refit when there is one.
+        const uint32_t top = n > kMidBlock ? 1u : 0u;
+        uint32_t* top_list = b.span_top.as<uint32_t>() + static_cast<size_t>(mid_blocks) * kListCap;
+        hipLaunchKernelGGL(k_refit_mid, dim3(mid_blocks + top), dim3(1024), 0, st, b.children.as<uint32_t>(),
                            b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                            b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(),
                            b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>(), n_lists1,
+                           b.span_top.as<uint32_t>(), mid_blocks, top_list, b.top_links.as<uint32_t>(),
                            b.slots.as<uint32_t>(), stage_max, s.status.as<uint32_t>());
-    }
-    if (n > kMidBlock) {
-        hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
-                           b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
-                           b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(),
-                           b.span_top.as<uint32_t>(), b.span_count.as<uint32_t>() + n_lists1, mid_blocks,
-                           b.span_top.as<uint32_t>() + static_cast<size_t>(mid_blocks) * kListCap,
-                           b.slots.as<uint32_t>(), stage_max, s.status.as<uint32_t>());
+        if (top) {
+            hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
+                               b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
+                               b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(), top_list,
+                               b.top_links.as<uint32_t>(), stage_max);
+        }
     }
     PT_HIP(hipGetLastError());
 }
