@@ -109,7 +109,7 @@ struct DeviceBvh {
     DevBuf nodes, tris, root_box;
     DevBuf tile_hist, digit_total;  // radix sort scratch
     DevBuf span_list, span_top, span_count;  // refit: per-level node lists
-    DevBuf top_links;                        // refit: the level-2 list's staged links
+    DevBuf mid_links, top_links;             // refit: the level-1 and level-2 lists' staged links
     DevBuf slots;                            // refit: each listed node's position in its list
 };
 

@@ -534,6 +534,217 @@ struct LdsBoxes {
     }
 };
 
+// Refit of the level-L nodes listed in `list` (count at *count), all inside
+// one workgroup so the counter hand-off needs only workgroup-scope ordering.
+// Every child box of a node sits in the node's own record: lower-level
+// children were stored by earlier phases, same-level children are stored here
+// before their parent's counter is incremented.  Each lower-level child of a
+// listed node is one "arrival"; the thread completing a node climbs on while
+// the parent is of the same level.
+__device__ __forceinline__ void refit_level(const uint32_t* __restrict__ children,
+                                            const uint32_t* __restrict__ scene_of,
+                                            const uint32_t* __restrict__ parents_internal,
+                                            uint32_t* __restrict__ counters,
+                                            float4* __restrict__ nodes,
+                                            float* __restrict__ root_box,
+                                            const uint32_t* list, uint32_t count, uint32_t level) {
+    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
+        const uint32_t s = list[idx];
+        const uint32_t w = children[s];
+        const uint32_t l = w >> 2, r = l + 1;
+        int arrivals = 0;
+        if (!(w & 1u) || (counters[l] >> kLevelShift) < level) ++arrivals;
+        if (!(w & 2u) || (counters[r] >> kLevelShift) < level) ++arrivals;
+        for (int a = 0; a < arrivals; ++a) {
+            uint32_t p = s;
+            for (;;) {
+                const uint32_t old = __hip_atomic_fetch_add(counters + p, 1u, __ATOMIC_ACQ_REL,
+                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+                if ((old & kCountMask) == 0u) break;
+                const float4* rec = nodes + 4 * static_cast<size_t>(p);
+                const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+                const Box lb{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y};
+                const Box rb{r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+                const Box b = unite(lb, rb);
+                nodes[4 * static_cast<size_t>(p) + 3] = make_float4(__uint_as_float(children[p]), 0.0f, 0.0f, 0.0f);
+                const uint32_t pp = parents_internal[p];
+                if (pp == kInvalid) {
+                    store_root(root_box + 6 * static_cast<size_t>(scene_of[p]), b);
+                    break;
+                }
+                store_slot(nodes, pp, (children[pp] >> 2) == p, b);
+                if ((counters[pp] >> kLevelShift) != level) break;
+                p = pp;
+            }
+        }
+    }
+}
+
+// The same refit with the list staged in LDS (lists of up to kStageCap
+// nodes): every listed node's record (the child boxes the earlier phases
+// stored), its parent and its count of same-level children are gathered
+// first, so the climb itself touches only LDS.  Each completed node's
+// record is written whole; a box whose parent belongs to a later phase also
+// goes into that parent's record.
+constexpr uint32_t kStageCap = 1024;
+constexpr uint32_t kUpOutside = 0xffffffffu;  // parent handled by a later phase, or none
+
+struct LevelStage {
+    float box[12][kStageCap];  // left child box, then right, in record order
+    uint32_t node[kStageCap];  // the listed node
+    uint32_t word[kStageCap];  // its child word
+    uint32_t par[kStageCap];   // its parent (kInvalid at a scene root)
+    uint32_t up[kStageCap];    // parent's slot << 1 | right side, or kUpOutside
+    uint32_t wait[kStageCap];  // same-level children not yet complete
+};
+
+// A listed node's links: its child word, parent, the parent's slot in the
+// same list (or kUpOutside) and its number of same-level children.
+struct Links {
+    uint32_t word, par, up, wait;
+};
+
+__device__ __forceinline__ Links node_links(uint32_t s, const uint32_t* __restrict__ children,
+                                            const uint32_t* __restrict__ parents_internal,
+                                            const uint32_t* __restrict__ counters,
+                                            const uint32_t* __restrict__ slots, const uint32_t* list_start,
+                                            uint32_t level) {
+    Links L;
+    L.word = children[s];
+    const uint32_t l = L.word >> 2, r = l + 1;
+    L.wait = static_cast<uint32_t>((L.word & 1u) && (counters[l] >> kLevelShift) == level) +
+             static_cast<uint32_t>((L.word & 2u) && (counters[r] >> kLevelShift) == level);
+    L.par = parents_internal[s];
+    L.up = kUpOutside;
+    if (L.par != kInvalid && (counters[L.par] >> kLevelShift) == level) {
+        const uint32_t sl = slots[L.par];
+        L.up = ((list_start[sl >> 6] + (sl & 63u)) << 1) | static_cast<uint32_t>((children[L.par] >> 2) != s);
+    }
+    return L;
+}
+
+__device__ __forceinline__ void stage_links(LevelStage& S, uint32_t idx, uint32_t s, const Links& L) {
+    S.node[idx] = s;
+    S.word[idx] = L.word;
+    S.par[idx] = L.par;
+    S.up[idx] = L.up;
+    S.wait[idx] = L.wait;
+}
+
+// A listed node's two child boxes, from its record (stored by earlier phases).
+__device__ __forceinline__ void stage_boxes(LevelStage& S, uint32_t idx, uint32_t s,
+                                            const float4* __restrict__ nodes) {
+    const float4* rec = nodes + 4 * static_cast<size_t>(s);
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    const float v[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+    for (int k = 0; k < 12; ++k) S.box[k][idx] = v[k];
+}
+
+// The climb over a staged list (after a barrier): every node with no
+// same-level child starts one; the thread completing a node's second
+// same-level child goes on to the node.
+__device__ __forceinline__ void climb_staged(LevelStage& S, const uint32_t* __restrict__ children,
+                                             const uint32_t* __restrict__ scene_of, float4* __restrict__ nodes,
+                                             float* __restrict__ root_box, uint32_t count) {
+    for (uint32_t first = threadIdx.x; first < count; first += blockDim.x) {
+        if (S.wait[first] != 0u) continue;
+        uint32_t idx = first;
+        for (;;) {
+            const Box lb{S.box[0][idx], S.box[1][idx], S.box[2][idx], S.box[3][idx], S.box[4][idx], S.box[5][idx]};
+            const Box rb{S.box[6][idx], S.box[7][idx], S.box[8][idx], S.box[9][idx], S.box[10][idx], S.box[11][idx]};
+            const Box b = unite(lb, rb);
+            const uint32_t s = S.node[idx];
+            store_record(nodes, s, lb, rb, S.word[idx]);
+            const uint32_t pp = S.par[idx];
+            if (pp == kInvalid) {
+                store_root(root_box + 6 * static_cast<size_t>(scene_of[s]), b);
+                break;
+            }
+            const uint32_t up = S.up[idx];
+            if (up == kUpOutside) {
+                store_slot(nodes, pp, (children[pp] >> 2) == s, b);
+                break;
+            }
+            const uint32_t ps = up >> 1;
+            const int o = (up & 1u) ? 6 : 0;
+            S.box[o + 0][ps] = b.mnx; S.box[o + 1][ps] = b.mxx; S.box[o + 2][ps] = b.mny;
+            S.box[o + 3][ps] = b.mxy; S.box[o + 4][ps] = b.mnz; S.box[o + 5][ps] = b.mxz;
+            const uint32_t old = __hip_atomic_fetch_sub(&S.wait[ps], 1u, __ATOMIC_ACQ_REL,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old != 1u) break;  // the other same-level child completes the parent
+            idx = ps;
+        }
+    }
+}
+
+__device__ __forceinline__ void refit_level_staged(LevelStage& S, const uint32_t* __restrict__ children,
+                                                   const uint32_t* __restrict__ scene_of,
+                                                   const uint32_t* __restrict__ parents_internal,
+                                                   const uint32_t* __restrict__ counters,
+                                                   const uint32_t* __restrict__ slots, const uint32_t* list_start,
+                                                   float4* __restrict__ nodes, float* __restrict__ root_box,
+                                                   const uint32_t* list, uint32_t count, uint32_t level) {
+    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
+        const uint32_t s = list[idx];
+        stage_boxes(S, idx, s, nodes);
+        stage_links(S, idx, s, node_links(s, children, parents_internal, counters, slots, list_start, level));
+    }
+    __syncthreads();
+    climb_staged(S, children, scene_of, nodes, root_box, count);
+}
+
+// The lists of one level, concatenated in LDS-held order: list k starts at
+// start[k] (exclusive prefix of the counts, one thread per list).
+__device__ __forceinline__ uint32_t concat_lists(const uint32_t* __restrict__ counts, uint32_t nlists,
+                                                 const uint32_t* __restrict__ lists, uint32_t* start,
+                                                 uint32_t* out, uint32_t* status) {
+    __shared__ uint32_t s_carry;
+    __shared__ uint32_t s_w[16];
+    if (threadIdx.x == 0) s_carry = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t b0 = 0; b0 < nlists; b0 += blockDim.x) {
+        const uint32_t k = b0 + threadIdx.x;
+        uint32_t c = 0;
+        if (k < nlists) {
+            c = counts[k];
+            if (c > kListCap) {  // counted past a full list (flagged by the append)
+                raise_status(status, PT_STATUS_REFIT_OVERFLOW);
+                c = kListCap;
+            }
+        }
+        uint32_t x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[wave] = x;
+        __syncthreads();
+        uint32_t pre = s_carry;
+        for (int w = 0; w < wave; ++w) pre += s_w[w];
+        if (k < nlists) start[k] = pre + x - c;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) s_carry = pre + x;
+        __syncthreads();
+    }
+    const uint32_t total = s_carry;
+    if (threadIdx.x == 0) start[nlists] = total;
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < nlists * kListCap; e += blockDim.x) {
+        const uint32_t k = e / kListCap, j = e % kListCap;
+        if (j < start[k + 1] - start[k]) out[start[k] + j] = lists[static_cast<size_t>(k) * kListCap + j];
+    }
+    __syncthreads();
+    return total;
+}
+
+// Level-1 links prepared by k_refit_local's last workgroups, per 65536-leaf
+// block: [0] the block's list length, then per listed node the node, its
+// child word, parent, parent's slot and same-level child count (kStageCap
+// each).
+constexpr uint32_t kMidLinksWords = 1 + 5 * kStageCap;
+static_assert(sizeof(LdsBoxes) >= 4 * kMidCap, "level-1 list staging fits in a box array");
+
 // Refit, phase 1 (compute_internal_node_bbox_kernel, BVHBuilder.cu:54-91):
 // one thread per sorted leaf, one workgroup per block of kRefitBlock leaves.
 // The leaf writes its sorted triangle record (the reference instead permutes
@@ -547,14 +758,19 @@ struct LdsBoxes {
 // stored into that parent's record.  A node inside the block has its index
 // inside its leaf range, so the block's nodes are the indices [base, base +
 // kRefitBlock): their child words, parents and levels are loaded into LDS
-// once, coalesced, and the climb reads no global memory.
+// once, coalesced, and the climb reads no global memory.  The launch's last
+// mid_blocks workgroups instead concatenate each 65536-leaf block's level-1
+// lists and stage their links (children, parents, counters and slots, none
+// of which this launch writes) into mid_links for k_refit_mid.
 __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     const pt_triangle* __restrict__ in_tris, const uint32_t* __restrict__ perm,
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ scene_of,
     const uint32_t* __restrict__ children, uint32_t* __restrict__ parents_leaf,
     const uint32_t* __restrict__ parents_internal, const uint32_t* __restrict__ counters,
     float4* __restrict__ nodes, float4* __restrict__ tri_recs, float* __restrict__ root_box,
-    uint32_t n) {
+    uint32_t n, const uint32_t* __restrict__ span_list, const uint32_t* __restrict__ span_count,
+    const uint32_t* __restrict__ slots, uint32_t stage_max, uint32_t* __restrict__ mid_links,
+    uint32_t* __restrict__ status) {
     __shared__ uint32_t s_cnt[kRefitBlock];
     __shared__ uint32_t s_word[kRefitBlock];   // child word of node base + j
     __shared__ uint32_t s_par[kRefitBlock];    // its parent
@@ -562,6 +778,29 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     __shared__ LdsBoxes s_leaf;
     __shared__ LdsBoxes s_node;
     const int tid = threadIdx.x;
+    const uint32_t n_lists1 = (n + kRefitBlock - 1) / kRefitBlock;
+    if (blockIdx.x >= n_lists1) {  // level-1 link staging for 65536-leaf block m
+        constexpr uint32_t kLists = kMidBlock / kRefitBlock;
+        const uint32_t m = blockIdx.x - n_lists1;
+        uint32_t* s_list = reinterpret_cast<uint32_t*>(&s_leaf);   // kMidCap words
+        uint32_t* s_start = reinterpret_cast<uint32_t*>(&s_node);  // kLists + 1 words
+        const uint32_t k0 = m * kLists;
+        const uint32_t count = concat_lists(span_count + k0, min(kLists, n_lists1 - k0),
+                                            span_list + static_cast<size_t>(k0) * kListCap, s_start, s_list, status);
+        uint32_t* out = mid_links + static_cast<size_t>(m) * kMidLinksWords;
+        if (tid == 0) out[0] = count;
+        if (count > stage_max) return;
+        for (uint32_t idx = tid; idx < count; idx += kRefitBlock) {
+            const uint32_t sn = s_list[idx];
+            const Links L = node_links(sn, children, parents_internal, counters, slots, s_start, 1u);
+            out[1 + idx] = sn;
+            out[1 + kStageCap + idx] = L.word;
+            out[1 + 2 * kStageCap + idx] = L.par;
+            out[1 + 3 * kStageCap + idx] = L.up;
+            out[1 + 4 * kStageCap + idx] = L.wait;
+        }
+        return;
+    }
     const uint32_t base = blockIdx.x * kRefitBlock;
     const uint32_t k = base + tid;
     s_cnt[tid] = 0u;
@@ -629,201 +868,6 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_local(
     }
 }
 
-// Refit of the level-L nodes listed in `list` (count at *count), all inside
-// one workgroup so the counter hand-off needs only workgroup-scope ordering.
-// Every child box of a node sits in the node's own record: lower-level
-// children were stored by earlier phases, same-level children are stored here
-// before their parent's counter is incremented.  Each lower-level child of a
-// listed node is one "arrival"; the thread completing a node climbs on while
-// the parent is of the same level.
-__device__ __forceinline__ void refit_level(const uint32_t* __restrict__ children,
-                                            const uint32_t* __restrict__ scene_of,
-                                            const uint32_t* __restrict__ parents_internal,
-                                            uint32_t* __restrict__ counters,
-                                            float4* __restrict__ nodes,
-                                            float* __restrict__ root_box,
-                                            const uint32_t* list, uint32_t count, uint32_t level) {
-    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
-        const uint32_t s = list[idx];
-        const uint32_t w = children[s];
-        const uint32_t l = w >> 2, r = l + 1;
-        int arrivals = 0;
-        if (!(w & 1u) || (counters[l] >> kLevelShift) < level) ++arrivals;
-        if (!(w & 2u) || (counters[r] >> kLevelShift) < level) ++arrivals;
-        for (int a = 0; a < arrivals; ++a) {
-            uint32_t p = s;
-            for (;;) {
-                const uint32_t old = __hip_atomic_fetch_add(counters + p, 1u, __ATOMIC_ACQ_REL,
-                                                            __HIP_MEMORY_SCOPE_WORKGROUP);
-                if ((old & kCountMask) == 0u) break;
-                const float4* rec = nodes + 4 * static_cast<size_t>(p);
-                const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-                const Box lb{r0.x, r0.y, r0.z, r0.w, r1.x, r1.y};
-                const Box rb{r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-                const Box b = unite(lb, rb);
-                nodes[4 * static_cast<size_t>(p) + 3] = make_float4(__uint_as_float(children[p]), 0.0f, 0.0f, 0.0f);
-                const uint32_t pp = parents_internal[p];
-                if (pp == kInvalid) {
-                    store_root(root_box + 6 * static_cast<size_t>(scene_of[p]), b);
-                    break;
-                }
-                store_slot(nodes, pp, (children[pp] >> 2) == p, b);
-                if ((counters[pp] >> kLevelShift) != level) break;
-                p = pp;
-            }
-        }
-    }
-}
-
-// The same refit with the list staged in LDS (lists of up to kStageCap
-// nodes): every listed node's record (the child boxes the earlier phases
-// stored), its parent and its count of same-level children are gathered
-// first, so the climb itself touches only LDS.  Each completed node's
-// record is written whole; a box whose parent belongs to a later phase also
-// goes into that parent's record.
-constexpr uint32_t kStageCap = 1024;
-constexpr uint32_t kUpOutside = 0xffffffffu;  // parent handled by a later phase, or none
-
-struct LevelStage {
-    float box[12][kStageCap];  // left child box, then right, in record order
-    uint32_t node[kStageCap];  // the listed node
-    uint32_t word[kStageCap];  // its child word
-    uint32_t par[kStageCap];   // its parent (kInvalid at a scene root)
-    uint32_t up[kStageCap];    // parent's slot << 1 | right side, or kUpOutside
-    uint32_t wait[kStageCap];  // same-level children not yet complete
-};
-
-// A listed node's links: its child word, parent, the parent's slot in the
-// same list (or kUpOutside) and its number of same-level children.
-__device__ __forceinline__ void stage_links(LevelStage& S, uint32_t idx, uint32_t s,
-                                            const uint32_t* __restrict__ children,
-                                            const uint32_t* __restrict__ parents_internal,
-                                            const uint32_t* __restrict__ counters,
-                                            const uint32_t* __restrict__ slots, const uint32_t* list_start,
-                                            uint32_t level) {
-    const uint32_t w = children[s];
-    const uint32_t l = w >> 2, r = l + 1;
-    S.wait[idx] = static_cast<uint32_t>((w & 1u) && (counters[l] >> kLevelShift) == level) +
-                  static_cast<uint32_t>((w & 2u) && (counters[r] >> kLevelShift) == level);
-    S.node[idx] = s;
-    S.word[idx] = w;
-    const uint32_t pp = parents_internal[s];
-    S.par[idx] = pp;
-    uint32_t up = kUpOutside;
-    if (pp != kInvalid && (counters[pp] >> kLevelShift) == level) {
-        const uint32_t sl = slots[pp];
-        up = ((list_start[sl >> 6] + (sl & 63u)) << 1) | static_cast<uint32_t>((children[pp] >> 2) != s);
-    }
-    S.up[idx] = up;
-}
-
-// A listed node's two child boxes, from its record (stored by earlier phases).
-__device__ __forceinline__ void stage_boxes(LevelStage& S, uint32_t idx, uint32_t s,
-                                            const float4* __restrict__ nodes) {
-    const float4* rec = nodes + 4 * static_cast<size_t>(s);
-    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
-    const float v[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-    for (int k = 0; k < 12; ++k) S.box[k][idx] = v[k];
-}
-
-// The climb over a staged list (after a barrier): every node with no
-// same-level child starts one; the thread completing a node's second
-// same-level child goes on to the node.
-__device__ __forceinline__ void climb_staged(LevelStage& S, const uint32_t* __restrict__ children,
-                                             const uint32_t* __restrict__ scene_of, float4* __restrict__ nodes,
-                                             float* __restrict__ root_box, uint32_t count) {
-    for (uint32_t first = threadIdx.x; first < count; first += blockDim.x) {
-        if (S.wait[first] != 0u) continue;
-        uint32_t idx = first;
-        for (;;) {
-            const Box lb{S.box[0][idx], S.box[1][idx], S.box[2][idx], S.box[3][idx], S.box[4][idx], S.box[5][idx]};
-            const Box rb{S.box[6][idx], S.box[7][idx], S.box[8][idx], S.box[9][idx], S.box[10][idx], S.box[11][idx]};
-            const Box b = unite(lb, rb);
-            const uint32_t s = S.node[idx];
-            store_record(nodes, s, lb, rb, S.word[idx]);
-            const uint32_t pp = S.par[idx];
-            if (pp == kInvalid) {
-                store_root(root_box + 6 * static_cast<size_t>(scene_of[s]), b);
-                break;
-            }
-            const uint32_t up = S.up[idx];
-            if (up == kUpOutside) {
-                store_slot(nodes, pp, (children[pp] >> 2) == s, b);
-                break;
-            }
-            const uint32_t ps = up >> 1;
-            const int o = (up & 1u) ? 6 : 0;
-            S.box[o + 0][ps] = b.mnx; S.box[o + 1][ps] = b.mxx; S.box[o + 2][ps] = b.mny;
-            S.box[o + 3][ps] = b.mxy; S.box[o + 4][ps] = b.mnz; S.box[o + 5][ps] = b.mxz;
-            const uint32_t old = __hip_atomic_fetch_sub(&S.wait[ps], 1u, __ATOMIC_ACQ_REL,
-                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old != 1u) break;  // the other same-level child completes the parent
-            idx = ps;
-        }
-    }
-}
-
-__device__ __forceinline__ void refit_level_staged(LevelStage& S, const uint32_t* __restrict__ children,
-                                                   const uint32_t* __restrict__ scene_of,
-                                                   const uint32_t* __restrict__ parents_internal,
-                                                   const uint32_t* __restrict__ counters,
-                                                   const uint32_t* __restrict__ slots, const uint32_t* list_start,
-                                                   float4* __restrict__ nodes, float* __restrict__ root_box,
-                                                   const uint32_t* list, uint32_t count, uint32_t level) {
-    for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
-        const uint32_t s = list[idx];
-        stage_boxes(S, idx, s, nodes);
-        stage_links(S, idx, s, children, parents_internal, counters, slots, list_start, level);
-    }
-    __syncthreads();
-    climb_staged(S, children, scene_of, nodes, root_box, count);
-}
-
-// The lists of one level, concatenated in LDS-held order: list k starts at
-// start[k] (exclusive prefix of the counts, one thread per list).
-__device__ __forceinline__ uint32_t concat_lists(const uint32_t* __restrict__ counts, uint32_t nlists,
-                                                 const uint32_t* __restrict__ lists, uint32_t* start,
-                                                 uint32_t* out, uint32_t* status) {
-    __shared__ uint32_t s_carry;
-    __shared__ uint32_t s_w[16];
-    if (threadIdx.x == 0) s_carry = 0u;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t b0 = 0; b0 < nlists; b0 += blockDim.x) {
-        const uint32_t k = b0 + threadIdx.x;
-        uint32_t c = 0;
-        if (k < nlists) {
-            c = counts[k];
-            if (c > kListCap) {  // counted past a full list (flagged by the append)
-                raise_status(status, PT_STATUS_REFIT_OVERFLOW);
-                c = kListCap;
-            }
-        }
-        uint32_t x = c;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) s_w[wave] = x;
-        __syncthreads();
-        uint32_t pre = s_carry;
-        for (int w = 0; w < wave; ++w) pre += s_w[w];
-        if (k < nlists) start[k] = pre + x - c;
-        __syncthreads();
-        if (threadIdx.x == blockDim.x - 1) s_carry = pre + x;
-        __syncthreads();
-    }
-    const uint32_t total = s_carry;
-    if (threadIdx.x == 0) start[nlists] = total;
-    __syncthreads();
-    for (uint32_t e = threadIdx.x; e < nlists * kListCap; e += blockDim.x) {
-        const uint32_t k = e / kListCap, j = e % kListCap;
-        if (j < start[k + 1] - start[k]) out[start[k] + j] = lists[static_cast<size_t>(k) * kListCap + j];
-    }
-    __syncthreads();
-    return total;
-}
-
 // The level-2 stage prepared by k_refit_mid's last workgroup for k_refit_top:
 // [0] the list's length, then per listed node its child word, parent,
 // parent's slot and same-level child count (kStageCap each).
@@ -844,6 +888,7 @@ __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ span_count, uint32_t n_lists1,
                                                     const uint32_t* __restrict__ span_top, uint32_t n_lists2,
                                                     uint32_t* __restrict__ top_list, uint32_t* __restrict__ top_links,
+                                                    const uint32_t* __restrict__ mid_links,
                                                     const uint32_t* __restrict__ slots, uint32_t stage_max,
                                                     uint32_t* __restrict__ status) {
     constexpr uint32_t kLists = kMidBlock / kRefitBlock;
@@ -856,12 +901,28 @@ __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__
         if (threadIdx.x == 0) top_links[0] = count;
         if (count > stage_max) return;
         for (uint32_t idx = threadIdx.x; idx < count; idx += blockDim.x) {
-            stage_links(S, idx, top_list[idx], children, parents_internal, counters, slots, s_start, 2u);
-            top_links[1 + idx] = S.word[idx];
-            top_links[1 + kStageCap + idx] = S.par[idx];
-            top_links[1 + 2 * kStageCap + idx] = S.up[idx];
-            top_links[1 + 3 * kStageCap + idx] = S.wait[idx];
+            const Links L = node_links(top_list[idx], children, parents_internal, counters, slots, s_start, 2u);
+            top_links[1 + idx] = L.word;
+            top_links[1 + kStageCap + idx] = L.par;
+            top_links[1 + 2 * kStageCap + idx] = L.up;
+            top_links[1 + 3 * kStageCap + idx] = L.wait;
         }
+        return;
+    }
+    const uint32_t* links = mid_links + static_cast<size_t>(blockIdx.x) * kMidLinksWords;
+    const uint32_t staged = links[0];
+    if (staged <= stage_max) {
+        for (uint32_t idx = threadIdx.x; idx < staged; idx += blockDim.x) {
+            const uint32_t sn = links[1 + idx];
+            S.node[idx] = sn;
+            S.word[idx] = links[1 + kStageCap + idx];
+            S.par[idx] = links[1 + 2 * kStageCap + idx];
+            S.up[idx] = links[1 + 3 * kStageCap + idx];
+            S.wait[idx] = links[1 + 4 * kStageCap + idx];
+            stage_boxes(S, idx, sn, nodes);
+        }
+        __syncthreads();
+        climb_staged(S, children, scene_of, nodes, root_box, staged);
         return;
     }
     uint32_t* s_list = s_buf;
@@ -869,12 +930,7 @@ __global__ __launch_bounds__(1024) void k_refit_mid(const uint32_t* __restrict__
     const uint32_t k0 = blockIdx.x * kLists;
     const uint32_t count = concat_lists(span_count + k0, min(kLists, n_lists1 - k0),
                                         span_list + static_cast<size_t>(k0) * kListCap, s_start, s_list, status);
-    if (count <= stage_max) {
-        refit_level_staged(S, children, scene_of, parents_internal, counters, slots, s_start, nodes, root_box, s_list,
-                           count, 1u);
-    } else {
-        refit_level(children, scene_of, parents_internal, counters, nodes, root_box, s_list, count, 1u);
-    }
+    refit_level(children, scene_of, parents_internal, counters, nodes, root_box, s_list, count, 1u);
 }
 
 // Level-2 nodes: one workgroup, over the list and links k_refit_mid prepared.
@@ -985,6 +1041,7 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
     b.span_top.reserve(static_cast<size_t>(mid_blocks) * kListCap * 4 * 2);  // lists, then their concatenation
     b.span_count.reserve((static_cast<size_t>(n_lists1) + mid_blocks) * 4);
     b.top_links.reserve(kTopLinksWords * 4);
+    b.mid_links.reserve(static_cast<size_t>(mid_blocks) * kMidLinksWords * 4);
     b.slots.reserve(n4);
 
     uint32_t* kin = b.keys.as<uint32_t>();
@@ -1044,12 +1101,15 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
         PT_HIP(hipGetLastError());
     }
     const uint32_t stage_max = build_tuning().staged ? kStageCap : 0u;
-    hipLaunchKernelGGL(k_refit_local, dim3(div_up(n, kRefitBlock)), dim3(kRefitBlock), 0, st,
+    // With level-1 nodes present, mid_blocks more workgroups stage their links.
+    const uint32_t mid_prep = n > static_cast<uint32_t>(kRefitBlock) ? mid_blocks : 0u;
+    hipLaunchKernelGGL(k_refit_local, dim3(n_lists1 + mid_prep), dim3(kRefitBlock), 0, st,
                        s.tris.as<pt_triangle>(), b.vals.as<uint32_t>(), s.off.as<uint32_t>(),
                        b.scene_of.as<uint32_t>(), b.children.as<uint32_t>(),
                        b.parents_leaf.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
                        b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.tris.as<float4>(),
-                       b.root_box.as<float>(), n);
+                       b.root_box.as<float>(), n, b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>(),
+                       b.slots.as<uint32_t>(), stage_max, b.mid_links.as<uint32_t>(), s.status.as<uint32_t>());
     if (n > static_cast<uint32_t>(kRefitBlock)) {
         // One more workgroup prepares the level-2 refit when there is one.
         const uint32_t top = n > kMidBlock ? 1u : 0u;
@@ -1059,7 +1119,7 @@ void build_lbvh(DeviceScene& s, hipStream_t st) {
                            b.counters.as<uint32_t>(), b.nodes.as<float4>(), b.root_box.as<float>(),
                            b.span_list.as<uint32_t>(), b.span_count.as<uint32_t>(), n_lists1,
                            b.span_top.as<uint32_t>(), mid_blocks, top_list, b.top_links.as<uint32_t>(),
-                           b.slots.as<uint32_t>(), stage_max, s.status.as<uint32_t>());
+                           b.mid_links.as<uint32_t>(), b.slots.as<uint32_t>(), stage_max, s.status.as<uint32_t>());
         if (top) {
             hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, b.children.as<uint32_t>(),
                                b.scene_of.as<uint32_t>(), b.parents_internal.as<uint32_t>(),
