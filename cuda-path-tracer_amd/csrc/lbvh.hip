@@ -194,10 +194,18 @@ __global__ __launch_bounds__(kScanWaves * 64) void k_sort_rowscan(uint32_t* __re
     const uint32_t per = (num_tiles + kScanWaves - 1) / kScanWaves;
     const uint32_t t0 = min(num_tiles, wave * per), t1 = min(num_tiles, t0 + per);
     // Loads in batches of kScanBatch, issued together (the counts come from
-    // other XCDs' writes: each load is a far round trip).
+    // other XCDs' writes: each load is a far round trip).  The first batch
+    // stays in registers for the second sweep (up to 256 tiles, 2^20 keys,
+    // that is all of it).
     constexpr uint32_t kScanBatch = 16;
+    uint32_t v0[kScanBatch];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanBatch; ++k)
+        v0[k] = t0 + k < t1 ? tile_hist[static_cast<size_t>(t0 + k) * kRadix + d] : 0u;
     uint32_t sum = 0;
-    for (uint32_t t = t0; t < t1; t += kScanBatch) {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanBatch; ++k) sum += v0[k];
+    for (uint32_t t = t0 + kScanBatch; t < t1; t += kScanBatch) {
         uint32_t v[kScanBatch];
 #pragma unroll
         for (uint32_t k = 0; k < kScanBatch; ++k)
@@ -209,7 +217,12 @@ __global__ __launch_bounds__(kScanWaves * 64) void k_sort_rowscan(uint32_t* __re
     __syncthreads();
     uint32_t run = 0;
     for (int w = 0; w < wave; ++w) run += s_part[w][lane];
-    for (uint32_t t = t0; t < t1; t += kScanBatch) {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanBatch; ++k) {
+        if (t0 + k < t1) tile_hist[static_cast<size_t>(t0 + k) * kRadix + d] = run;
+        run += v0[k];
+    }
+    for (uint32_t t = t0 + kScanBatch; t < t1; t += kScanBatch) {
         uint32_t v[kScanBatch];
 #pragma unroll
         for (uint32_t k = 0; k < kScanBatch; ++k)
