@@ -16,8 +16,10 @@
 //   k_radix_tree    Karras split per internal node, parents, counter reset,
 //                   list of nodes whose leaf range crosses a refit block
 //   k_refit_local   sorted triangle records + bottom-up AABBs inside each
-//                   1024-leaf block (LDS counters/boxes, workgroup scope)
-//   k_refit_mid     nodes inside each 65536-leaf block, one workgroup each
+//                   1024-leaf block (LDS counters/boxes, workgroup scope);
+//                   extra workgroups stage the level-1 nodes' links
+//   k_refit_mid     nodes inside each 65536-leaf block, one workgroup each;
+//                   one more workgroup stages the level-2 nodes' links
 //   k_refit_top     the remaining nodes, one workgroup
 // Each node's traversal record is written whole by the thread that
 // completes the node's box.
