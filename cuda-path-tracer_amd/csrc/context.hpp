@@ -231,9 +231,10 @@ struct Renderer {
     hipEvent_t chain_join[7] = {};
     DevBuf cache_io;           // cache-frame queries and training batch
     // A cache frame traces its training paths, from their primary rays to
-    // their ends, on `side` while the pixel paths' two levels, inference,
-    // resolve and accumulation run on the frame's stream (fork and join
-    // events); created on the first cache frame, default priority.
+    // their ends, on `side` while the pixel paths (two bounce levels, or one
+    // path-persistent launch), inference, resolve and accumulation run on
+    // the frame's stream (fork and join events); created on the first cache
+    // frame, default priority.
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     uint32_t last_train = 0;   // training paths of the last cache frame
